@@ -1,0 +1,448 @@
+"""MI355X-native PAGANI adaptive cubature -- Python binding of libpagani.so.
+
+The product is the HIP library behind the C ABI in include/pagani.h and the
+C++ drop-in headers in include/pagani/.  This module binds the C ABI with
+ctypes so tests and benchmarks can drive it; the phase-level calls take torch
+device tensors (torch only provides device memory here).  Loading fails
+loudly when the library has not been built: there is no CPU fallback.
+"""
+import ctypes
+import os
+from dataclasses import dataclass
+
+__all__ = [
+    "load", "Workspace", "IntegrationResult", "FAMILY", "FP64", "FP32",
+    "PaganiError", "LIB_PATH",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpagani.so")
+
+MIN_DIM, MAX_DIM = 2, 8
+NSETS, NRULES, MAX_SLICES = 9, 5, 8
+FP64, FP32 = 64, 32
+OK, NOT_CONVERGED, EBADARG, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, -1, -2, -3, -4
+
+# Genz family codes (include/pagani.h)
+FAMILY = {
+    "oscillatory": 1,
+    "product_peak": 2,
+    "corner_peak": 3,
+    "gaussian": 4,
+    "c0": 5,
+    "discontinuous": 6,
+}
+
+
+class PaganiError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pagani error {code}: {msg}")
+        self.code = code
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [
+        ("estimate", ctypes.c_double),
+        ("errorest", ctypes.c_double),
+        ("neval", ctypes.c_size_t),
+        ("nregions", ctypes.c_size_t),
+        ("nFinishedRegions", ctypes.c_size_t),
+        ("status", ctypes.c_int),
+        ("lastPhase", ctypes.c_int),
+        ("chi_sq", ctypes.c_double),
+        ("iters", ctypes.c_size_t),
+    ]
+
+
+class _Genz(ctypes.Structure):
+    _fields_ = [
+        ("family", ctypes.c_int),
+        ("ndim", ctypes.c_int),
+        ("a", ctypes.c_double * MAX_DIM),
+        ("u", ctypes.c_double * MAX_DIM),
+    ]
+
+
+class EvalConsts(ctypes.Structure):
+    _fields_ = [
+        ("w", (ctypes.c_double * NRULES) * NSETS),
+        ("scale", (ctypes.c_double * NRULES) * NSETS),
+        ("norm", (ctypes.c_double * NRULES) * NSETS),
+        ("lows", ctypes.c_double * MAX_DIM),
+        ("range", ctypes.c_double * MAX_DIM),
+        ("jac", ctypes.c_double),
+        ("lambda1", ctypes.c_double),
+        ("lambda2", ctypes.c_double),
+        ("ratio", ctypes.c_double),
+        ("gtab", ctypes.c_void_p),
+        ("ndim", ctypes.c_int),
+        ("feval", ctypes.c_int),
+        ("set_begin", ctypes.c_int * (NSETS + 1)),
+        ("slices", ctypes.c_int),
+        ("slice_begin", ctypes.c_int * (MAX_SLICES + 1)),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("eval_ms", ctypes.c_double),
+        ("refine_ms", ctypes.c_double),
+        ("finalize_ms", ctypes.c_double),
+        ("compact_ms", ctypes.c_double),
+        ("classify_ms", ctypes.c_double),
+        ("wall_ms", ctypes.c_double),
+        ("regions_evaluated", ctypes.c_uint64),
+        ("eval_launches", ctypes.c_uint64),
+        ("iterations", ctypes.c_uint64),
+        ("classify_passes", ctypes.c_uint64),
+        ("max_regions", ctypes.c_uint64),
+        ("integrals", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ClassRes(ctypes.Structure):
+    _fields_ = [
+        ("pass_mem", ctypes.c_int),
+        ("pass_errorest_budget", ctypes.c_int),
+        ("threshold", ctypes.c_double),
+        ("percent_mem_active", ctypes.c_double),
+        ("range_low", ctypes.c_double),
+        ("range_high", ctypes.c_double),
+        ("num_active", ctypes.c_size_t),
+        ("finished_errorest", ctypes.c_double),
+        ("max_budget_perc_to_cover", ctypes.c_double),
+        ("max_active_perc", ctypes.c_double),
+        ("threshold_passes", ctypes.c_int),
+    ]
+
+
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pagani_rule_tables": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp,
+                                          ctypes.POINTER(ctypes.c_int), _dp]),
+    "pagani_ws_create": (_vp, [ctypes.c_int, ctypes.c_int]),
+    "pagani_create_error": (ctypes.c_char_p, []),
+    "pagani_ws_destroy": (None, [_vp]),
+    "pagani_ws_last_error": (ctypes.c_char_p, [_vp]),
+    "pagani_ws_ndim": (ctypes.c_int, [_vp]),
+    "pagani_ws_set_device_mem": (ctypes.c_int, [_vp, _sz]),
+    "pagani_ws_device_mem": (_sz, [_vp]),
+    "pagani_ws_set_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
+    "pagani_ws_reset_stats": (ctypes.c_int, [_vp]),
+    "pagani_ws_eval_consts": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.POINTER(EvalConsts)]),
+    "pagani_ws_stream": (_vp, [_vp]),
+    "pagani_ws_trace": (_sz, [_vp, _sz, _vp, _vp, _vp, _vp]),
+    "pagani_ws_live_regions": (ctypes.c_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_vp),
+                                              ctypes.POINTER(_vp)]),
+    "pagani_integrate_genz": (ctypes.c_int, [_vp, ctypes.POINTER(_Genz), ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                             ctypes.c_int, _sz, ctypes.POINTER(_Result)]),
+    "pagani_integrate_cb": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                           _dp, _dp, ctypes.c_int, _vp, _sz,
+                                           ctypes.POINTER(_Result)]),
+    "pagani_uniform_split_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp]),
+    "pagani_eval_genz_dev": (ctypes.c_int, [_vp, ctypes.POINTER(_Genz), ctypes.c_int, _sz,
+                                            _vp, _vp, _dp, _dp, _vp, _vp, _vp]),
+    "pagani_refine_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                         ctypes.c_int, _vp, _dp, ctypes.POINTER(_sz)]),
+    "pagani_filter_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _vp, _vp, _vp, ctypes.POINTER(_sz)]),
+    "pagani_split_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),
+    "pagani_filter_split_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, _vp, ctypes.POINTER(_sz)]),
+    "pagani_sum_dev": (ctypes.c_int, [_vp, _vp, _sz, _dp]),
+    "pagani_dot_dev": (ctypes.c_int, [_vp, _vp, _vp, _sz, _dp]),
+    "pagani_minmax_dev": (ctypes.c_int, [_vp, _vp, _sz, _dp, _dp]),
+    "pagani_scan_flags_dev": (ctypes.c_int, [_vp, _vp, _sz, _vp, ctypes.POINTER(_sz)]),
+    "pagani_classify_dev": (ctypes.c_int, [_vp, _vp, _sz, ctypes.c_double, _dp,
+                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                           _vp, ctypes.POINTER(ClassRes)]),
+    "do_integration_from_c": (ctypes.c_int, [_dp]),
+}
+
+_LIB = None
+
+
+def load():
+    """Load libpagani.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libpagani.so not built at {LIB_PATH}; run __graft_entry__.build() "
+            "(the HIP library is required, there is no CPU fallback)")
+    # One HIP runtime per process: when torch is present it brings its own
+    # libamdhip64; load it first so libpagani.so binds to the same runtime
+    # (two runtimes in one process cannot both initialise the device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def rule_tables(ndim):
+    """Host-side rule tables of the library: (w, scale, norm, lambda, set_begin, gtab)."""
+    import numpy as np
+    lib = load()
+    w = np.zeros(45)
+    sc = np.zeros(45)
+    nm = np.zeros(45)
+    lam = np.zeros(5)
+    sb = (ctypes.c_int * (NSETS + 1))()
+    F = lib.pagani_rule_tables(int(ndim), None, None, None, None, None, None)
+    if F < 0:
+        raise PaganiError(F, f"unsupported ndim {ndim}")
+    g = np.zeros((F - 1 - 4 * ndim) * ndim)
+    lib.pagani_rule_tables(int(ndim), w.ctypes.data_as(_dp), sc.ctypes.data_as(_dp),
+                           nm.ctypes.data_as(_dp), lam.ctypes.data_as(_dp), sb,
+                           g.ctypes.data_as(_dp))
+    return (w.reshape(9, 5), sc.reshape(9, 5), nm.reshape(9, 5), lam, list(sb),
+            g.reshape(-1, ndim))
+
+
+@dataclass
+class IntegrationResult:
+    """numint::integration_result (common/integration_result.hh:11-21)."""
+    estimate: float
+    errorest: float
+    neval: int
+    nregions: int
+    nFinishedRegions: int
+    status: int
+    lastPhase: int
+    chi_sq: float
+    iters: int
+
+    @classmethod
+    def _from(cls, r):
+        return cls(r.estimate, r.errorest, r.neval, r.nregions, r.nFinishedRegions, r.status,
+                   r.lastPhase, r.chi_sq, r.iters)
+
+
+def _arr(vals, n=None):
+    if vals is None:
+        return None
+    vals = [float(v) for v in vals]
+    return (ctypes.c_double * len(vals))(*vals)
+
+
+def genz(family, a, u=None):
+    fam = FAMILY[family] if isinstance(family, str) else int(family)
+    a = [float(v) for v in a]
+    d = len(a)
+    if not (MIN_DIM <= d <= MAX_DIM):
+        raise ValueError(f"ndim {d} outside [{MIN_DIM}, {MAX_DIM}]")
+    u = [0.0] * d if u is None else [float(v) for v in u]
+    g = _Genz()
+    g.family = fam
+    g.ndim = d
+    for i in range(d):
+        g.a[i] = a[i]
+        g.u[i] = u[i]
+    return g
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Workspace:
+    """Owns a pagani_ws: rule tables, region pool and stream for one ndim."""
+
+    def __init__(self, ndim, device=-1):
+        self.lib = load()
+        self.ndim = int(ndim)
+        self._ws = self.lib.pagani_ws_create(self.ndim, int(device))
+        if not self._ws:
+            why = self.lib.pagani_create_error()
+            raise PaganiError(EHIP, f"pagani_ws_create({ndim}) failed: "
+                                    f"{why.decode() if why else 'unknown'}")
+
+    def close(self):
+        if getattr(self, "_ws", None):
+            self.lib.pagani_ws_destroy(self._ws)
+            self._ws = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- error handling --------------------------------------------------------
+    def _check(self, rc):
+        if rc < 0:
+            msg = self.lib.pagani_ws_last_error(self._ws)
+            raise PaganiError(rc, msg.decode() if msg else "")
+        return rc
+
+    # -- configuration -------------------------------------------------------
+    def set_device_mem(self, nbytes):
+        self._check(self.lib.pagani_ws_set_device_mem(self._ws, int(nbytes)))
+
+    def device_mem(self):
+        return self.lib.pagani_ws_device_mem(self._ws)
+
+    def set_slices(self, s):
+        self._check(self.lib.pagani_ws_set_slices(self._ws, int(s)))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.pagani_ws_set_timing(self._ws, 1 if on else 0))
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.pagani_ws_get_stats(self._ws, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self.lib.pagani_ws_reset_stats(self._ws))
+
+    def eval_consts(self, n=1 << 20, lows=None, highs=None):
+        c = EvalConsts()
+        self._check(self.lib.pagani_ws_eval_consts(self._ws, _arr(lows), _arr(highs), int(n),
+                                                   ctypes.byref(c)))
+        return c
+
+    def trace(self, cap=1000):
+        import numpy as np
+        n = np.zeros(cap, dtype=np.uint64)
+        m = np.zeros(cap, dtype=np.uint64)
+        e = np.zeros(cap)
+        r = np.zeros(cap)
+        k = self.lib.pagani_ws_trace(self._ws, cap, n.ctypes.data, m.ctypes.data, e.ctypes.data,
+                                     r.ctypes.data)
+        k = min(k, cap)
+        return dict(n=n[:k].astype(np.int64), m=m[:k].astype(np.int64), estimate=e[:k],
+                    errorest=r[:k])
+
+    # -- integration ---------------------------------------------------------
+    def integrate_genz(self, family, a, u=None, epsrel=1e-3, epsabs=1e-12, lows=None,
+                       highs=None, precision=FP64, relerr=True, init_k=0):
+        g = genz(family, a, u)
+        if g.ndim != self.ndim:
+            raise ValueError("integrand ndim != workspace ndim")
+        res = _Result()
+        rc = self.lib.pagani_integrate_genz(self._ws, ctypes.byref(g), int(precision),
+                                            float(epsrel), float(epsabs), _arr(lows),
+                                            _arr(highs), 1 if relerr else 0, int(init_k),
+                                            ctypes.byref(res))
+        if rc < 0 and rc != ENOMEM:
+            self._check(rc)
+        return IntegrationResult._from(res)
+
+    # -- phases on torch device tensors ----------------------------------------
+    def uniform_split(self, k, L, H):
+        self._check(self.lib.pagani_uniform_split_dev(self._ws, int(k), _ptr(L), _ptr(H)))
+
+    def eval_genz(self, family, a, u, n, L, H, est, err, dim, lows=None, highs=None,
+                  precision=FP64):
+        g = genz(family, a, u)
+        self._check(self.lib.pagani_eval_genz_dev(self._ws, ctypes.byref(g), int(precision),
+                                                  int(n), _ptr(L), _ptr(H), _arr(lows),
+                                                  _arr(highs), _ptr(est), _ptr(err), _ptr(dim)))
+
+    def refine(self, n, est, err_in, err_out, parent_est, epsrel, relerr, active):
+        sums = (ctypes.c_double * 4)()
+        na = ctypes.c_size_t(0)
+        self._check(self.lib.pagani_refine_dev(self._ws, int(n), _ptr(est), _ptr(err_in),
+                                               _ptr(err_out), _ptr(parent_est), float(epsrel),
+                                               1 if relerr else 0, _ptr(active), sums,
+                                               ctypes.byref(na)))
+        return list(sums), na.value
+
+    def filter(self, n, L, H, active, est, err, dim, L2, H2, pest, perr, dim2):
+        m = ctypes.c_size_t(0)
+        self._check(self.lib.pagani_filter_dev(self._ws, int(n), _ptr(L), _ptr(H), _ptr(active),
+                                               _ptr(est), _ptr(err), _ptr(dim), _ptr(L2),
+                                               _ptr(H2), _ptr(pest), _ptr(perr), _ptr(dim2),
+                                               ctypes.byref(m)))
+        return m.value
+
+    def split(self, m, L, H, dim, L2, H2):
+        self._check(self.lib.pagani_split_dev(self._ws, int(m), _ptr(L), _ptr(H), _ptr(dim),
+                                              _ptr(L2), _ptr(H2)))
+
+    def filter_split(self, n, L, H, active, est, err, dim, L2, H2, pest, perr):
+        m = ctypes.c_size_t(0)
+        self._check(self.lib.pagani_filter_split_dev(self._ws, int(n), _ptr(L), _ptr(H),
+                                                     _ptr(active), _ptr(est), _ptr(err),
+                                                     _ptr(dim), _ptr(L2), _ptr(H2), _ptr(pest),
+                                                     _ptr(perr), ctypes.byref(m)))
+        return m.value
+
+    def sum(self, x, n=None):
+        out = ctypes.c_double(0)
+        n = x.numel() if n is None else n
+        self._check(self.lib.pagani_sum_dev(self._ws, _ptr(x), int(n), ctypes.byref(out)))
+        return out.value
+
+    def dot(self, flags, x, n=None):
+        out = ctypes.c_double(0)
+        n = x.numel() if n is None else n
+        self._check(self.lib.pagani_dot_dev(self._ws, _ptr(flags), _ptr(x), int(n),
+                                            ctypes.byref(out)))
+        return out.value
+
+    def minmax(self, x, n=None):
+        mn, mx = ctypes.c_double(0), ctypes.c_double(0)
+        n = x.numel() if n is None else n
+        self._check(self.lib.pagani_minmax_dev(self._ws, _ptr(x), int(n), ctypes.byref(mn),
+                                               ctypes.byref(mx)))
+        return mn.value, mx.value
+
+    def scan_flags(self, flags, offsets, n=None):
+        tot = ctypes.c_size_t(0)
+        n = flags.numel() if n is None else n
+        self._check(self.lib.pagani_scan_flags_dev(self._ws, _ptr(flags), int(n), _ptr(offsets),
+                                                   ctypes.byref(tot)))
+        return tot.value
+
+    def classify(self, err, n, epsrel, est_hist, iter_errorest, iter_finished_errorest,
+                 total_finished_errorest, flags):
+        res = ClassRes()
+        self._check(self.lib.pagani_classify_dev(self._ws, _ptr(err), int(n), float(epsrel),
+                                                 _arr(est_hist), float(iter_errorest),
+                                                 float(iter_finished_errorest),
+                                                 float(total_finished_errorest), _ptr(flags),
+                                                 ctypes.byref(res)))
+        return {k: getattr(res, k) for k, _ in ClassRes._fields_}
+
+
+def do_integration_from_c():
+    """The reference's C interop shim: (status, estimate)."""
+    lib = load()
+    out = ctypes.c_double(0)
+    rc = lib.do_integration_from_c(ctypes.byref(out))
+    return rc, out.value

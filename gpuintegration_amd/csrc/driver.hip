@@ -1,0 +1,1051 @@
+// driver.hip -- the PAGANI iteration loop and the C ABI (include/pagani.h).
+//
+// Host restatement of Workspace::integrate (cuda/pagani/quad/GPUquad/
+// Workspace.cuh:74-494) over the device phases of phases.hip and an
+// evaluation callback.  Per iteration: evaluate -> refine+classify (with the
+// four region sums and the per-chunk active counts) -> one finalize kernel
+// -> ONE device-to-host copy of a 64-byte scalar record -> the host decides
+// (convergence, budget fix, heuristic classification) -> fused filter+split.
+// Buffers are owned by the workspace and grow geometrically; the steady
+// state allocates nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pagani.h"
+#include "pagani_ext.h"
+#include "phases.hpp"
+#include "rule.hpp"
+
+namespace ph = pagani::phases;
+
+namespace {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  std::size_t cap = 0;
+};
+
+struct TraceRec {
+  std::uint64_t n, m;
+  double estimate, errorest;
+};
+
+}  // namespace
+
+struct pagani_ws {
+  int ndim = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  std::size_t hbm_total = 0;
+  std::size_t mem_model_total = 0;  // 0 = hbm_total
+  int num_cus = 256;
+  int slices_override = 0;
+  bool timing = false;
+  pagani_stats stats{};
+  pagani::RuleTables rule;
+  double* d_gtab = nullptr;
+
+  DevBuf<double> L[2], H[2];
+  int cur = 0;
+  DevBuf<double> est, err0, err1, pest, perr;
+  DevBuf<std::uint8_t> dim, active, flags;
+  DevBuf<double> part;
+  DevBuf<std::uint32_t> cnt;
+  DevBuf<std::uint64_t> off;
+  ph::Scalars* d_scal = nullptr;
+  ph::Scalars* h_scal = nullptr;
+  hipEvent_t ev[8] = {};
+  bool compact_pending = false;
+  std::vector<TraceRec> trace;
+  std::size_t live_n = 0;  // regions in L[cur], H[cur] after the last call
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// small utilities
+
+int fail(pagani_ws* ws, int code, const std::string& msg)
+{
+  if (ws) ws->error = msg;
+  return code;
+}
+
+int hip_fail(pagani_ws* ws, hipError_t e, const char* where)
+{
+  return fail(ws, PAGANI_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ws, expr)                                  \
+  do {                                                     \
+    const hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(ws, e_, #expr);  \
+  } while (0)
+
+template <class T>
+int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
+{
+  if (n <= b.cap) return PAGANI_OK;
+  if (b.p) {
+    hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  std::size_t want = std::max(n, b.cap + b.cap / 2);
+  void* p = nullptr;
+  if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    want = n;
+    if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ws, PAGANI_ENOMEM,
+                  std::string("out of device memory allocating ") + what + " (" +
+                      std::to_string(n * sizeof(T)) + " bytes)");
+    }
+  }
+  b.p = static_cast<T*>(p);
+  b.cap = want;
+  return PAGANI_OK;
+}
+
+template <class T>
+void release(DevBuf<T>& b)
+{
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+int ensure_chunks(pagani_ws* ws, std::size_t n)
+{
+  const std::size_t nch = std::max<std::size_t>(ph::num_chunks(n), 1);
+  int rc = ensure(ws, ws->part, nch * ph::kPartDoubles, "chunk sums");
+  if (rc) return rc;
+  rc = ensure(ws, ws->cnt, nch, "chunk counts");
+  if (rc) return rc;
+  return ensure(ws, ws->off, nch, "chunk offsets");
+}
+
+// Finalize the chunk records of an n-region pass and read the scalars back.
+int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+{
+  const std::size_t nch = ph::num_chunks(n);
+  HIP_TRY(ws, ph::finalize(ws->stream, nch, ws->part.p, counts ? ws->cnt.p : nullptr,
+                           counts ? ws->off.p : nullptr, ws->d_scal, minmax ? 1 : 0));
+  HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
+                             hipMemcpyDeviceToHost, ws->stream));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+enum { EV_EVAL0, EV_EVAL1, EV_REF0, EV_REF1, EV_CMP0, EV_CMP1, EV_CLS0, EV_CLS1 };
+
+void record(pagani_ws* ws, int i)
+{
+  if (ws->timing) hipEventRecord(ws->ev[i], ws->stream);
+}
+
+double elapsed(pagani_ws* ws, int a, int b)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ws->ev[a], ws->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0;
+  }
+  return ms;
+}
+
+void harvest_compact(pagani_ws* ws)
+{
+  if (ws->timing && ws->compact_pending) {
+    ws->stats.compact_ms += elapsed(ws, EV_CMP0, EV_CMP1);
+    ws->compact_pending = false;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// evaluation launch data
+
+int choose_slices(const pagani_ws* ws, std::size_t n)
+{
+  if (ws->slices_override > 0) return std::min(ws->slices_override, PAGANI_MAX_SLICES);
+  const std::size_t batches = (n + 63) / 64;
+  const std::size_t target = static_cast<std::size_t>(ws->num_cus) * 4 * 4;
+  const int generic = ws->rule.feval - ws->rule.set_begin[3];
+  int s = 1;
+  while (s < PAGANI_MAX_SLICES && batches * s < target && generic / (2 * s) >= 16) s *= 2;
+  return s;
+}
+
+void make_consts(const pagani_ws* ws, const double* lows, const double* highs, std::size_t n,
+                 pagani_eval_consts& C)
+{
+  std::memset(&C, 0, sizeof(C));
+  const int d = ws->ndim;
+  const pagani::RuleTables& R = ws->rule;
+  std::memcpy(C.w, R.w, sizeof(C.w));
+  std::memcpy(C.scale, R.scale, sizeof(C.scale));
+  std::memcpy(C.norm, R.norm, sizeof(C.norm));
+  double jac = 1.0;
+  for (int j = 0; j < d; ++j) {
+    C.lows[j] = lows ? lows[j] : 0.0;
+    C.range[j] = (highs ? highs[j] : 1.0) - C.lows[j];
+    jac = jac * C.range[j];
+  }
+  C.jac = jac;
+  C.lambda1 = R.lambda[0];
+  C.lambda2 = R.lambda[1];
+  const double q = R.lambda[1] / R.lambda[0];
+  C.ratio = q * q;
+  C.gtab = ws->d_gtab;
+  C.ndim = d;
+  C.feval = R.feval;
+  for (int s = 0; s <= PAGANI_NSETS; ++s) C.set_begin[s] = R.set_begin[s];
+  const int S = choose_slices(ws, n);
+  C.slices = S;
+  const int P0 = R.set_begin[3], F = R.feval, G = F - P0;
+  for (int i = 0; i <= PAGANI_MAX_SLICES; ++i) C.slice_begin[i] = F;
+  C.slice_begin[0] = P0;
+  if (S == 1) {
+    C.slice_begin[1] = F;
+  } else {
+    // wave 0 also evaluates the 1+4d points of sets 0-2
+    const int share0 = std::max(0, F / S - P0);
+    const int rest = G - share0;
+    C.slice_begin[1] = P0 + share0;
+    for (int w = 1; w < S; ++w) {
+      const int sh = rest / (S - 1) + ((w - 1) < rest % (S - 1) ? 1 : 0);
+      C.slice_begin[w + 1] = C.slice_begin[w] + sh;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Heuristic classifier, heuristic_classifier.cuh:89-452 (host logic; the
+// threshold passes run on the device).
+
+std::size_t mem_required(std::size_t n, int d)
+{
+  // num_doubles_needed / num_ints_needed / device_mem_required_for_full_split
+  const std::size_t nd = n * static_cast<std::size_t>(d);
+  const std::size_t dbl = 8 * n + 10 * nd;
+  const std::size_t ints = 4 * n;
+  return 8 * dbl + 4 * ints;
+}
+
+std::size_t free_mem(std::size_t n, int d, std::size_t total)
+{
+  const std::size_t occ = mem_required(n, d);
+  return total > occ ? total - occ : 1;
+}
+
+double mem_ratio(std::size_t n, int d, std::size_t total)
+{
+  return static_cast<double>(mem_required(n, d)) / static_cast<double>(free_mem(n, d, total));
+}
+
+struct Classifier {
+  double epsrel = 0, epsabs = 0;
+  int required_digits = 0;
+  double hist[3] = {0, 0, 0};  // uninitialised in the reference; zeroed here
+  std::size_t iters_collected = 0;
+  double max_budget = .25, max_active = .5;
+  std::size_t total = 0;
+  int d = 0;
+
+  Classifier(double rel, double abs_, std::size_t tot, int ndim)
+      : epsrel(rel), epsabs(abs_), total(tot), d(ndim)
+  {
+    required_digits = static_cast<int>(std::ceil(std::log10(1 / epsrel)));
+  }
+
+  void store(double e)
+  {
+    hist[0] = hist[1];
+    hist[1] = hist[2];
+    hist[2] = e;
+    iters_collected++;
+  }
+
+  // sigDigitsSame, :170-203
+  bool sig_digits_same() const
+  {
+    double third = std::fabs(hist[0]), second = std::fabs(hist[1]), first = std::fabs(hist[2]);
+    while (first != 0. && first < 1.) first *= 10;
+    while (second != 0. && second < 1.) second *= 10;
+    while (third != 0. && third < 1.) third *= 10;
+    char a[512] = {0}, b[512] = {0}, c[512] = {0};
+    std::snprintf(a, sizeof a - 1, "%.15f", third);
+    std::snprintf(b, sizeof b - 1, "%.15f", second);
+    std::snprintf(c, sizeof c - 1, "%.15f", first);
+    bool verdict = true;
+    int sig = 0;
+    for (int i = 0; i < required_digits + 1 && sig < required_digits && verdict && i < 500; ++i) {
+      verdict = c[i] == b[i] && b[i] == a[i];
+      sig += (verdict && c[i] != '.') ? 1 : 0;
+    }
+    return verdict;
+  }
+
+  bool estimate_converged() const
+  {
+    return !(iters_collected - 1 < 1 || !sig_digits_same());
+  }
+
+  bool criteria_met(std::size_t n) const
+  {
+    const double ratio = mem_ratio(n, d, total);
+    if (ratio > 1.) return true;
+    return ratio > .1 && estimate_converged();
+  }
+};
+
+// One threshold pass: flags = err > thr, count and sum(flag*err).
+int threshold_pass(pagani_ws* ws, const double* err, std::size_t n, double thr,
+                   std::uint8_t* flags, std::size_t& count, double& dot)
+{
+  HIP_TRY(ws, ph::threshold(ws->stream, n, err, thr, flags, ws->part.p, ws->cnt.p));
+  const int rc = finalize_read(ws, n, true, false);
+  if (rc) return rc;
+  count = ws->h_scal->count;
+  dot = ws->h_scal->sum[3];
+  ws->stats.classify_passes++;
+  return PAGANI_OK;
+}
+
+// classify, :362-451 (with apply_threshold :282-303,
+// get_larger_threshold_results :327-346, evaluate_error_budget :305-325).
+int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
+             double iter_fin_err, double tot_fin_err, std::uint8_t* flags, pagani_class_res& res)
+{
+  std::memset(&res, 0, sizeof(res));
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::minmax(ws->stream, n, err, ws->part.p));
+  rc = finalize_read(ws, n, false, true);
+  if (rc) return rc;
+  const double mn = ws->h_scal->mn, mx = ws->h_scal->mx;
+  res.range_low = mn;
+  res.range_high = mx;
+  res.threshold = iter_err / static_cast<double>(n);
+  const double target_error = std::fabs(c.hist[2]) * c.epsrel;
+  std::size_t n_inc = 0, n_dec = 0;
+  double last_dot = 0.;
+  auto apply = [&]() -> int {
+    std::size_t cnt = 0;
+    const int r = threshold_pass(ws, err, n, res.threshold, flags, cnt, last_dot);
+    if (r) return r;
+    res.threshold_passes++;
+    res.num_active = cnt;
+    res.percent_mem_active = static_cast<double>(static_cast<int>(cnt)) /
+                             static_cast<double>(static_cast<int>(n));
+    res.pass_mem = res.percent_mem_active <= c.max_active;
+    return PAGANI_OK;
+  };
+  do {
+    if (!res.pass_mem && n_inc <= 20) {
+      res.pass_mem = 0;
+      std::size_t counter = 0;
+      while (!res.pass_mem && counter < 20) {
+        rc = apply();
+        if (rc) return rc;
+        if (!res.pass_mem) {
+          res.range_low = res.threshold;
+          const double diff = std::fabs(res.range_high - res.threshold);
+          res.threshold += diff * .5;
+        }
+        counter++;
+      }
+      n_inc++;
+    }
+    if (res.pass_mem) {
+      const double extra = iter_err - last_dot - iter_fin_err;
+      const double budget = target_error - tot_fin_err;
+      res.pass_errorest_budget = extra <= c.max_budget * budget;
+      res.finished_errorest = extra;
+      if (!res.pass_errorest_budget && n_dec <= 20) {
+        res.range_high = res.threshold;
+        const double diff = std::fabs(res.range_low - res.threshold);
+        res.threshold -= diff * .5;
+        res.pass_mem = 0;
+        n_dec++;
+      }
+    }
+    const bool exhausted = n_dec >= 20 || n_inc >= 20;
+    if (exhausted && c.max_budget < .7) {
+      c.max_budget += 0.1;
+      n_dec = 0;
+      n_inc = 0;
+      res.range_low = mn;
+      res.range_high = mx;
+      res.threshold = iter_err / static_cast<double>(n);
+    } else if (exhausted && c.max_budget >= .7 && c.max_active <= .7) {
+      c.max_active += .1;
+    } else if (exhausted) {
+      break;
+    }
+  } while (!res.pass_mem || !res.pass_errorest_budget);
+  res.max_budget_perc_to_cover = c.max_budget;
+  res.max_active_perc = c.max_active;
+  c.max_active = .5;
+  c.max_budget = .25;
+  return PAGANI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the loop
+
+int check_volume(pagani_ws* ws, const double* lows, const double* highs)
+{
+  for (int j = 0; j < ws->ndim; ++j) {
+    const double lo = lows ? lows[j] : 0.0, hi = highs ? highs[j] : 1.0;
+    if (!std::isfinite(lo) || !std::isfinite(hi))
+      return fail(ws, PAGANI_EBADARG, "non-finite integration bounds");
+  }
+  return PAGANI_OK;
+}
+
+int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel, double epsabs,
+                   const double* lows, const double* highs, int relerr,
+                   const pagani_region_list* init, std::size_t init_k, pagani_result* out)
+{
+  if (!ws || !eval || !out) return fail(ws, PAGANI_EBADARG, "null argument");
+  if (!(epsrel >= 0.0) || !(epsabs >= 0.0) || std::isnan(epsrel))
+    return fail(ws, PAGANI_EBADARG, "epsrel and epsabs must be >= 0");
+  int rc = check_volume(ws, lows, highs);
+  if (rc) return rc;
+  const auto wall0 = std::chrono::steady_clock::now();
+  const int d = ws->ndim;
+  std::memset(out, 0, sizeof(*out));
+  out->status = 1;
+  out->lastPhase = -1;
+  ws->trace.clear();
+  harvest_compact(ws);
+
+  // initial region list
+  std::size_t n = 0;
+  const bool user_regions = init != nullptr;
+  if (init) {
+    n = init->n;
+    if (n == 0 || !init->L || !init->H) return fail(ws, PAGANI_EBADARG, "empty region list");
+    if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
+    if ((rc = ensure(ws, ws->H[ws->cur], n * d, "region bounds"))) return rc;
+    HIP_TRY(ws, hipMemcpyAsync(ws->L[ws->cur].p, init->L, n * d * sizeof(double),
+                               hipMemcpyDeviceToDevice, ws->stream));
+    HIP_TRY(ws, hipMemcpyAsync(ws->H[ws->cur].p, init->H, n * d * sizeof(double),
+                               hipMemcpyDeviceToDevice, ws->stream));
+  } else {
+    std::size_t k = init_k ? init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
+    n = 1;
+    for (int j = 0; j < d; ++j) {
+      if (n > (std::size_t(1) << 40) / k) return fail(ws, PAGANI_EBADARG, "initial split too large");
+      n *= k;
+    }
+    if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
+    if ((rc = ensure(ws, ws->H[ws->cur], n * d, "region bounds"))) return rc;
+    HIP_TRY(ws, ph::uniform_split(ws->stream, d, k, n, ws->L[ws->cur].p, ws->H[ws->cur].p));
+  }
+
+  const std::size_t total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
+  Classifier clf(epsrel, epsabs, total, d);
+  pagani_result cum;
+  std::memset(&cum, 0, sizeof cum);
+  cum.status = 1;
+  cum.lastPhase = -1;
+  bool has_parents = false;
+  int status_code = PAGANI_OK;
+
+  for (std::size_t it = 0; it < 700 && n > 0; it++) {
+    if ((rc = ensure(ws, ws->est, n, "estimates")) || (rc = ensure(ws, ws->err0, n, "errors")) ||
+        (rc = ensure(ws, ws->err1, n, "errors")) || (rc = ensure(ws, ws->dim, n, "split axes")) ||
+        (rc = ensure(ws, ws->active, n, "flags")) || (rc = ensure_chunks(ws, n)))
+      return rc;
+
+    // evaluate (Cubature_rules::apply_cubature_integration_rules)
+    pagani_eval_args args;
+    args.n = n;
+    args.L = ws->L[ws->cur].p;
+    args.H = ws->H[ws->cur].p;
+    args.est = ws->est.p;
+    args.err = ws->err0.p;
+    args.dim = ws->dim.p;
+    args.stream = ws->stream;
+    make_consts(ws, lows, highs, n, args.consts);
+    record(ws, EV_EVAL0);
+    rc = eval(ctx, &args);
+    if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
+    record(ws, EV_EVAL1);
+    ws->stats.regions_evaluated += n;
+    ws->stats.eval_launches++;
+    ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, n);
+
+    // two-level error + relerr flags + the four region sums + counts
+    double* errs = has_parents ? ws->err1.p : ws->err0.p;
+    record(ws, EV_REF0);
+    HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
+                           has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
+                           ws->part.p, ws->cnt.p));
+    HIP_TRY(ws, ph::finalize(ws->stream, ph::num_chunks(n), ws->part.p, ws->cnt.p, ws->off.p,
+                             ws->d_scal, 0));
+    record(ws, EV_REF1);
+    HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
+                               hipMemcpyDeviceToHost, ws->stream));
+    HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+    if (ws->timing) {
+      ws->stats.eval_ms += elapsed(ws, EV_EVAL0, EV_EVAL1);
+      ws->stats.refine_ms += elapsed(ws, EV_REF0, EV_REF1);
+    }
+    harvest_compact(ws);
+    ws->stats.iterations++;
+
+    const double iter_est = ws->h_scal->sum[0];
+    const double iter_err = ws->h_scal->sum[1];
+    double act_est = ws->h_scal->sum[2];
+    double act_err = ws->h_scal->sum[3];
+    std::size_t m = ws->h_scal->count;
+    if (user_regions) cum.iters++;
+
+    // accuracy_reached, PaganiUtils.cuh:387-394
+    const double tot_est = std::fabs(cum.estimate + iter_est);
+    const double tot_err = cum.errorest + iter_err;
+    if (tot_err / tot_est <= epsrel || tot_err <= epsabs) {
+      ws->trace.push_back({n, 0, cum.estimate + iter_est, tot_err});
+      cum.estimate += iter_est;
+      cum.errorest += iter_err;
+      cum.status = 0;
+      cum.nregions += n;
+      goto done;
+    }
+    clf.store(cum.estimate + iter_est);
+    {
+      // compute_finished_estimates, PaganiUtils.cuh:368-385
+      double fin_est = iter_est - act_est;
+      double fin_err = iter_err - act_err;
+      bool recount = false;
+      // fix_error_budget_overflow, Workspace.cuh:121-146
+      if (cum.errorest + fin_err > std::fabs(cum.estimate + iter_est) * epsrel) {
+        HIP_TRY(ws, ph::fill_u8(ws->stream, n, ws->active.p, 1));
+        fin_err = 0.;
+        fin_est = 0.;
+        recount = true;
+      }
+      // heuristic_classify, Workspace.cuh:74-118
+      const bool necessary = mem_ratio(n, d, total) > 1.;
+      bool terminate;
+      if (!clf.criteria_met(n)) {
+        terminate = necessary;
+      } else {
+        if ((rc = ensure(ws, ws->flags, n, "flags"))) return rc;
+        pagani_class_res res;
+        record(ws, EV_CLS0);
+        rc = classify(ws, clf, errs, n, iter_err, fin_err, cum.errorest, ws->flags.p, res);
+        record(ws, EV_CLS1);
+        if (rc) return rc;
+        const bool ok = res.pass_mem && res.pass_errorest_budget;
+        if (ok) {
+          std::swap(ws->active, ws->flags);
+          recount = true;
+        }
+        terminate = (!ok && necessary) || res.num_active == 0;
+        if (ok) fin_err = res.finished_errorest;
+        if (ws->timing) {
+          HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+          ws->stats.classify_ms += elapsed(ws, EV_CLS0, EV_CLS1);
+        }
+        if (recount) {
+          HIP_TRY(ws, ph::count_flags(ws->stream, n, ws->active.p, ws->est.p, errs, ws->part.p,
+                                      ws->cnt.p));
+          if ((rc = finalize_read(ws, n, true, false))) return rc;
+          act_est = ws->h_scal->sum[2];
+          m = ws->h_scal->count;
+          recount = false;
+          if (ok) fin_est = iter_est - act_est;
+        }
+      }
+      if (recount) {
+        HIP_TRY(ws, ph::count_flags(ws->stream, n, ws->active.p, ws->est.p, errs, ws->part.p,
+                                    ws->cnt.p));
+        if ((rc = finalize_read(ws, n, true, false))) return rc;
+        m = ws->h_scal->count;
+      }
+      if (terminate) {
+        ws->trace.push_back({n, 0, cum.estimate + iter_est, tot_err});
+        cum.estimate += iter_est;
+        cum.errorest += iter_err;
+        cum.nregions += n;
+        goto done;
+      }
+      if (m > 0) {
+        const int nxt = 1 - ws->cur;
+        if ((rc = ensure(ws, ws->L[nxt], 2 * m * d, "region bounds")) ||
+            (rc = ensure(ws, ws->H[nxt], 2 * m * d, "region bounds")) ||
+            (rc = ensure(ws, ws->pest, m, "parent estimates")) ||
+            (rc = ensure(ws, ws->perr, m, "parent errors"))) {
+          // the device cannot hold the next split: stop like the reference's
+          // memory termination (status 1, all leaves summed)
+          ws->trace.push_back({n, 0, cum.estimate + iter_est, tot_err});
+          cum.estimate += iter_est;
+          cum.errorest += iter_err;
+          cum.nregions += n;
+          status_code = rc;
+          goto done;
+        }
+      }
+      ws->trace.push_back({n, m, cum.estimate + iter_est, tot_err});
+      cum.estimate += fin_est;
+      cum.errorest += fin_err;
+    }
+    // Sub_regions_filter::filter + Sub_region_splitter::split, fused
+    cum.nregions += n - m;
+    if (m == 0) {
+      n = 0;
+    } else {
+      const int nxt = 1 - ws->cur;
+      record(ws, EV_CMP0);
+      HIP_TRY(ws, ph::compact(ws->stream, 1, d, n, m, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                              ws->active.p, ws->est.p, errs, ws->dim.p, ws->off.p, ws->L[nxt].p,
+                              ws->H[nxt].p, ws->pest.p, ws->perr.p, nullptr));
+      record(ws, EV_CMP1);
+      ws->compact_pending = ws->timing;
+      ws->cur = nxt;
+      n = 2 * m;
+      has_parents = true;
+    }
+    if (!user_regions) cum.iters++;
+  }
+  cum.nregions += n;
+done:
+  ws->live_n = n;
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  harvest_compact(ws);
+  ws->stats.integrals++;
+  ws->stats.wall_ms +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+  *out = cum;
+  return status_code;
+}
+
+struct GenzCtx {
+  pagani_genz g;
+  int precision;
+};
+
+int genz_trampoline(void* ctx, const pagani_eval_args* a)
+{
+  const GenzCtx* c = static_cast<const GenzCtx*>(ctx);
+  return pagani_genz_eval_launch(&c->g, c->precision, a);
+}
+
+int check_genz(pagani_ws* ws, const pagani_genz* f, int precision)
+{
+  if (!f) return fail(ws, PAGANI_EBADARG, "null integrand");
+  if (f->ndim != ws->ndim) return fail(ws, PAGANI_EBADARG, "integrand ndim != workspace ndim");
+  if (f->family < PAGANI_GENZ_OSCILLATORY || f->family > PAGANI_GENZ_DISCONTINUOUS)
+    return fail(ws, PAGANI_EBADARG, "unknown integrand family");
+  if (precision != PAGANI_FP64 && precision != PAGANI_FP32)
+    return fail(ws, PAGANI_EBADARG, "precision must be 64 or 32");
+  return PAGANI_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int pagani_rule_tables(int ndim, double* w, double* scale, double* norm, double* lambda,
+                       int* set_begin, double* gtab)
+{
+  pagani::RuleTables R;
+  if (!pagani::build_rule(ndim, R)) return PAGANI_EBADARG;
+  for (int s = 0; s < PAGANI_NSETS; ++s)
+    for (int r = 0; r < PAGANI_NRULES; ++r) {
+      if (w) w[s * PAGANI_NRULES + r] = R.w[s][r];
+      if (scale) scale[s * PAGANI_NRULES + r] = R.scale[s][r];
+      if (norm) norm[s * PAGANI_NRULES + r] = R.norm[s][r];
+    }
+  if (lambda)
+    for (int i = 0; i < 5; ++i) lambda[i] = R.lambda[i];
+  if (set_begin)
+    for (int s = 0; s <= PAGANI_NSETS; ++s) set_begin[s] = R.set_begin[s];
+  if (gtab) std::memcpy(gtab, R.gtab.data(), R.gtab.size() * sizeof(double));
+  return R.feval;
+}
+
+static thread_local std::string g_create_error;
+
+const char* pagani_create_error(void) { return g_create_error.c_str(); }
+
+pagani_ws* pagani_ws_create(int ndim, int device)
+{
+  g_create_error.clear();
+  auto bail = [](pagani_ws* ws, const std::string& why, hipError_t e) -> pagani_ws* {
+    g_create_error = why + (e != hipSuccess ? std::string(": ") + hipGetErrorString(e) : "");
+    (void)hipGetLastError();
+    if (ws) pagani_ws_destroy(ws);
+    return nullptr;
+  };
+  if (ndim < PAGANI_MIN_DIM || ndim > PAGANI_MAX_DIM)
+    return bail(nullptr, "ndim outside [2, 8]", hipSuccess);
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) return bail(nullptr, "hipGetDeviceCount", e);
+  if (device < 0 && (e = hipGetDevice(&device)) != hipSuccess)
+    return bail(nullptr, "hipGetDevice", e);
+  if (device >= ndev) return bail(nullptr, "device index out of range", hipSuccess);
+  pagani_ws* ws = new pagani_ws();
+  ws->ndim = ndim;
+  ws->device = device;
+  if ((e = hipSetDevice(device)) != hipSuccess) return bail(ws, "hipSetDevice", e);
+  if ((e = hipStreamCreate(&ws->stream)) != hipSuccess) return bail(ws, "hipStreamCreate", e);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ws->hbm_total = prop.totalGlobalMem;
+    ws->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  }
+  (void)hipGetLastError();
+  if (!pagani::build_rule(ndim, ws->rule)) return bail(ws, "rule tables", hipSuccess);
+  const std::size_t gbytes = ws->rule.gtab.size() * sizeof(double);
+  if ((e = hipMalloc(&ws->d_gtab, gbytes)) != hipSuccess) return bail(ws, "hipMalloc(gtab)", e);
+  if ((e = hipMemcpy(ws->d_gtab, ws->rule.gtab.data(), gbytes, hipMemcpyHostToDevice)) !=
+      hipSuccess)
+    return bail(ws, "hipMemcpy(gtab)", e);
+  if ((e = hipMalloc(&ws->d_scal, sizeof(ph::Scalars))) != hipSuccess)
+    return bail(ws, "hipMalloc(scalars)", e);
+  if ((e = hipHostMalloc(&ws->h_scal, sizeof(ph::Scalars), hipHostMallocDefault)) != hipSuccess)
+    return bail(ws, "hipHostMalloc(scalars)", e);
+  for (auto& ev : ws->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(ws, "hipEventCreate", e);
+  return ws;
+}
+
+void pagani_ws_destroy(pagani_ws* ws)
+{
+  if (!ws) return;
+  if (ws->stream) hipStreamSynchronize(ws->stream);
+  for (int i = 0; i < 2; ++i) {
+    release(ws->L[i]);
+    release(ws->H[i]);
+  }
+  release(ws->est);
+  release(ws->err0);
+  release(ws->err1);
+  release(ws->pest);
+  release(ws->perr);
+  release(ws->dim);
+  release(ws->active);
+  release(ws->flags);
+  release(ws->part);
+  release(ws->cnt);
+  release(ws->off);
+  if (ws->d_gtab) hipFree(ws->d_gtab);
+  if (ws->d_scal) hipFree(ws->d_scal);
+  if (ws->h_scal) hipHostFree(ws->h_scal);
+  for (auto& e : ws->ev)
+    if (e) hipEventDestroy(e);
+  if (ws->stream) hipStreamDestroy(ws->stream);
+  delete ws;
+}
+
+const char* pagani_ws_last_error(const pagani_ws* ws)
+{
+  return ws ? ws->error.c_str() : "null workspace";
+}
+
+int pagani_ws_ndim(const pagani_ws* ws) { return ws ? ws->ndim : PAGANI_EBADARG; }
+
+int pagani_ws_set_device_mem(pagani_ws* ws, size_t total_bytes)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->mem_model_total = total_bytes;
+  return PAGANI_OK;
+}
+
+size_t pagani_ws_device_mem(const pagani_ws* ws)
+{
+  if (!ws) return 0;
+  return ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
+}
+
+int pagani_ws_set_slices(pagani_ws* ws, int slices)
+{
+  if (!ws || slices < 0 || slices > PAGANI_MAX_SLICES || (slices & (slices - 1)))
+    return fail(ws, PAGANI_EBADARG, "slices must be 0 or a power of two <= 8");
+  ws->slices_override = slices;
+  return PAGANI_OK;
+}
+
+int pagani_ws_set_timing(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->timing = enable != 0;
+  return PAGANI_OK;
+}
+
+int pagani_ws_get_stats(const pagani_ws* ws, pagani_stats* out)
+{
+  if (!ws || !out) return PAGANI_EBADARG;
+  *out = ws->stats;
+  return PAGANI_OK;
+}
+
+int pagani_ws_reset_stats(pagani_ws* ws)
+{
+  if (!ws) return PAGANI_EBADARG;
+  std::memset(&ws->stats, 0, sizeof(ws->stats));
+  return PAGANI_OK;
+}
+
+int pagani_ws_eval_consts(pagani_ws* ws, const double* lows, const double* highs, size_t n,
+                          pagani_eval_consts* out)
+{
+  if (!ws || !out) return PAGANI_EBADARG;
+  make_consts(ws, lows, highs, n, *out);
+  return PAGANI_OK;
+}
+
+void* pagani_ws_stream(pagani_ws* ws) { return ws ? ws->stream : nullptr; }
+
+size_t pagani_ws_trace(const pagani_ws* ws, size_t cap, uint64_t* n, uint64_t* m, double* est,
+                       double* err)
+{
+  if (!ws) return 0;
+  const std::size_t k = std::min(cap, ws->trace.size());
+  for (std::size_t i = 0; i < k; ++i) {
+    if (n) n[i] = ws->trace[i].n;
+    if (m) m[i] = ws->trace[i].m;
+    if (est) est[i] = ws->trace[i].estimate;
+    if (err) err[i] = ws->trace[i].errorest;
+  }
+  return ws->trace.size();
+}
+
+int pagani_ws_live_regions(const pagani_ws* ws, size_t* n, const double** L, const double** H)
+{
+  if (!ws || !n || !L || !H) return PAGANI_EBADARG;
+  *n = ws->live_n;
+  *L = ws->L[ws->cur].p;
+  *H = ws->H[ws->cur].p;
+  return PAGANI_OK;
+}
+
+int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
+                        double epsabs, const double* lows, const double* highs,
+                        int relerr_classification, const pagani_region_list* init, size_t init_k,
+                        pagani_result* out)
+{
+  if (!ws) return PAGANI_EBADARG;
+  if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+  return integrate_impl(ws, eval, ctx, epsrel, epsabs, lows, highs, relerr_classification, init,
+                        init_k, out);
+}
+
+int pagani_integrate_genz(pagani_ws* ws, const pagani_genz* f, int precision, double epsrel,
+                          double epsabs, const double* lows, const double* highs,
+                          int relerr_classification, size_t init_k, pagani_result* out)
+{
+  if (!ws) return PAGANI_EBADARG;
+  int rc = check_genz(ws, f, precision);
+  if (rc) return rc;
+  GenzCtx ctx{*f, precision};
+  return pagani_integrate_cb(ws, genz_trampoline, &ctx, epsrel, epsabs, lows, highs,
+                             relerr_classification, nullptr, init_k, out);
+}
+
+int pagani_uniform_split_dev(pagani_ws* ws, size_t k, double* L, double* H)
+{
+  if (!ws || !L || !H || k == 0) return fail(ws, PAGANI_EBADARG, "bad uniform split arguments");
+  std::size_t n = 1;
+  for (int j = 0; j < ws->ndim; ++j) n *= k;
+  HIP_TRY(ws, ph::uniform_split(ws->stream, ws->ndim, k, n, L, H));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+int pagani_eval_genz_dev(pagani_ws* ws, const pagani_genz* f, int precision, size_t n,
+                         const double* L, const double* H, const double* lows,
+                         const double* highs, double* est, double* err, uint8_t* dim)
+{
+  if (!ws) return PAGANI_EBADARG;
+  int rc = check_genz(ws, f, precision);
+  if (rc) return rc;
+  if (n == 0) return PAGANI_OK;
+  if (!L || !H || !est || !err || !dim) return fail(ws, PAGANI_EBADARG, "null device pointer");
+  if ((rc = check_volume(ws, lows, highs))) return rc;
+  pagani_eval_args args;
+  args.n = n;
+  args.L = L;
+  args.H = H;
+  args.est = est;
+  args.err = err;
+  args.dim = dim;
+  args.stream = ws->stream;
+  make_consts(ws, lows, highs, n, args.consts);
+  record(ws, EV_EVAL0);
+  rc = pagani_genz_eval_launch(f, precision, &args);
+  if (rc) return fail(ws, rc, "evaluation launch failed");
+  record(ws, EV_EVAL1);
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  if (ws->timing) ws->stats.eval_ms += elapsed(ws, EV_EVAL0, EV_EVAL1);
+  ws->stats.regions_evaluated += n;
+  ws->stats.eval_launches++;
+  return PAGANI_OK;
+}
+
+int pagani_refine_dev(pagani_ws* ws, size_t n, const double* est, const double* err_in,
+                      double* err_out, const double* parent_est, double epsrel,
+                      int relerr_classification, uint8_t* active, double* sums,
+                      size_t* num_active)
+{
+  if (!ws || !est || !err_in || !err_out || !active)
+    return fail(ws, PAGANI_EBADARG, "null argument");
+  if (parent_est && (n % 2)) return fail(ws, PAGANI_EBADARG, "refinement needs an even count");
+  if (parent_est && err_in == err_out)
+    return fail(ws, PAGANI_EBADARG, "refinement cannot run in place");
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::refine(ws->stream, n, est, err_in, err_out, parent_est, epsrel,
+                         relerr_classification, active, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, true, false))) return rc;
+  if (sums)
+    for (int i = 0; i < 4; ++i) sums[i] = ws->h_scal->sum[i];
+  if (num_active) *num_active = ws->h_scal->count;
+  return PAGANI_OK;
+}
+
+int pagani_filter_dev(pagani_ws* ws, size_t n, const double* L, const double* H,
+                      const uint8_t* active, const double* est, const double* err,
+                      const uint8_t* dim, double* L2, double* H2, double* parent_est,
+                      double* parent_err, uint8_t* dim2, size_t* m_out)
+{
+  if (!ws || !L || !H || !active || !est || !err || !dim || !L2 || !H2 || !m_out)
+    return fail(ws, PAGANI_EBADARG, "null argument");
+  *m_out = 0;
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, true, false))) return rc;
+  const std::size_t m = ws->h_scal->count;
+  *m_out = m;
+  if (m == 0) return PAGANI_OK;
+  HIP_TRY(ws, ph::compact(ws->stream, 0, ws->ndim, n, m, L, H, active, est, err, dim, ws->off.p,
+                          L2, H2, parent_est, parent_err, dim2));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+int pagani_split_dev(pagani_ws* ws, size_t m, const double* L, const double* H,
+                     const uint8_t* dim, double* L2, double* H2)
+{
+  if (!ws || !L || !H || !dim || !L2 || !H2) return fail(ws, PAGANI_EBADARG, "null argument");
+  HIP_TRY(ws, ph::split_all(ws->stream, ws->ndim, m, L, H, dim, L2, H2));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+int pagani_filter_split_dev(pagani_ws* ws, size_t n, const double* L, const double* H,
+                            const uint8_t* active, const double* est, const double* err,
+                            const uint8_t* dim, double* L2, double* H2, double* parent_est,
+                            double* parent_err, size_t* m_out)
+{
+  if (!ws || !L || !H || !active || !est || !err || !dim || !L2 || !H2 || !m_out)
+    return fail(ws, PAGANI_EBADARG, "null argument");
+  *m_out = 0;
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, true, false))) return rc;
+  const std::size_t m = ws->h_scal->count;
+  *m_out = m;
+  if (m == 0) return PAGANI_OK;
+  HIP_TRY(ws, ph::compact(ws->stream, 1, ws->ndim, n, m, L, H, active, est, err, dim, ws->off.p,
+                          L2, H2, parent_est, parent_err, nullptr));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+int pagani_sum_dev(pagani_ws* ws, const double* x, size_t n, double* out)
+{
+  if (!ws || !out || (n && !x)) return fail(ws, PAGANI_EBADARG, "null argument");
+  *out = 0.0;
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::sum_dot(ws->stream, n, x, nullptr, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, false, false))) return rc;
+  *out = ws->h_scal->sum[0];
+  return PAGANI_OK;
+}
+
+int pagani_dot_dev(pagani_ws* ws, const uint8_t* flags, const double* x, size_t n, double* out)
+{
+  if (!ws || !out || (n && (!x || !flags))) return fail(ws, PAGANI_EBADARG, "null argument");
+  *out = 0.0;
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::sum_dot(ws->stream, n, x, flags, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, false, false))) return rc;
+  *out = ws->h_scal->sum[2];
+  return PAGANI_OK;
+}
+
+int pagani_minmax_dev(pagani_ws* ws, const double* x, size_t n, double* mn, double* mx)
+{
+  if (!ws || !x || !mn || !mx || n == 0) return fail(ws, PAGANI_EBADARG, "bad minmax arguments");
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::minmax(ws->stream, n, x, ws->part.p));
+  if ((rc = finalize_read(ws, n, false, true))) return rc;
+  *mn = ws->h_scal->mn;
+  *mx = ws->h_scal->mx;
+  return PAGANI_OK;
+}
+
+int pagani_scan_flags_dev(pagani_ws* ws, const uint8_t* flags, size_t n, uint64_t* offsets,
+                          size_t* total)
+{
+  if (!ws || !flags || !offsets || !total) return fail(ws, PAGANI_EBADARG, "null argument");
+  *total = 0;
+  if (n == 0) return PAGANI_OK;
+  int rc = ensure_chunks(ws, n);
+  if (rc) return rc;
+  HIP_TRY(ws, ph::sum_dot(ws->stream, n, nullptr, flags, ws->part.p, ws->cnt.p));
+  if ((rc = finalize_read(ws, n, true, false))) return rc;
+  *total = ws->h_scal->count;
+  HIP_TRY(ws, ph::scan_flags(ws->stream, n, flags, ws->off.p, offsets));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+int pagani_classify_dev(pagani_ws* ws, const double* err, size_t n, double epsrel,
+                        const double* est_hist, double iter_errorest,
+                        double iter_finished_errorest, double total_finished_errorest,
+                        uint8_t* flags, pagani_class_res* out)
+{
+  if (!ws || !err || !flags || !out || n == 0 || !est_hist)
+    return fail(ws, PAGANI_EBADARG, "bad classify arguments");
+  const std::size_t total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
+  Classifier c(epsrel, 0.0, total, ws->ndim);
+  c.hist[0] = est_hist[0];
+  c.hist[1] = est_hist[1];
+  c.hist[2] = est_hist[2];
+  c.iters_collected = 3;
+  return classify(ws, c, err, n, iter_errorest, iter_finished_errorest, total_finished_errorest,
+                  flags, *out);
+}
+
+}  // extern "C"

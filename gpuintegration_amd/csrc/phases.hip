@@ -1,0 +1,592 @@
+// phases.hip -- region-list kernels for gfx950 (see phases.hpp).
+//
+// Compiled with -ffp-contract=off: the refinement and classification
+// arithmetic is plain IEEE double in the reference's operation order, so
+// these phases are bit-reproducible against a CPU restatement.
+#include "phases.hpp"
+
+#include <cmath>
+
+namespace pagani {
+namespace phases {
+
+namespace {
+
+struct DD {
+  double hi, lo;
+};
+
+// Compensated addition (two-sum of the leading parts, then the tails).
+// Non-finite sums fall back to the plain sum so inf/NaN propagate as in an
+// uncompensated reduction.
+__device__ __forceinline__ DD dd_add(DD a, DD b)
+{
+  const double s = a.hi + b.hi;
+  if (!__builtin_isfinite(s)) return DD{s, 0.0};
+  const double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b.hi - bb);
+  e += a.lo + b.lo;
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, double b) { return dd_add(a, DD{b, 0.0}); }
+
+__device__ __forceinline__ DD wave_reduce(DD v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    DD o;
+    o.hi = __shfl_xor(v.hi, off, 64);
+    o.lo = __shfl_xor(v.lo, off, 64);
+    v = dd_add(v, o);
+  }
+  return v;
+}
+
+// Reduce a DD over a 256-thread block; every thread gets the result.
+__device__ __forceinline__ DD block_reduce(DD v, DD* sh)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  v = wave_reduce(v);
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  DD r = sh[0];
+#pragma unroll
+  for (int w = 1; w < kThreads / 64; ++w) r = dd_add(r, sh[w]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ std::uint32_t block_count(std::uint32_t c, std::uint32_t* sh)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+  if (lane == 0) sh[wave] = c;
+  __syncthreads();
+  std::uint32_t r = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) r += sh[w];
+  __syncthreads();
+  return r;
+}
+
+// Write the chunk record: four DD sums and a count.
+__device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::uint32_t c,
+                                             double* part, std::uint32_t* cnt)
+{
+  __shared__ DD sh[kThreads / 64];
+  __shared__ std::uint32_t shc[kThreads / 64];
+  s0 = block_reduce(s0, sh);
+  s1 = block_reduce(s1, sh);
+  s2 = block_reduce(s2, sh);
+  s3 = block_reduce(s3, sh);
+  c = block_count(c, shc);
+  if (threadIdx.x == 0) {
+    double* p = part + static_cast<std::size_t>(blockIdx.x) * kPartDoubles;
+    p[0] = s0.hi;
+    p[1] = s0.lo;
+    p[2] = s1.hi;
+    p[3] = s1.lo;
+    p[4] = s2.hi;
+    p[5] = s2.lo;
+    p[6] = s3.hi;
+    p[7] = s3.lo;
+    if (cnt) cnt[blockIdx.x] = c;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_uniform_split(int d, std::size_t k, std::size_t n,
+                                                       double len, double* L, double* H)
+{
+  const std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  std::size_t q = t;
+  for (int j = 0; j < d; ++j) {
+    const std::size_t id = q % k;
+    q /= k;
+    L[static_cast<std::size_t>(j) * n + t] = static_cast<double>(id) * len;
+    H[static_cast<std::size_t>(j) * n + t] = len;
+  }
+}
+
+// RefineError (Phases.cuh:219-268) and the relerr flag; children t and t+n/2
+// share parent t mod n/2.
+__global__ void __launch_bounds__(kThreads)
+    k_refine(std::size_t n, const double* __restrict__ est, const double* err_in,
+             double* err_out, const double* __restrict__ pest, double epsrel, int relerr,
+             std::uint8_t* __restrict__ active, double* __restrict__ part,
+             std::uint32_t* __restrict__ cnt)
+{
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  const std::size_t h = n / 2;
+  DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
+  std::uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      const double e = est[t];
+      double r = err_in[t];
+      bool a = true;
+      if (pest) {
+        const bool right = t >= h;
+        const std::size_t sib = right ? t - h : t + h;
+        const std::size_t par = right ? t - h : t;
+        const double es = est[sib];
+        const double rsib = err_in[sib];
+        const double diff = fabs(.25 * ((es + e) - pest[par]));
+        const double E = r + rsib;
+        if (E > 0.0) {
+          const double cc = 1 + 2 * diff / E;
+          r *= cc;
+        }
+        r += diff;
+        const bool pass = relerr && r < fmax(epsrel * fabs(e), 1e-200);
+        a = !pass;
+      }
+      err_out[t] = r;
+      active[t] = a ? 1 : 0;
+      se = dd_add(se, e);
+      sr = dd_add(sr, r);
+      if (a) {
+        sae = dd_add(sae, e);
+        sar = dd_add(sar, r);
+        ++c;
+      }
+    }
+  }
+  write_record(se, sr, sae, sar, c, part, cnt);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_count_flags(std::size_t n, const std::uint8_t* __restrict__ flags,
+                  const double* __restrict__ est, const double* __restrict__ err,
+                  double* __restrict__ part, std::uint32_t* __restrict__ cnt)
+{
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
+  std::uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      const double e = est[t], r = err[t];
+      se = dd_add(se, e);
+      sr = dd_add(sr, r);
+      if (flags[t] == 1) {
+        sae = dd_add(sae, e);
+        sar = dd_add(sar, r);
+        ++c;
+      }
+    }
+  }
+  write_record(se, sr, sae, sar, c, part, cnt);
+}
+
+// device_set_true_for_larger_than + the reductions of apply_threshold and
+// evaluate_error_budget (heuristic_classifier.cuh:60-87, 282-325).
+__global__ void __launch_bounds__(kThreads)
+    k_threshold(std::size_t n, const double* __restrict__ err, double thr,
+                std::uint8_t* __restrict__ flags, double* __restrict__ part,
+                std::uint32_t* __restrict__ cnt)
+{
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  DD z{0, 0}, sr{0, 0}, sar{0, 0};
+  std::uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      const double r = err[t];
+      const bool a = r > thr;
+      flags[t] = a ? 1 : 0;
+      sr = dd_add(sr, r);
+      if (a) {
+        sar = dd_add(sar, r);
+        ++c;
+      }
+    }
+  }
+  write_record(z, sr, z, sar, c, part, cnt);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_sum_dot(std::size_t n, const double* __restrict__ x,
+              const std::uint8_t* __restrict__ flags, double* __restrict__ part,
+              std::uint32_t* __restrict__ cnt)
+{
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  DD z{0, 0}, sx{0, 0}, sfx{0, 0};
+  std::uint32_t c = 0;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      const double v = x ? x[t] : 0.0;
+      sx = dd_add(sx, v);
+      if (flags) {
+        sfx = dd_add(sfx, static_cast<double>(flags[t]) * v);
+        c += flags[t] != 0;
+      }
+    }
+  }
+  write_record(sx, z, sfx, z, c, part, cnt);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_minmax(std::size_t n, const double* __restrict__ x, double* __restrict__ part)
+{
+  __shared__ double smn[kThreads / 64], smx[kThreads / 64];
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  double mn = HUGE_VAL, mx = -HUGE_VAL;
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      mn = fmin(mn, x[t]);
+      mx = fmax(mx, x[t]);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, off, 64));
+    mx = fmax(mx, __shfl_xor(mx, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    smn[wave] = mn;
+    smx[wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      mn = fmin(mn, smn[w]);
+      mx = fmax(mx, smx[w]);
+    }
+    double* p = part + static_cast<std::size_t>(blockIdx.x) * kPartDoubles;
+    p[8] = mn;
+    p[9] = mx;
+  }
+}
+
+// One workgroup reduces all chunk records in rounds of 1024 chunks: the
+// counts are scanned round by round (chunk order) into exclusive offsets;
+// each thread folds its chunks' sums (fixed strided order), then a fixed
+// tree combines the threads.
+constexpr int kFinThreads = 1024;
+
+__global__ void __launch_bounds__(kFinThreads)
+    k_finalize(std::size_t nchunks, const double* __restrict__ part,
+               const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+               Scalars* __restrict__ out, int with_minmax)
+{
+  __shared__ std::uint64_t wtot[kFinThreads / 64];
+  __shared__ DD tree[kFinThreads];
+  __shared__ double tmn[kFinThreads], tmx[kFinThreads];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double mn = HUGE_VAL, mx = -HUGE_VAL;
+  std::uint64_t running = 0;
+  for (std::size_t r0 = 0; r0 < nchunks; r0 += kFinThreads) {
+    const std::size_t c = r0 + tid;
+    std::uint64_t v = 0;
+    if (c < nchunks) {
+      const double* p = part + c * kPartDoubles;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) s[i] = dd_add(s[i], DD{p[2 * i], p[2 * i + 1]});
+      if (with_minmax) {
+        mn = fmin(mn, p[8]);
+        mx = fmax(mx, p[9]);
+      }
+      if (cnt) v = cnt[c];
+    }
+    if (cnt) {
+      // inclusive scan within the wave
+      std::uint64_t inc = v;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const std::uint64_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+      }
+      if (lane == 63) wtot[wave] = inc;
+      __syncthreads();
+      std::uint64_t wbase = 0, total = 0;
+      for (int w = 0; w < kFinThreads / 64; ++w) {
+        if (w < wave) wbase += wtot[w];
+        total += wtot[w];
+      }
+      if (c < nchunks && off) off[c] = running + wbase + inc - v;
+      running += total;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    tree[tid] = s[i];
+    __syncthreads();
+    for (int stride = kFinThreads / 2; stride > 0; stride >>= 1) {
+      if (tid < stride) tree[tid] = dd_add(tree[tid], tree[tid + stride]);
+      __syncthreads();
+    }
+    if (tid == 0) out->sum[i] = tree[0].hi + tree[0].lo;
+    __syncthreads();
+  }
+  if (with_minmax) {
+    tmn[tid] = mn;
+    tmx[tid] = mx;
+    __syncthreads();
+    for (int stride = kFinThreads / 2; stride > 0; stride >>= 1) {
+      if (tid < stride) {
+        tmn[tid] = fmin(tmn[tid], tmn[tid + stride]);
+        tmx[tid] = fmax(tmx[tid], tmx[tid + stride]);
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0) {
+    out->count = running;
+    out->mn = with_minmax ? tmn[0] : 0.0;
+    out->mx = with_minmax ? tmx[0] : 0.0;
+    out->pad = 0;
+  }
+}
+
+// Chunk-local exclusive position of the active element t (ballot within the
+// wave, wave totals through LDS).  Returns the block total.
+__device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
+                                                    std::uint32_t* wsum)
+{
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long ballot = __ballot(a);
+  const unsigned long long below = lane ? (ballot & ((1ull << lane) - 1ull)) : 0ull;
+  const std::uint32_t pre = static_cast<std::uint32_t>(__popcll(below));
+  if (lane == 0) wsum[wave] = static_cast<std::uint32_t>(__popcll(ballot));
+  __syncthreads();
+  std::uint32_t woff = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) {
+    if (w < wave) woff += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  rank = woff + pre;
+  return tot;
+}
+
+// alignRegions (Sub_region_filter.cuh:8-47) and, with split, the bisection
+// of divideIntervalsGPU (Sub_region_splitter.cuh:8-46) fused: the active
+// region t at compacted index idx produces children idx and m+idx.
+__global__ void __launch_bounds__(kThreads)
+    k_compact(int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
+              const double* __restrict__ H, const std::uint8_t* __restrict__ active,
+              const double* __restrict__ est, const double* __restrict__ err,
+              const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
+              double* __restrict__ L2, double* __restrict__ H2, double* __restrict__ pest,
+              double* __restrict__ perr, std::uint8_t* __restrict__ dim2)
+{
+  __shared__ std::uint32_t wsum[kThreads / 64];
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  std::uint64_t run = off[blockIdx.x];
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    const bool a = t < n && active[t] == 1;
+    std::uint32_t rank;
+    const std::uint32_t tot = chunk_rank(a, rank, wsum);
+    if (a) {
+      const std::size_t idx = run + rank;
+      if (idx < m) {
+        if (split) {
+          const std::size_t n2 = 2 * m;
+          const int b = dim[t];
+          for (int j = 0; j < d; ++j) {
+            const double l = L[static_cast<std::size_t>(j) * n + t];
+            const double h = H[static_cast<std::size_t>(j) * n + t];
+            const std::size_t o = static_cast<std::size_t>(j) * n2 + idx;
+            if (j == b) {
+              const double half = h / 2;
+              L2[o] = l;
+              L2[o + m] = l + half;
+              H2[o] = half;
+              H2[o + m] = half;
+            } else {
+              L2[o] = l;
+              L2[o + m] = l;
+              H2[o] = h;
+              H2[o + m] = h;
+            }
+          }
+        } else {
+          for (int j = 0; j < d; ++j) {
+            L2[static_cast<std::size_t>(j) * m + idx] = L[static_cast<std::size_t>(j) * n + t];
+            H2[static_cast<std::size_t>(j) * m + idx] = H[static_cast<std::size_t>(j) * n + t];
+          }
+          if (dim2) dim2[idx] = dim[t];
+        }
+        if (pest) pest[idx] = est[t];
+        if (perr) perr[idx] = err[t];
+      }
+    }
+    run += tot;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_split_all(int d, std::size_t m, const double* __restrict__ L, const double* __restrict__ H,
+                const std::uint8_t* __restrict__ dim, double* __restrict__ L2,
+                double* __restrict__ H2)
+{
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const std::size_t n2 = 2 * m;
+  const int b = dim[i];
+  for (int j = 0; j < d; ++j) {
+    const double l = L[static_cast<std::size_t>(j) * m + i];
+    const double h = H[static_cast<std::size_t>(j) * m + i];
+    const std::size_t o = static_cast<std::size_t>(j) * n2 + i;
+    if (j == b) {
+      const double half = h / 2;
+      L2[o] = l;
+      L2[o + m] = l + half;
+      H2[o] = half;
+      H2[o + m] = half;
+    } else {
+      L2[o] = l;
+      L2[o + m] = l;
+      H2[o] = h;
+      H2[o + m] = h;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_scan_flags(std::size_t n, const std::uint8_t* __restrict__ flags,
+                 const std::uint64_t* __restrict__ off, std::uint64_t* __restrict__ out)
+{
+  __shared__ std::uint32_t wsum[kThreads / 64];
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  std::uint64_t run = off[blockIdx.x];
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    const bool a = t < n && flags[t] != 0;
+    std::uint32_t rank;
+    const std::uint32_t tot = chunk_rank(a, rank, wsum);
+    if (t < n) out[t] = run + rank;
+    run += tot;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u8(std::size_t n, std::uint8_t* p, std::uint8_t v)
+{
+  const std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
+inline dim3 grid_for(std::size_t n, unsigned threads)
+{
+  return dim3(static_cast<unsigned>((n + threads - 1) / threads));
+}
+
+}  // namespace
+
+hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, double* L,
+                         double* H)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_uniform_split, grid_for(n, 256), dim3(256), 0, s, d, k, n,
+                     1. / static_cast<double>(k), L, H);
+  return hipGetLastError();
+}
+
+hipError_t refine(hipStream_t s, std::size_t n, const double* est, const double* err_in,
+                  double* err_out, const double* parent_est, double epsrel, int relerr,
+                  std::uint8_t* active, double* part, std::uint32_t* cnt)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_refine, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
+                     n, est, err_in, err_out, parent_est, epsrel, relerr, active, part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t count_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
+                       const double* est, const double* err, double* part, std::uint32_t* cnt)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_flags, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads),
+                     0, s, n, flags, est, err, part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t threshold(hipStream_t s, std::size_t n, const double* err, double thr,
+                     std::uint8_t* flags, double* part, std::uint32_t* cnt)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_threshold, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0,
+                     s, n, err, thr, flags, part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t minmax(hipStream_t s, std::size_t n, const double* x, double* part)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_minmax, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
+                     n, x, part);
+  return hipGetLastError();
+}
+
+hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x, const std::uint8_t* flags,
+                   double* part, std::uint32_t* cnt)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sum_dot, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
+                     n, x, flags, part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
+                    const std::uint32_t* cnt, std::uint64_t* off, Scalars* out, int with_minmax)
+{
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, nchunks, part, cnt, off, out,
+                     with_minmax);
+  return hipGetLastError();
+}
+
+hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m,
+                   const double* L, const double* H, const std::uint8_t* active,
+                   const double* est, const double* err, const std::uint8_t* dim,
+                   const std::uint64_t* off, double* L2, double* H2, double* pest,
+                   double* perr, std::uint8_t* dim2)
+{
+  if (n == 0 || m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
+                     split, d, n, m, L, H, active, est, err, dim, off, L2, H2, pest, perr, dim2);
+  return hipGetLastError();
+}
+
+hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,
+                     const std::uint8_t* dim, double* L2, double* H2)
+{
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_all, grid_for(m, 256), dim3(256), 0, s, d, m, L, H, dim, L2, H2);
+  return hipGetLastError();
+}
+
+hipError_t scan_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
+                      const std::uint64_t* off, std::uint64_t* out)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scan_flags, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0,
+                     s, n, flags, off, out);
+  return hipGetLastError();
+}
+
+hipError_t fill_u8(hipStream_t s, std::size_t n, std::uint8_t* p, std::uint8_t v)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_u8, grid_for(n, 256), dim3(256), 0, s, n, p, v);
+  return hipGetLastError();
+}
+
+}  // namespace phases
+}  // namespace pagani

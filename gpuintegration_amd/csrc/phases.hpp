@@ -1,0 +1,90 @@
+// phases.hpp -- launchers of the region-list kernels (everything except the
+// rule evaluation).  All launches are asynchronous on the given stream.
+//
+// Work is cut into fixed chunks of kChunk consecutive regions, one workgroup
+// per chunk.  A chunk kernel writes per-chunk partial results (four
+// compensated double-double sums and a 32-bit count); finalize() reduces them
+// in a fixed order and turns the counts into exclusive chunk offsets, so
+// every sum is deterministic run to run and the compaction needs no atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace pagani {
+namespace phases {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 8;
+constexpr std::size_t kChunk = static_cast<std::size_t>(kThreads) * kPerThread;
+
+inline std::size_t num_chunks(std::size_t n) { return (n + kChunk - 1) / kChunk; }
+
+// Results of finalize(), read back by the host driver.
+struct Scalars {
+  double sum[4];   // the four chunk sums, compensated
+  double mn, mx;   // min / max (minmax pass only)
+  std::uint64_t count;
+  std::uint64_t pad;
+};
+
+// per-chunk partial record: 4 (hi, lo) sums + min + max
+constexpr int kPartDoubles = 10;
+
+hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, double* L,
+                         double* H);
+
+// RefineError + relerr classification (parent_est == nullptr: iteration 0,
+// err_out may equal err_in).  Partial sums: {est, err_out, act*est, act*err_out}.
+hipError_t refine(hipStream_t s, std::size_t n, const double* est, const double* err_in,
+                  double* err_out, const double* parent_est, double epsrel, int relerr,
+                  std::uint8_t* active, double* part, std::uint32_t* cnt);
+
+// Same partial sums for given flags (after the budget fix or a classification).
+hipError_t count_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
+                       const double* est, const double* err, double* part,
+                       std::uint32_t* cnt);
+
+// flags = err > thr; partial sums {0, err, 0, flag*err}, count.
+hipError_t threshold(hipStream_t s, std::size_t n, const double* err, double thr,
+                     std::uint8_t* flags, double* part, std::uint32_t* cnt);
+
+// min / max of x into the partial records.
+hipError_t minmax(hipStream_t s, std::size_t n, const double* x, double* part);
+
+// Sum of x (partial sum 0) and dot(flags, x) (partial sum 2) for the
+// thrust-equivalent entry points.
+hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x,
+                   const std::uint8_t* flags, double* part, std::uint32_t* cnt);
+
+// Reduce nchunks partial records; chunk offsets (exclusive scan of cnt) into
+// off (may be nullptr); results into *out.
+hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
+                    const std::uint32_t* cnt, std::uint64_t* off, Scalars* out,
+                    int with_minmax);
+
+// Order-preserving compaction of the active regions (chunk offsets from
+// finalize).  split == 0: filter only, bounds at stride m, split axes
+// copied to dim2.  split == 1: the two children of every active region at
+// stride 2m ([left m | right m]), the chosen axis halved.  Parent estimates
+// and errors go to pest/perr[0, m).
+hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m,
+                   const double* L, const double* H, const std::uint8_t* active,
+                   const double* est, const double* err, const std::uint8_t* dim,
+                   const std::uint64_t* off, double* L2, double* H2, double* pest,
+                   double* perr, std::uint8_t* dim2);
+
+// Children of all m regions (Sub_region_splitter::split without a filter).
+hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,
+                     const std::uint8_t* dim, double* L2, double* H2);
+
+// Per-element exclusive scan of 0/1 flags using chunk offsets.
+hipError_t scan_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
+                      const std::uint64_t* off, std::uint64_t* out);
+
+hipError_t fill_u8(hipStream_t s, std::size_t n, std::uint8_t* p, std::uint8_t v);
+
+}  // namespace phases
+}  // namespace pagani

@@ -1,0 +1,295 @@
+/*
+ * pagani.h -- C ABI of the MI355X-native PAGANI adaptive cubature library
+ * (libpagani.so, HIP for gfx950).
+ *
+ * The boundary replaces the reference's C++ integrator surface
+ *   Workspace<T,ndim>::integrate(...)        cuda/pagani/quad/GPUquad/Workspace.cuh:23-71
+ *   numint::integration_result               common/integration_result.hh:11-21
+ * and its phase classes (cited per entry point below).  Plain C types only:
+ * pointers, sizes, PODs.  Every entry point returns an int status
+ * (PAGANI_OK == 0, negative on error) and never throws or aborts; the text
+ * of the last error of a workspace is available from pagani_ws_last_error.
+ *
+ * Device pointers ("_dev" entry points) are HIP device allocations on the
+ * workspace's device; those entry points run on the workspace's stream and
+ * return after it has drained.
+ *
+ * Data layout (HBM, structure of arrays, unit-cube coordinates):
+ *   L[j*n + t]  left coordinate of region t along axis j   (FP64)
+ *   H[j*n + t]  length of region t along axis j            (FP64)
+ *   est[t], err[t]                                          (FP64)
+ *   dim[t]      split axis, uint8                           (reference: int)
+ *   active[t]   0/1 flag, uint8                             (reference: double)
+ */
+#ifndef PAGANI_H
+#define PAGANI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PAGANI_MIN_DIM 2
+#define PAGANI_MAX_DIM 8
+#define PAGANI_NSETS 9
+#define PAGANI_NRULES 5
+#define PAGANI_MAX_SLICES 8
+
+/* status codes */
+#define PAGANI_OK 0
+#define PAGANI_NOT_CONVERGED 1
+#define PAGANI_EBADARG (-1)
+#define PAGANI_EHIP (-2)
+#define PAGANI_ENOMEM (-3)
+#define PAGANI_EUNSUPPORTED (-4)
+
+/* evaluation precision of the integrand and the rule sums */
+#define PAGANI_FP64 64
+#define PAGANI_FP32 32 /* FP32 integrand, FP64 accumulation and error logic */
+
+/* integrand families of the precompiled descriptor path */
+#define PAGANI_GENZ_OSCILLATORY 1   /* cos(2 pi u_0 + sum a_i x_i)              */
+#define PAGANI_GENZ_PRODUCT_PEAK 2  /* prod 1/(a_i^-2 + (x_i - u_i)^2)          */
+#define PAGANI_GENZ_CORNER_PEAK 3   /* (1 + sum a_i x_i)^-(d+1)                 */
+#define PAGANI_GENZ_GAUSSIAN 4      /* exp(-sum a_i^2 (x_i - u_i)^2)            */
+#define PAGANI_GENZ_C0 5            /* exp(-sum a_i |x_i - u_i|)                */
+#define PAGANI_GENZ_DISCONTINUOUS 6 /* 0 if some x_i > u_i else exp(sum a_i x_i) */
+
+/* numint::integration_result, common/integration_result.hh:11-21.
+ * PAGANI fills estimate, errorest, nregions, status, iters. */
+typedef struct pagani_result {
+  double estimate;
+  double errorest;
+  size_t neval;
+  size_t nregions;
+  size_t nFinishedRegions;
+  int status; /* 0 converged, 1 not converged / memory termination */
+  int lastPhase;
+  double chi_sq;
+  size_t iters;
+} pagani_result;
+
+/* Precompiled Genz integrand descriptor (POD).  The reference's fixed
+ * functors F_k_dD (common/cuda/integrands.cuh:1303-1934) are instances. */
+typedef struct pagani_genz {
+  int family;
+  int ndim;
+  double a[PAGANI_MAX_DIM];
+  double u[PAGANI_MAX_DIM];
+} pagani_genz;
+
+/* Uniform data of one evaluation launch: the degree-9 rule (Rule.cuh),
+ * the integration volume and the launch shape.  Built by the library; an
+ * evaluation callback passes it unchanged to pagani::eval_regions. */
+typedef struct pagani_eval_consts {
+  double w[PAGANI_NSETS][PAGANI_NRULES];     /* rule weights              */
+  double scale[PAGANI_NSETS][PAGANI_NRULES]; /* null-rule scale           */
+  double norm[PAGANI_NSETS][PAGANI_NRULES];  /* null-rule norm            */
+  double lows[PAGANI_MAX_DIM];               /* integration volume        */
+  double range[PAGANI_MAX_DIM];              /* highs - lows              */
+  double jac;                                /* prod range                */
+  double lambda1, lambda2, ratio;            /* 4th-difference constants  */
+  const double* gtab; /* device: generator coordinates of points 1+4d..F-1,
+                         point-major, ndim doubles per point               */
+  int ndim, feval;
+  int set_begin[PAGANI_NSETS + 1]; /* first point index of each set       */
+  int slices;                      /* waves sharing one batch of 64 regions */
+  int slice_begin[PAGANI_MAX_SLICES + 1]; /* generic-point slice bounds     */
+} pagani_eval_consts;
+
+/* One evaluation request: regions [0, n) of the SoA pool. */
+typedef struct pagani_eval_args {
+  size_t n;
+  const double* L;
+  const double* H;
+  double* est;
+  double* err;
+  uint8_t* dim;
+  void* stream; /* hipStream_t */
+  pagani_eval_consts consts;
+} pagani_eval_args;
+
+/* An evaluation callback launches the region-evaluation kernel for one
+ * integrand on args->stream (pagani::eval_regions<ndim, IntegT> from
+ * <pagani/eval_kernel.hpp> does it) and returns 0 or a negative status. */
+typedef int (*pagani_eval_fn)(void* ctx, const pagani_eval_args* args);
+
+/* Initial region list for the Sub_regions& overload (device SoA). */
+typedef struct pagani_region_list {
+  size_t n;
+  const double* L;
+  const double* H;
+} pagani_region_list;
+
+/* Per-phase kernel time (HIP events on the workspace stream, summed over
+ * the calls since the last reset) and work counters. */
+typedef struct pagani_stats {
+  double eval_ms;
+  double refine_ms;
+  double finalize_ms;
+  double compact_ms;
+  double classify_ms;
+  double wall_ms;
+  uint64_t regions_evaluated;
+  uint64_t eval_launches;
+  uint64_t iterations;
+  uint64_t classify_passes;
+  uint64_t max_regions;
+  uint64_t integrals;
+} pagani_stats;
+
+/* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
+typedef struct pagani_class_res {
+  int pass_mem;
+  int pass_errorest_budget;
+  double threshold;
+  double percent_mem_active;
+  double range_low, range_high;
+  size_t num_active;
+  double finished_errorest;
+  double max_budget_perc_to_cover;
+  double max_active_perc;
+  int threshold_passes;
+} pagani_class_res;
+
+typedef struct pagani_ws pagani_ws;
+
+/* ---- rule tables (host only, no device needed) ------------------------------ */
+/* quad::Rule<T>::Init / Rule9Generate, Rule.cuh:24-270, 370-510: weights,
+ * null-rule scale/norm ([set*5 + rule], 45 each), lambda[5], the first point
+ * index of each set (set_begin[10]) and, when gtab is not NULL, the generator
+ * coordinates of points 1+4*ndim .. feval-1 (ndim doubles each).  Returns the
+ * number of points per region, or a negative status. */
+int pagani_rule_tables(int ndim, double* w, double* scale, double* norm, double* lambda,
+                       int* set_begin, double* gtab);
+
+/* ---- workspace ----------------------------------------------------------- */
+/* Replaces the Workspace constructor (Workspace.cuh:50) and the rule setup of
+ * Cubature_rules (PaganiUtils.cuh:52-96).  device < 0 selects the current
+ * device.  Returns NULL on failure. */
+pagani_ws* pagani_ws_create(int ndim, int device);
+/* why the calling thread's last pagani_ws_create returned NULL */
+const char* pagani_create_error(void);
+void pagani_ws_destroy(pagani_ws* ws);
+const char* pagani_ws_last_error(const pagani_ws* ws);
+int pagani_ws_ndim(const pagani_ws* ws);
+/* Total device memory assumed by the classifier's memory model
+ * (total_device_mem(), heuristic_classifier.cuh:89-94, hard-coded 16e9 in
+ * the reference).  0 selects the device's HBM size (the default). */
+int pagani_ws_set_device_mem(pagani_ws* ws, size_t total_bytes);
+size_t pagani_ws_device_mem(const pagani_ws* ws);
+/* waves per batch of 64 regions in the evaluation kernel; 0 = automatic */
+int pagani_ws_set_slices(pagani_ws* ws, int slices);
+int pagani_ws_set_timing(pagani_ws* ws, int enable);
+int pagani_ws_get_stats(const pagani_ws* ws, pagani_stats* out);
+int pagani_ws_reset_stats(pagani_ws* ws);
+/* the uniform launch data for n regions (for evaluation callbacks/tests) */
+int pagani_ws_eval_consts(pagani_ws* ws, const double* lows,
+                          const double* highs, size_t n,
+                          pagani_eval_consts* out);
+void* pagani_ws_stream(pagani_ws* ws); /* hipStream_t */
+/* Per-iteration record of the last integrate call (the reference's debug
+ * recorder, Workspace.cuh:171-242): regions evaluated, regions kept active
+ * (0 on the final iteration), cumulative estimate and error.  Copies up to
+ * cap records (any pointer may be NULL) and returns the number recorded. */
+size_t pagani_ws_trace(const pagani_ws* ws, size_t cap, uint64_t* n, uint64_t* m,
+                       double* estimate, double* errorest);
+/* The region list the last integrate call ended with (the reference leaves
+ * it in the caller's Sub_regions, Sub_region_filter.cuh:129-135,
+ * Sub_region_splitter.cuh:81-84): *n regions, device SoA pointers owned by
+ * the workspace and valid until its next call. */
+int pagani_ws_live_regions(const pagani_ws* ws, size_t* n, const double** L,
+                           const double** H);
+
+/* ---- integration ------------------------------------------------------ */
+/* Workspace::integrate(integrand, epsrel, epsabs, vol, relerr_classification)
+ * Workspace.cuh:361-494, for a precompiled Genz integrand.  lows/highs may
+ * be NULL (unit cube).  init_k = regions per axis of the initial uniform
+ * split, 0 = the reference's default (4 for d<5, 2 for d<=10). */
+int pagani_integrate_genz(pagani_ws* ws, const pagani_genz* f, int precision,
+                          double epsrel, double epsabs, const double* lows,
+                          const double* highs, int relerr_classification,
+                          size_t init_k, pagani_result* out);
+/* Same loop for any integrand through an evaluation callback.  init = NULL
+ * uses the default uniform split (Workspace.cuh:362-494); otherwise the
+ * given region list, with the Sub_regions& overload's bookkeeping
+ * (Workspace.cuh:148-358). */
+int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx,
+                        double epsrel, double epsabs, const double* lows,
+                        const double* highs, int relerr_classification,
+                        const pagani_region_list* init, size_t init_k,
+                        pagani_result* out);
+
+/* ---- phases on caller-owned device arrays --------------------------------- */
+/* Sub_regions::uniform_split / create_uniform_split, Sub_regions.cuh:11-36,
+ * 165-185: k^ndim boxes into L, H (capacity k^ndim regions). */
+int pagani_uniform_split_dev(pagani_ws* ws, size_t k, double* L, double* H);
+/* Cubature_rules::apply_cubature_integration_rules, PaganiUtils.cuh:291-343:
+ * per-region estimate, error and split axis for a Genz integrand. */
+int pagani_eval_genz_dev(pagani_ws* ws, const pagani_genz* f, int precision,
+                         size_t n, const double* L, const double* H,
+                         const double* lows, const double* highs, double* est,
+                         double* err, uint8_t* dim);
+/* two_level_errorest_and_relerr_classify / RefineError, hybrid.cuh:11-45,
+ * Phases.cuh:219-268.  parent_est = NULL is iteration 0 (no refinement, all
+ * regions active).  sums[4] (host, may be NULL) = {sum est, sum err_out,
+ * sum active*est, sum active*err_out}; *num_active (may be NULL). */
+int pagani_refine_dev(pagani_ws* ws, size_t n, const double* est,
+                      const double* err_in, double* err_out,
+                      const double* parent_est, double epsrel,
+                      int relerr_classification, uint8_t* active,
+                      double* sums, size_t* num_active);
+/* Sub_regions_filter::filter, Sub_region_filter.cuh:86-139: order-preserving
+ * compaction of the active regions (bounds at stride m, parent estimates,
+ * split axes).  Returns m in *m_out. */
+int pagani_filter_dev(pagani_ws* ws, size_t n, const double* L,
+                      const double* H, const uint8_t* active,
+                      const double* est, const double* err,
+                      const uint8_t* dim, double* L2, double* H2,
+                      double* parent_est, double* parent_err, uint8_t* dim2,
+                      size_t* m_out);
+/* Sub_region_splitter::split, Sub_region_splitter.cuh:48-87: m regions ->
+ * 2m children [left m | right m] (L2/H2 at stride 2m). */
+int pagani_split_dev(pagani_ws* ws, size_t m, const double* L,
+                     const double* H, const uint8_t* dim, double* L2,
+                     double* H2);
+/* Filter and split fused (the library's own phase): the active regions'
+ * children at stride 2m, their parents' estimates at [0, m). */
+int pagani_filter_split_dev(pagani_ws* ws, size_t n, const double* L,
+                            const double* H, const uint8_t* active,
+                            const double* est, const double* err,
+                            const uint8_t* dim, double* L2, double* H2,
+                            double* parent_est, double* parent_err,
+                            size_t* m_out);
+/* thrust_utils.cuh:16-88 equivalents (deterministic, compensated sums) */
+int pagani_sum_dev(pagani_ws* ws, const double* x, size_t n, double* out);
+int pagani_dot_dev(pagani_ws* ws, const uint8_t* flags, const double* x,
+                   size_t n, double* out);
+int pagani_minmax_dev(pagani_ws* ws, const double* x, size_t n, double* mn,
+                      double* mx);
+/* exclusive scan of 0/1 flags (Sub_region_filter.cuh:62-82 scans them):
+ * offsets[t] = number of set flags before t; *total = all set flags */
+int pagani_scan_flags_dev(pagani_ws* ws, const uint8_t* flags, size_t n,
+                          uint64_t* offsets, size_t* total);
+/* Heuristic_classifier::classify, heuristic_classifier.cuh:362-451, with
+ * the threshold passes on the device.  est_hist[3] = the classifier's last
+ * three stored estimates (oldest first); flags receives the threshold
+ * classification of the last pass. */
+int pagani_classify_dev(pagani_ws* ws, const double* err, size_t n,
+                        double epsrel, const double* est_hist,
+                        double iter_errorest, double iter_finished_errorest,
+                        double total_finished_errorest, uint8_t* flags,
+                        pagani_class_res* out);
+
+/* ---- the reference's C interop shim ---------------------------------------- */
+/* test/cuda/pagani/do_integration_from_c.cu:20-38: integrates x*y over the
+ * unit square to epsrel 1e-6, epsabs 1e-12; writes *res only when the
+ * status is 0; returns the status (1 on a NULL pointer). */
+int do_integration_from_c(double* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,222 @@
+// eval_kernel.hpp -- region evaluation with the degree-9 rule on gfx950.
+//
+// Replaces INTEGRATE_GPU_PHASE1 / INIT_REGION_POOL / SampleRegionBlock /
+// computePermutation (cuda/pagani/quad/GPUquad/Phases.cuh:301-404,
+// Sample.cuh:103-290).  Included by the library (precompiled Genz families)
+// and by user translation units compiled with hipcc (any functor with
+// `double operator()(double, ..., double)`), so one kernel template serves
+// both.
+//
+// Mapping: one LANE per region.  A wave owns 64 consecutive regions; every
+// lane walks the same sequence of rule points, so the point's generator
+// coordinates, the weights and the null-rule constants are wave-uniform and
+// come from scalar loads; the five rule sums, the fourth differences and the
+// error estimate are per-lane registers and need no cross-lane reduction.
+// For small region lists a workgroup holds `slices` waves that share one
+// batch of 64 regions and split its points; their partial sums are combined
+// through LDS in a fixed order.
+//
+// Per point the lane does d FMAs for the point map x = c - g*h (c, h: region
+// centre and width in user coordinates), one integrand call and one add into
+// the running sum of the point's symmetric set; the five rule sums are
+// formed per set as sum_s W[s][r] * S_s (all points of a set share their
+// weights, Rule.cuh:36-147).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <utility>
+
+#include "pagani.h"
+
+namespace pagani {
+namespace detail {
+
+template <class F, class Real, int D, std::size_t... I>
+__device__ __forceinline__ double call_at(F& f, const Real (&x)[D],
+                                          std::index_sequence<I...>)
+{
+  return static_cast<double>(f(x[I]...));
+}
+
+template <class F, class Real, int D>
+__device__ __forceinline__ double call(F& f, const Real (&x)[D])
+{
+  return call_at(f, x, std::make_index_sequence<D>());
+}
+
+// One set's contribution from generic (table-driven) points [a, b).
+template <int D, class F, class Real>
+__device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
+                                          int a, int b, const double (&cR)[D],
+                                          const double (&hR)[D])
+{
+  double acc = 0.0;
+#pragma unroll 1
+  for (int p = a; p < b; ++p, g += D) {
+    Real x[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      x[j] = static_cast<Real>(__builtin_fma(-g[j], hR[j], cR[j]));
+    acc += call(f, x);
+  }
+  return acc;
+}
+
+}  // namespace detail
+
+// Kernel: regions [blockIdx.x*64, +64) of an n-region SoA pool.
+// blockDim.x == 64 * C.slices.  Dynamic LDS: (slices-1)*5*64 doubles.
+template <int D, class F, class Real>
+__global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
+    eval_kernel(F f, std::size_t n, const double* __restrict__ L,
+                const double* __restrict__ H, double* __restrict__ est,
+                double* __restrict__ err, std::uint8_t* __restrict__ dim,
+                const pagani_eval_consts C)
+{
+  extern __shared__ double part[];
+  const int lane = threadIdx.x & 63;
+  // wave index is uniform across the wave: keep it (and everything derived
+  // from it, the slice bounds and the point loop) in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
+  const bool live = t0 < n;
+  const std::size_t t = live ? t0 : n - 1;
+
+  // Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
+  // width in user coordinates.
+  double cR[D], hR[D];
+  double vol = 1.0, maxr = 0.0;
+  int maxdim = 0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const double lo = L[static_cast<std::size_t>(j) * n + t];
+    const double up = lo + H[static_cast<std::size_t>(j) * n + t];
+    const double r = up - lo;
+    vol *= r;
+    if (r > maxr) {
+      maxr = r;
+      maxdim = j;
+    }
+    cR[j] = C.lows[j] + (0.5 * (lo + up)) * C.range[j];
+    hR[j] = r * C.range[j];
+  }
+
+  double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int bisect = maxdim;
+
+  if (wave == 0) {
+    // Centre and the +-lambda1, +-lambda2 axis points (sets 0-2), whose
+    // values give the fourth difference per axis, Sample.cuh:176-218.
+    Real xc[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) xc[j] = static_cast<Real>(cR[j]);
+    const double f0 = detail::call(f, xc);
+    const double base = (f0 * C.jac) * 2 * (1 - C.ratio);
+    double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Real x[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) x[k] = xc[k];
+      x[j] = static_cast<Real>(__builtin_fma(-C.lambda1, hR[j], cR[j]));
+      const double fa = detail::call(f, x);
+      x[j] = static_cast<Real>(__builtin_fma(C.lambda1, hR[j], cR[j]));
+      const double fb = detail::call(f, x);
+      x[j] = static_cast<Real>(__builtin_fma(-C.lambda2, hR[j], cR[j]));
+      const double fc = detail::call(f, x);
+      x[j] = static_cast<Real>(__builtin_fma(C.lambda2, hR[j], cR[j]));
+      const double fd = detail::call(f, x);
+      s1 += fa + fb;
+      s2 += fc + fd;
+      const double diff = __builtin_fabs(
+          base + C.ratio * (fa * C.jac + fb * C.jac) - (fc * C.jac + fd * C.jac));
+      if (diff > maxdiff) {
+        maxdiff = diff;
+        bisect = j;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r)
+      rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
+  }
+
+  // Sets 3..8 from the generator table, this wave's slice of the points.
+  const int pa = C.slice_begin[wave];
+  const int pb = C.slice_begin[wave + 1];
+  const int p0 = C.set_begin[3];
+#pragma unroll
+  for (int s = 3; s < PAGANI_NSETS; ++s) {
+    const int a = pa > C.set_begin[s] ? pa : C.set_begin[s];
+    const int b = pb < C.set_begin[s + 1] ? pb : C.set_begin[s + 1];
+    if (a < b) {
+      const double acc = detail::set_sum<D, F, Real>(
+          f, C.gtab + static_cast<std::size_t>(a - p0) * D, a, b, cR, hR);
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r)
+        rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
+    }
+  }
+
+  if (C.slices > 1) {
+    if (wave > 0) {
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r)
+        part[((wave - 1) * PAGANI_NRULES + r) * 64 + lane] = rs[r];
+    }
+    __syncthreads();
+    if (wave > 0) return;
+    for (int w = 1; w < C.slices; ++w) {
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r)
+        rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
+    }
+  }
+
+  // Null-rule error estimate, Sample.cuh:261-289.
+  double s[PAGANI_NRULES];
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r) s[r] = rs[r] * C.jac;
+  double e1 = 0.0, e2 = 0.0, e3 = 0.0;
+#pragma unroll
+  for (int k = 0; k < PAGANI_NSETS; ++k) {
+    e1 = __builtin_fmax(e1, __builtin_fabs(s[2] + C.scale[k][1] * s[1]) * C.norm[k][1]);
+    e2 = __builtin_fmax(e2, __builtin_fabs(s[3] + C.scale[k][2] * s[2]) * C.norm[k][2]);
+    e3 = __builtin_fmax(e3, __builtin_fabs(s[4] + C.scale[k][3] * s[3]) * C.norm[k][3]);
+  }
+  const double avg = vol * s[0];
+  const double er =
+      vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3)
+                 ? e1
+                 : 5.0 * __builtin_fmax(__builtin_fmax(e1, e2), e3));
+  if (live) {
+    est[t] = avg;
+    err[t] = er;
+    dim[t] = static_cast<std::uint8_t>(bisect);
+  }
+}
+
+// Launch the evaluation of args->n regions for integrand f.  Returns 0 or
+// PAGANI_EHIP.  Real is the integrand's arithmetic type (double or float).
+template <int D, class F, class Real = double>
+int eval_regions(const F& f, const pagani_eval_args* args)
+{
+  static_assert(D >= PAGANI_MIN_DIM && D <= PAGANI_MAX_DIM, "unsupported ndim");
+  if (args->consts.ndim != D) return PAGANI_EBADARG;
+  if (args->n == 0) return PAGANI_OK;
+  const int slices = args->consts.slices;
+  if (slices < 1 || slices > PAGANI_MAX_SLICES) return PAGANI_EBADARG;
+  const std::size_t blocks = (args->n + 63) / 64;
+  const std::size_t lds =
+      slices > 1 ? static_cast<std::size_t>(slices - 1) * PAGANI_NRULES * 64 * sizeof(double)
+                 : 0;
+  hipLaunchKernelGGL((eval_kernel<D, F, Real>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(64 * slices), lds, static_cast<hipStream_t>(args->stream), f,
+                     args->n, args->L, args->H, args->est, args->err, args->dim,
+                     args->consts);
+  return hipGetLastError() == hipSuccess ? PAGANI_OK : PAGANI_EHIP;
+}
+
+}  // namespace pagani

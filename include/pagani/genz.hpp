@@ -1,0 +1,136 @@
+// genz.hpp -- the six Genz test families as device functors.
+//
+// The reference's fixed-parameter functors F_1..F_6 (common/cuda/
+// integrands.cuh:1303-1934) are instances of these families with given
+// (a, u).  Real = double for the FP64 path, float for the mixed-precision
+// path (FP32 integrand, FP64 accumulation).  Parameters are pre-processed on
+// the host (a^2, 1/a^2, 2*pi*u_0) so the device does no divisions by a.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "pagani.h"
+
+namespace pagani {
+namespace math {
+__device__ __forceinline__ double exp_(double x) { return ::exp(x); }
+__device__ __forceinline__ float exp_(float x) { return ::expf(x); }
+__device__ __forceinline__ double cos_(double x) { return ::cos(x); }
+__device__ __forceinline__ float cos_(float x) { return ::cosf(x); }
+__device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
+}  // namespace math
+
+// cos(2 pi u_0 + sum a_i x_i)
+template <class Real, int D>
+struct Oscillatory {
+  Real a[D];
+  Real phase;  // 2 pi u_0
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    Real s = phase;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += a[i] * x[i];
+    return math::cos_(s);
+  }
+};
+
+// prod 1 / (a_i^-2 + (x_i - u_i)^2), evaluated as one division
+template <class Real, int D>
+struct ProductPeak {
+  Real ia2[D];  // 1 / a_i^2
+  Real u[D];
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    Real den = 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const Real t = x[i] - u[i];
+      den *= ia2[i] + t * t;
+    }
+    return Real(1) / den;
+  }
+};
+
+// (1 + sum a_i x_i)^-(d+1), the power by repeated multiplication
+template <class Real, int D>
+struct CornerPeak {
+  Real a[D];
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    Real s = 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += a[i] * x[i];
+    Real p = s;
+#pragma unroll
+    for (int i = 0; i < D; ++i) p *= s;
+    return Real(1) / p;
+  }
+};
+
+// exp(-sum a_i^2 (x_i - u_i)^2)
+template <class Real, int D>
+struct Gaussian {
+  Real a2[D];  // a_i^2
+  Real u[D];
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    const Real t0 = x[0] - u[0];
+    Real s = a2[0] * (t0 * t0);
+#pragma unroll
+    for (int i = 1; i < D; ++i) {
+      const Real t = x[i] - u[i];
+      s += a2[i] * (t * t);
+    }
+    return math::exp_(-s);
+  }
+};
+
+// exp(-sum a_i |x_i - u_i|)
+template <class Real, int D>
+struct C0 {
+  Real a[D];
+  Real u[D];
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    Real s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += a[i] * math::fabs_(x[i] - u[i]);
+    return math::exp_(-s);
+  }
+};
+
+// 0 if any x_i > u_i, else exp(sum a_i x_i)
+template <class Real, int D>
+struct Discontinuous {
+  Real a[D];
+  Real u[D];
+  template <class... X>
+  __device__ Real operator()(X... xs) const
+  {
+    const Real x[D] = {static_cast<Real>(xs)...};
+    bool out = false;
+    Real s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      out = out || (x[i] > u[i]);
+      s += a[i] * x[i];
+    }
+    const Real v = math::exp_(s);
+    return out ? Real(0) : v;
+  }
+};
+
+}  // namespace pagani

@@ -1,0 +1,240 @@
+// reftests.hip -- the reference's test integrands driven through the C++
+// drop-in header (include/pagani/workspace.hpp), exported to the pytest
+// suite through extern "C".  Built into tests/cxx/libreftests.so.
+//
+// Integrands (formulas as in the reference's tests):
+//   PTest/NTest/ZTest            Easy_Integrals.cu:25-52, RegionSampling.cu
+//   F_2_6D F_3_8D F_4_5D F_5_8D F_6_6D   common/cuda/integrands.cuh
+//   BoxIntegral8_15/22/25        cuda/pagani/demos/function.cuh:113-172
+//   Fun6                         accuracy_improves_with_epsrel.cu:13-20
+//   NaN_Integral                 NanIntegral.cu:16-24
+//   XY                           do_integration_from_c.cu:10-16
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <vector>
+
+#include "pagani/workspace.hpp"
+
+namespace {
+
+class PTest {
+ public:
+  __device__ __host__ double operator()(double, double) { return 15.37; }
+};
+class NTest {
+ public:
+  __device__ __host__ double operator()(double, double) { return -15.37; }
+};
+class ZTest {
+ public:
+  __device__ __host__ double operator()(double, double) { return 0.; }
+};
+struct XY {
+  __device__ double operator()(double x, double y) const { return x * y; }
+};
+class NaN_Integral {
+ public:
+  __device__ double operator()(double, double) { return NAN; }
+};
+
+class F_2_6D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double w, double v, double u)
+  {
+    const double a = 50., b = .5;
+    const double xs[6] = {x, y, z, w, v, u};
+    double val = 1.;
+    for (double t : xs) val *= 1. / ((1. / (a * a)) + (t - b) * (t - b));
+    return val;
+  }
+};
+
+class F_3_8D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double w, double v, double u,
+                                        double t, double s)
+  {
+    return pow(1. + 8. * s + 7. * t + 6. * u + 5. * v + 4. * w + 3. * x + 2. * y + z, -9.);
+  }
+};
+
+class F_4_5D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double w, double v)
+  {
+    const double beta = .5, a2 = 25. * 25.;
+    const double xs[5] = {x, y, z, w, v};
+    double s = 0.;
+    for (double t : xs) s += a2 * ((t - beta) * (t - beta));
+    return exp(-1.0 * s);
+  }
+};
+
+class F_5_8D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double k, double m, double n,
+                                        double p, double q)
+  {
+    const double beta = .5;
+    const double xs[8] = {x, y, z, k, m, n, p, q};
+    double t1 = 0.;
+    for (double t : xs) t1 -= 10. * fabs(t - beta);
+    return exp(t1);
+  }
+};
+
+class F_6_6D {
+ public:
+  __device__ __host__ double operator()(double y, double v, double x, double w, double z, double u)
+  {
+    if (y > .9 || v > .8 || x > .7 || w > .6 || z > .5 || u > .4) return 0.;
+    return exp(10. * y + 9. * v + 8. * x + 7. * w + 6. * z + 5. * u);
+  }
+};
+
+template <int S>
+class BoxIntegral8 {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double k, double l, double m,
+                                        double n, double o)
+  {
+    const double sum = x * x + y * y + z * z + k * k + l * l + m * m + n * n + o * o;
+    return pow(sum, S / 2.);
+  }
+};
+
+struct Fun6 {
+  __device__ __host__ double operator()(double u, double v, double w, double x, double y, double z)
+  {
+    return (12.0 / (7.0 - 6 * log(2.0) * log(2.0) + log(64.0))) *
+           (u * v + (pow(w, y) * x * y) / (1 + u) + z * z);
+  }
+};
+
+template <int D, class F>
+int run(double epsrel, double epsabs, pagani_result* out)
+{
+  Workspace<double, D> pagani;
+  quad::Volume<double, D> vol;
+  const numint::integration_result r = pagani.integrate(F{}, epsrel, epsabs, vol);
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->neval = r.neval;
+  out->nregions = r.nregions;
+  out->nFinishedRegions = r.nFinishedRegions;
+  out->status = r.status;
+  out->lastPhase = r.lastPhase;
+  out->chi_sq = r.chi_sq;
+  out->iters = r.iters;
+  return 0;
+}
+
+template <int D, class F>
+int run_split(int k, double epsrel, double epsabs, std::size_t mem_total, pagani_result* out,
+              std::size_t* final_n)
+{
+  Workspace<double, D> pagani;
+  if (mem_total) pagani_ws_set_device_mem(pagani.handle(), mem_total);
+  quad::Volume<double, D> vol;
+  Sub_regions<double, D> regions(static_cast<std::size_t>(k));
+  const numint::integration_result r = pagani.integrate(F{}, regions, epsrel, epsabs, vol);
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->nregions = r.nregions;
+  out->status = r.status;
+  out->iters = r.iters;
+  *final_n = regions.size;
+  return 0;
+}
+
+// RegionSampling.cu: evaluate every region of a k^D uniform split once.
+template <int D, class F>
+int sample(int k, double* est_host, double* err_host, std::size_t cap)
+{
+  Workspace<double, D> pagani;
+  Sub_regions<double, D> regions(static_cast<std::size_t>(k));
+  const std::size_t n = regions.size;
+  if (n > cap) return -1;
+  double *est = nullptr, *err = nullptr;
+  std::uint8_t* dim = nullptr;
+  if (hipMalloc(&est, n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&err, n * sizeof(double)) != hipSuccess || hipMalloc(&dim, n) != hipSuccess)
+    return -3;
+  pagani_eval_args args;
+  args.n = n;
+  args.L = regions.dLeftCoord;
+  args.H = regions.dLength;
+  args.est = est;
+  args.err = err;
+  args.dim = dim;
+  args.stream = pagani_ws_stream(pagani.handle());
+  int rc = pagani_ws_eval_consts(pagani.handle(), nullptr, nullptr, n, &args.consts);
+  if (rc == 0) rc = pagani::eval_regions<D, F, double>(F{}, &args);
+  if (rc == 0 && hipStreamSynchronize(static_cast<hipStream_t>(args.stream)) != hipSuccess) rc = -2;
+  if (rc == 0 && (hipMemcpy(est_host, est, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(err_host, err, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+    rc = -2;
+  (void)hipFree(est);
+  (void)hipFree(err);
+  (void)hipFree(dim);
+  return rc == 0 ? static_cast<int>(n) : rc;
+}
+
+}  // namespace
+
+extern "C" int reftest_integrate(const char* name, double epsrel, double epsabs,
+                                 pagani_result* out)
+{
+  if (!name || !out) return -1;
+  std::memset(out, 0, sizeof(*out));
+  try {
+    if (!std::strcmp(name, "PTest")) return run<2, PTest>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "NTest")) return run<2, NTest>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "ZTest")) return run<2, ZTest>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "XY")) return run<2, XY>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "NaN_Integral")) return run<2, NaN_Integral>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "F_2_6D")) return run<6, F_2_6D>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "F_3_8D")) return run<8, F_3_8D>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "F_4_5D")) return run<5, F_4_5D>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "F_5_8D")) return run<8, F_5_8D>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "F_6_6D")) return run<6, F_6_6D>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "BoxIntegral8_15")) return run<8, BoxIntegral8<15>>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "BoxIntegral8_22")) return run<8, BoxIntegral8<22>>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "BoxIntegral8_25")) return run<8, BoxIntegral8<25>>(epsrel, epsabs, out);
+    if (!std::strcmp(name, "Fun6")) return run<6, Fun6>(epsrel, epsabs, out);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}
+
+extern "C" int reftest_integrate_split(const char* name, int k, double epsrel, double epsabs,
+                                       std::size_t mem_total, pagani_result* out,
+                                       std::size_t* final_n)
+{
+  if (!name || !out || !final_n || k < 1) return -1;
+  std::memset(out, 0, sizeof(*out));
+  try {
+    if (!std::strcmp(name, "XY"))
+      return run_split<2, XY>(k, epsrel, epsabs, mem_total, out, final_n);
+    if (!std::strcmp(name, "F_4_5D"))
+      return run_split<5, F_4_5D>(k, epsrel, epsabs, mem_total, out, final_n);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}
+
+extern "C" int reftest_sample(const char* name, int k, double* est, double* err, std::size_t cap)
+{
+  if (!name || !est || !err || k < 1) return -1;
+  try {
+    if (!std::strcmp(name, "PTest")) return sample<2, PTest>(k, est, err, cap);
+    if (!std::strcmp(name, "NTest")) return sample<2, NTest>(k, est, err, cap);
+    if (!std::strcmp(name, "ZTest")) return sample<2, ZTest>(k, est, err, cap);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}

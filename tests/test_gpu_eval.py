@@ -1,0 +1,209 @@
+"""Per-region rule evaluation on the GPU vs the CPU oracle.
+
+Replaces the reference's RegionSampling.cu checks (constant integrands) with
+a direct comparison of every region's estimate, error estimate and split
+axis against the oracle's restatement of Sample.cuh / Phases.cuh, for all
+six Genz families and d = 2..8.
+
+Tolerances (written here, used below).  The GPU forms each rule sum per
+symmetric set (sum_s W[s][r] * sum_{p in s} f_p) while the oracle accumulates
+w * f point by point, so the sums round differently.  A_r = sum_p |w_r f_p|
+(reported per region by the oracle) bounds that rounding:
+  estimate:  |gpu - oracle| <= 1e-13 * vol * A_0
+  error:     the null-rule errors e_1..e_3 are re-derived with every rule sum
+             perturbed by 1e-12 * A_r (oracle_api.err_bounds); the GPU error
+             must match the oracle's branch of Sample.cuh:282-288 within that
+             bound, or either branch where the branch condition itself lies
+             within the rounding.
+  split axis: identical wherever the two largest fourth differences differ
+             by more than 1e-9 relative (closer ones are ties decided by
+             rounding)
+  FP32 integrand: estimate within 2e-5 * vol * A_0 (+1e-35 * vol for values
+             below FP32's normal range) of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle_api as O
+
+pytestmark = pytest.mark.gpu
+
+FAMILIES = ["oscillatory", "product_peak", "corner_peak", "gaussian", "c0", "discontinuous"]
+
+
+def genz_params(family, d, seed):
+    """Seeded Genz parameters (mt19937_64 draw order a_1..a_d, u_1..u_d)."""
+    rng = np.random.default_rng(seed)
+    if family == "oscillatory":
+        a = 1.0 + 0.5 * rng.random(d)
+        u = rng.random(d)
+    elif family == "gaussian":
+        a = 10.0 * (1 + 0.05 * (2 * rng.random(d) - 1))
+        u = 0.2 + 0.6 * rng.random(d)
+    elif family == "corner_peak":
+        a = 0.5 + rng.random(d)
+        u = np.zeros(d)
+    elif family == "discontinuous":
+        a = 5.0 + 10.0 * rng.random(d) * 0.3
+        u = 0.3 + 0.4 * rng.random(d)
+    else:
+        a = 5.0 + 10.0 * rng.random(d)
+        u = rng.random(d)
+    return a, u
+
+
+def random_regions(d, n, seed):
+    rng = np.random.default_rng(seed)
+    H = 10.0 ** rng.uniform(-3, -0.3, size=(d, n))
+    L = rng.random((d, n)) * (1.0 - H)
+    return np.ascontiguousarray(L.ravel()), np.ascontiguousarray(H.ravel())
+
+
+def to_dev(a, dev, dtype=None):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def gpu_eval(ws, dev, family, a, u, L, H, n, precision=64, lows=None, highs=None):
+    import torch
+    Ld, Hd = to_dev(L, dev), to_dev(H, dev)
+    est = torch.empty(n, dtype=torch.float64, device=dev)
+    err = torch.empty_like(est)
+    dim = torch.empty(n, dtype=torch.uint8, device=dev)
+    ws.eval_genz(family, a, u, n, Ld, Hd, est, err, dim, lows=lows, highs=highs,
+                 precision=precision)
+    return est.cpu().numpy(), err.cpu().numpy(), dim.cpu().numpy().astype(np.int32)
+
+
+def check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs):
+    scale = O.est_scale(rs)
+    bad = np.abs(ge - oe) > 1e-13 * scale + 1e-300
+    assert not bad.any(), (
+        f"{family} d={d}: {bad.sum()} estimates off; worst "
+        f"{np.max(np.abs(ge - oe) / np.maximum(scale, 1e-300)):.3e} of scale")
+    va, vb, ta, tb, cond, amb = O.err_bounds(d, rs)
+    # the oracle's own error must be its branch value (checks the record)
+    assert np.allclose(oerr, np.where(cond, va, vb), rtol=1e-15, atol=0)
+    ok_a = np.abs(gerr - va) <= ta + 1e-300
+    ok_b = np.abs(gerr - vb) <= tb + 1e-300
+    ok = np.where(amb, ok_a | ok_b, np.where(cond, ok_a, ok_b))
+    assert ok.all(), (f"{family} d={d}: {(~ok).sum()} error estimates off "
+                      f"({amb.sum()} ambiguous branches)")
+    srt = np.sort(fd, axis=1)
+    gap = (srt[:, -1] - srt[:, -2]) / np.maximum(srt[:, -1], 1e-300)
+    clear = gap > 1e-9
+    mism = (gdim != odim) & clear
+    assert not mism.any(), f"{family} d={d}: {mism.sum()} split axes differ on non-ties"
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+@pytest.mark.parametrize("d", [2, 3, 5, 8])
+def test_eval_random_regions_fp64(pg, gpu, family, d):
+    n = 3000 if d < 8 else 1000
+    a, u = genz_params(family, d, seed=100 + d)
+    L, H = random_regions(d, n, seed=7 * d)
+    with pg.Workspace(d, 0) as ws:
+        ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n)
+    prm = O.params(a, u)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, family, prm, L, H, n, want_extra=True)
+    check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+@pytest.mark.parametrize("d", [4, 6, 7])
+def test_eval_all_families_remaining_dims(pg, gpu, d):
+    n = 2000
+    L, H = random_regions(d, n, seed=11 * d)
+    with pg.Workspace(d, 0) as ws:
+        for family in FAMILIES:
+            a, u = genz_params(family, d, seed=200 + d)
+            ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n)
+            oe, oerr, odim, fd, rs = O.eval_regions(d, family, O.params(a, u), L, H, n,
+                                                    want_extra=True)
+            check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+@pytest.mark.parametrize("slices", [1, 2, 4, 8])
+def test_eval_slices_agree(pg, gpu, slices):
+    """Every launch shape (waves sharing a region batch) gives the oracle's sums."""
+    d, n = 8, 777
+    a, u = genz_params("gaussian", d, seed=5)
+    L, H = random_regions(d, n, seed=3)
+    with pg.Workspace(d, 0) as ws:
+        ws.set_slices(slices)
+        ge, gerr, gdim = gpu_eval(ws, gpu, "gaussian", a, u, L, H, n)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, "gaussian", O.params(a, u), L, H, n,
+                                            want_extra=True)
+    check_region_parity(d, "gaussian", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+def test_eval_uniform_split_8d_config(pg, gpu):
+    """The 8D Gaussian workload's first iteration: 4^8 = 65,536 regions."""
+    import torch
+    d, k = 8, 4
+    n = k ** d
+    a, u = genz_params("gaussian", d, seed=2024)
+    with pg.Workspace(d, 0) as ws:
+        Ld = torch.empty(n * d, dtype=torch.float64, device=gpu)
+        Hd = torch.empty_like(Ld)
+        ws.uniform_split(k, Ld, Hd)
+        L, H = Ld.cpu().numpy(), Hd.cpu().numpy()
+        Lo, Ho, _ = O.uniform_split(d, k)
+        assert np.array_equal(L, Lo) and np.array_equal(H, Ho)
+        ge, gerr, gdim = gpu_eval(ws, gpu, "gaussian", a, u, L, H, n)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, "gaussian", O.params(a, u), L, H, n,
+                                            want_extra=True)
+    check_region_parity(d, "gaussian", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+def test_eval_general_volume(pg, gpu):
+    """Non-unit integration box: the Jacobian and the affine map."""
+    d, n = 5, 500
+    lows, highs = [-1.0, 0.0, 2.0, -0.5, 0.25], [1.0, 3.0, 2.5, 0.5, 0.75]
+    a, u = genz_params("c0", d, seed=9)
+    L, H = random_regions(d, n, seed=19)
+    with pg.Workspace(d, 0) as ws:
+        ge, gerr, gdim = gpu_eval(ws, gpu, "c0", a, u, L, H, n, lows=lows, highs=highs)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, "c0", O.params(a, u), L, H, n,
+                                            lows=np.array(lows), highs=np.array(highs),
+                                            want_extra=True)
+    check_region_parity(d, "c0", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+def test_eval_fp32_integrand(pg, gpu, family):
+    """Mixed precision: FP32 integrand, FP64 sums, vs the all-FP64 oracle."""
+    d, n = 6, 2000
+    a, u = genz_params(family, d, seed=31)
+    L, H = random_regions(d, n, seed=37)
+    with pg.Workspace(d, 0) as ws:
+        ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n, precision=32)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, family, O.params(a, u), L, H, n,
+                                            want_extra=True)
+    scale = O.est_scale(rs) + 1e-35 * rs[:, 10]
+    if family == "discontinuous":
+        # FP32 coordinates can fall on the other side of a step: compare only
+        # regions whose points all lie away from the discontinuity
+        far = np.ones(n, dtype=bool)
+        Lm, Hm = L.reshape(d, n), H.reshape(d, n)
+        for j in range(d):
+            far &= (Lm[j] > u[j] + 1e-5) | (Lm[j] + Hm[j] < u[j] - 1e-5)
+        ge, oe, scale = ge[far], oe[far], scale[far]
+    assert np.all(np.abs(ge - oe) <= 2e-5 * scale + 1e-300)
+
+
+def test_eval_constant_kat(pg, gpu):
+    """RegionSampling.cu: constant c over a k^2 split gives c/n per region (1e-6)."""
+    import ctypes
+    import os
+    lib_path = os.path.join(os.path.dirname(__file__), "cxx", "libreftests.so")
+    lib = ctypes.CDLL(lib_path)
+    lib.reftest_sample.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_size_t]
+    for name, c in [(b"PTest", 15.37), (b"NTest", -15.37), (b"ZTest", 0.0)]:
+        for k in range(5, 15):
+            n = k * k
+            est = np.zeros(n)
+            err = np.zeros(n)
+            got = lib.reftest_sample(name, k, est.ctypes.data, err.ctypes.data, n)
+            assert got == n
+            assert np.allclose(est, c / n, rtol=1e-6, atol=1e-300)
