@@ -467,11 +467,14 @@ static void eval_one(const rule_t* R, int fam, const double* prm,
   const double ratio = q * q;
   const double base = fv[0] * 2 * (1 - ratio);
   double maxdiff = 0.;
+  double fdmag = 0.; /* magnitude of the terms of the largest difference */
   int bis = maxdim;
   for (int j = 0; j < d; ++j) {
     const double fp1 = fv[1 + 2 * j], fm1 = fv[2 + 2 * j];
     const double fp2 = fv[1 + 2 * d + 2 * j], fm2 = fv[2 + 2 * d + 2 * j];
     const double fd = fabs(base + ratio * (fp1 + fm1) - (fp2 + fm2));
+    const double mag = fabs(base) + ratio * (fabs(fp1) + fabs(fm1)) + fabs(fp2) + fabs(fm2);
+    if (mag > fdmag) fdmag = mag;
     if (fd_out) fd_out[j] = fd;
     if (fd > maxdiff) {
       maxdiff = fd;
@@ -495,6 +498,7 @@ static void eval_one(const rule_t* R, int fam, const double* prm,
     rs_out[11] = e[1];
     rs_out[12] = e[2];
     rs_out[13] = e[3];
+    rs_out[14] = fdmag;
   }
   *avg = vol * s[0];
   *err = vol * ((5. * e[1] <= e[2] && 5. * e[2] <= e[3])

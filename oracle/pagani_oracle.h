@@ -70,8 +70,10 @@ double orc_exact(int family, int ndim, const double* params);
  *   [0..4]  the raw rule sums s_0..s_4 (Jacobian included, volume not)
  *   [5..9]  A_r = sum_p |w_r f_p|, the magnitudes that bound their rounding
  *   [10]    the region volume
- *   [11..13] the null-rule errors e_1, e_2, e_3 before the final branch */
-#define ORC_RSUM 14
+ *   [11..13] the null-rule errors e_1, e_2, e_3 before the final branch
+ *   [14]    the largest magnitude of the terms of a fourth difference,
+ *           which bounds its rounding */
+#define ORC_RSUM 15
 int orc_eval_regions(int ndim, int family, const double* params,
                      const double* lows, const double* highs, size_t n,
                      const double* L, const double* H, double* est,

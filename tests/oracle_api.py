@@ -30,8 +30,8 @@ FAM = {
 }
 
 # per-region extra record of orc_eval_regions: s_0..s_4, A_0..A_4 (sum |w_r f|),
-# volume, e_1, e_2, e_3
-RSUM = 14
+# volume, e_1, e_2, e_3, fourth-difference magnitude
+RSUM = 15
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)

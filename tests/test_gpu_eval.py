@@ -16,8 +16,8 @@ w * f point by point, so the sums round differently.  A_r = sum_p |w_r f_p|
              bound, or either branch where the branch condition itself lies
              within the rounding.
   split axis: identical wherever the two largest fourth differences differ
-             by more than 1e-9 relative (closer ones are ties decided by
-             rounding)
+             by more than 1e-12 times the magnitude of their terms (a fourth
+             difference cancels; closer ones are ties decided by rounding)
   FP32 integrand: estimate within 2e-5 * vol * A_0 (+1e-35 * vol for values
              below FP32's normal range) of the FP64 oracle.
 """
@@ -90,8 +90,7 @@ def check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs):
     assert ok.all(), (f"{family} d={d}: {(~ok).sum()} error estimates off "
                       f"({amb.sum()} ambiguous branches)")
     srt = np.sort(fd, axis=1)
-    gap = (srt[:, -1] - srt[:, -2]) / np.maximum(srt[:, -1], 1e-300)
-    clear = gap > 1e-9
+    clear = (srt[:, -1] - srt[:, -2]) > 1e-12 * rs[:, 14]
     mism = (gdim != odim) & clear
     assert not mism.any(), f"{family} d={d}: {mism.sum()} split axes differ on non-ties"
 
