@@ -1,17 +1,21 @@
-"""Benchmark: PAGANI on one MI355X, 8D Genz Gaussian to epsrel 1e-6.
+"""Benchmark: PAGANI on one MI355X.
 
-One step = one full integration to convergence of the seeded 8D Gaussian
+Default (--config 2, the headline line): the seeded 8D Genz Gaussian
 f(x) = exp(-sum a_i^2 (x_i - u_i)^2), a_i = 10 (1 + 0.05 (2U - 1)),
-u_i = 0.2 + 0.6 U (std::mt19937_64, U = (x >> 11) 2^-53), epsrel 1e-6,
-initial uniform split 4 per axis (65,536 regions), FP64.  Inputs are
-synthetic and resident on the device; the whole loop (evaluation,
-refinement, classification, filtering, splitting) is inside the timed
-region.
+u_i = 0.2 + 0.6 U (std::mt19937_64, U = (x >> 11) 2^-53), integrated to
+epsrel 1e-6 from an initial uniform split of 4 per axis (65,536 regions),
+FP64.  One step = one full integration to convergence; inputs are synthetic
+and resident on the device; the whole loop (evaluation, refinement,
+classification, filtering, splitting) is inside the timed region.
+value = regions evaluated per second.
 
-value = regions evaluated per second (all iterations, all steps).
-Also reported: integrals/s, the FP32-integrand (mixed precision) run of the
-same workload, the evaluation kernel's roofline (algorithmic FP64 FLOPs /
-its HIP-event time vs 78.6 TFLOP/s) and the CPU oracle on the host cores.
+Other configurations (each prints its own JSON line):
+  --config 1  5D oscillatory, epsrel 1e-3: the sequential Cuhre-equivalent on
+              one host core (max 1e6 evaluations) beside the GPU
+  --config 3  mixed precision: 6D corner peak and product peak, epsrel
+              1e-3..1e-7, FP32 integrand vs FP64, error vs analytic per run
+  --config 5  memory-bound: 8D cos(sum (i+1) x_i) and 8D discontinuous at
+              epsrel 1e-8 with the library's own memory model
 """
 import argparse
 import json
@@ -24,7 +28,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "regions/s + integrals/s to epsrel 1e-6 (8D Genz), mixed vs FP64, 1 GPU"
 FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
@@ -37,19 +40,29 @@ def _sync():
         pass
 
 
-def run_gpu(pg, W, fam, a, u, epsrel, steps, warmup, precision, init_k=4):
+def noneval_bytes(trace, d):
+    """Algorithmic HBM bytes of refine + compaction + split over a run:
+    n (16d + 33) + m (32d + 16) per iteration."""
+    return sum(int(n) * (16 * d + 33) + int(m) * (32 * d + 16)
+               for n, m in zip(trace["n"], trace["m"]))
+
+
+def run_gpu(pg, fam, a, u, epsrel, steps, warmup, precision, init_k=0, mem_model=None,
+            epsabs=1e-20):
     d = len(a)
     ws = pg.Workspace(d, 0)
     ws.set_timing(True)
+    if mem_model is not None:
+        ws.set_mem_model(mem_model)
     for _ in range(warmup):
-        ws.integrate_genz(fam, a, u, epsrel=epsrel, epsabs=1e-20, precision=precision,
+        ws.integrate_genz(fam, a, u, epsrel=epsrel, epsabs=epsabs, precision=precision,
                           init_k=init_k)
     _sync()
     ws.reset_stats()
     t0 = time.perf_counter()
     res = None
     for _ in range(steps):
-        res = ws.integrate_genz(fam, a, u, epsrel=epsrel, epsabs=1e-20, precision=precision,
+        res = ws.integrate_genz(fam, a, u, epsrel=epsrel, epsabs=epsabs, precision=precision,
                                 init_k=init_k)
     _sync()
     dt = time.perf_counter() - t0
@@ -59,11 +72,37 @@ def run_gpu(pg, W, fam, a, u, epsrel, steps, warmup, precision, init_k=4):
     return res, dt, st, tr
 
 
+def phase_report(W, fam, d, res, dt, st, tr, steps):
+    regions = st["regions_evaluated"]
+    eval_s = st["eval_ms"] / 1e3
+    flops = W.eval_flops_per_region(fam, d) * regions
+    ne_bytes = noneval_bytes(tr, d) * steps
+    ne_s = (st["refine_ms"] + st["compact_ms"]) / 1e3
+    return {
+        "regions_per_s": regions / dt,
+        "integrals_per_s": steps / dt,
+        "ms_per_integral": dt / steps * 1e3,
+        "estimate": res.estimate,
+        "errorest": res.errorest,
+        "status": res.status,
+        "nregions": res.nregions,
+        "iters": res.iters,
+        "regions_evaluated_per_integral": regions // steps,
+        "max_regions": st["max_regions"],
+        "eval_tflops": flops / eval_s / 1e12 if eval_s > 0 else None,
+        "eval_frac_fp64_peak": flops / eval_s / 1e12 / FP64_PEAK_TFLOPS if eval_s > 0 else None,
+        "noneval_gbs": ne_bytes / ne_s / 1e9 if ne_s > 0 else None,
+        "noneval_frac_hbm": ne_bytes / ne_s / 1e9 / HBM_PEAK_GBS if ne_s > 0 else None,
+        "phase_ms_per_integral": {k: st[k] / steps for k in
+                                  ("eval_ms", "refine_ms", "compact_ms", "classify_ms")},
+        "classify_passes_per_integral": st["classify_passes"] / steps,
+    }
+
+
 def cpu_baseline(fam, a, u, budget_s):
     """The oracle's region evaluation on the host cores: repeated passes over
     the workload's first iteration (the 4^8 uniform split) for ~budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
     import oracle_api as O
     d = len(a)
     L, H, n = O.uniform_split(d, 4)
@@ -84,37 +123,18 @@ def cpu_baseline(fam, a, u, budget_s):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--epsrel", type=float, default=1e-6)
-    ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mixed", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(each integral runs on one device); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    import gpuintegration_amd as pg
-    from gpuintegration_amd import workloads as W
-
+def config2(args, pg, W):
     fam, a, u = W.gaussian_8d(args.seed)
     d = len(a)
     exact = W.gaussian_exact(a, u)
-    res, dt, st, tr = run_gpu(pg, W, fam, a, u, args.epsrel, args.steps, args.warmup, pg.FP64)
+    res, dt, st, tr = run_gpu(pg, fam, a, u, args.epsrel, args.steps, args.warmup, pg.FP64,
+                              init_k=4)
+    rep = phase_report(W, fam, d, res, dt, st, tr, args.steps)
     regions = st["regions_evaluated"]
-    value = regions / dt
-    flops_region = W.eval_flops_per_region(fam, d)
-    eval_s = st["eval_ms"] / 1e3
-    achieved = flops_region * regions / eval_s / 1e12 if eval_s > 0 else 0.0
+    achieved = rep["eval_tflops"] or 0.0
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": regions / dt,
         "unit": "regions/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -142,8 +162,9 @@ def main():
         "iters": res.iters,
         "regions_evaluated_per_step": regions // args.steps,
         "max_regions": st["max_regions"],
-        "phase_ms_per_step": {k: st[k] / args.steps for k in
-                              ("eval_ms", "refine_ms", "compact_ms", "classify_ms")},
+        "phase_ms_per_step": rep["phase_ms_per_integral"],
+        "classify_passes_per_step": rep["classify_passes_per_integral"],
+        "noneval_gbs": rep["noneval_gbs"],
         "roofline": {
             "bound": "fp64-valu",
             "kernel": "pagani::eval_kernel<8, Gaussian<double,8>, double>",
@@ -151,31 +172,141 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "flops_per_region": flops_region,
+            "flops_per_region": W.eval_flops_per_region(fam, d),
             "eval_launches": st["eval_launches"],
             "avg_launch_ms": st["eval_ms"] / max(st["eval_launches"], 1),
             "traffic": None,
         },
     }
     if not args.no_mixed:
-        r32, dt32, st32, _ = run_gpu(pg, W, fam, a, u, args.epsrel, args.steps, args.warmup,
-                                     pg.FP32)
-        e32 = st32["eval_ms"] / 1e3
+        # mixed precision beside FP64 at one stated tolerance both can certify
+        eps_m = args.mixed_epsrel
+        r32, dt32, st32, tr32 = run_gpu(pg, fam, a, u, eps_m, args.steps, args.warmup, pg.FP32,
+                                        init_k=4)
+        r64, dt64, st64, tr64 = run_gpu(pg, fam, a, u, eps_m, args.steps, args.warmup, pg.FP64,
+                                        init_k=4)
         out["mixed"] = {
-            "precision": "fp32 integrand + fp64 sums/errors",
-            "value": st32["regions_evaluated"] / dt32,
-            "unit": "regions/s",
-            "integrals_per_s": args.steps / dt32,
-            "estimate": r32.estimate,
-            "errorest": r32.errorest,
-            "abs_err_vs_analytic": abs(r32.estimate - exact),
-            "status": r32.status,
-            "nregions": r32.nregions,
-            "eval_regions_per_s": st32["regions_evaluated"] / e32 if e32 > 0 else None,
+            "epsrel": eps_m,
+            "fp32": dict(precision="fp32 integrand + fp64 sums/errors",
+                         regions_per_s=st32["regions_evaluated"] / dt32,
+                         integrals_per_s=args.steps / dt32, status=r32.status,
+                         abs_err_vs_analytic=abs(r32.estimate - exact), errorest=r32.errorest,
+                         nregions=r32.nregions),
+            "fp64": dict(regions_per_s=st64["regions_evaluated"] / dt64,
+                         integrals_per_s=args.steps / dt64, status=r64.status,
+                         abs_err_vs_analytic=abs(r64.estimate - exact), errorest=r64.errorest,
+                         nregions=r64.nregions),
         }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fam, a, u, args.cpu_seconds)
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def config1(args, pg, W):
+    """5D oscillatory, epsrel 1e-3, epsabs 1e-20, max 1e6 evaluations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    fam, a, u = W.oscillatory_5d(args.seed)
+    d = len(a)
+    prm = O.params(a, u)
+    exact = O.exact(fam, d, prm)
+    O.LIB.orc_set_num_threads(1)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        rc = O.cuhre_seq(d, fam, prm, 1e-3, 1e-20, 1_000_000)
+        reps += 1
+        if time.perf_counter() - t0 > 3.0:
+            break
+    cdt = (time.perf_counter() - t0) / reps
+    res, dt, st, tr = run_gpu(pg, fam, a, u, 1e-3, args.steps, args.warmup, pg.FP64,
+                              epsabs=1e-20)
+    rep = phase_report(W, fam, d, res, dt, st, tr, args.steps)
+    return {
+        "metric": "integrals/s, 5D Genz oscillatory to epsrel 1e-3 (CPU plain method vs GPU)",
+        "config": {"workload": "cos(2 pi u1 + sum a_i x_i), a_i=1+0.5U, u1=U, 5D, epsrel 1e-3, "
+                               "epsabs 1e-20, max 1e6 evals", "seed": args.seed},
+        "analytic": exact,
+        "gpu": dict(rep, abs_err_vs_analytic=abs(res.estimate - exact)),
+        "cpu_cuhre_equivalent": {
+            "cores": 1, "integrals_per_s": 1 / cdt, "ms_per_integral": cdt * 1e3,
+            "estimate": rc.estimate, "errorest": rc.errorest, "status": rc.status,
+            "neval": rc.neval, "nregions": rc.nregions,
+            "abs_err_vs_analytic": abs(rc.estimate - exact),
+        },
+    }
+
+
+def config3(args, pg, W):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    rows = []
+    for family in ("corner_peak", "product_peak"):
+        fam, a, u = W.mixed_6d(family, args.seed)
+        d = len(a)
+        exact = O.exact(fam, d, O.params(a, u))
+        for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
+            for prec, name in ((pg.FP64, "fp64"), (pg.FP32, "fp32")):
+                res, dt, st, tr = run_gpu(pg, fam, a, u, eps, 1, 0, prec)
+                rows.append({
+                    "family": fam, "epsrel": eps, "precision": name, "status": res.status,
+                    "estimate": res.estimate, "errorest": res.errorest,
+                    "rel_err_vs_analytic": abs(res.estimate - exact) / abs(exact),
+                    "regions_per_s": st["regions_evaluated"] / dt,
+                    "ms_per_integral": dt * 1e3, "nregions": res.nregions,
+                })
+    return {"metric": "mixed precision (FP32 integrand) vs FP64, 6D corner/product peak",
+            "config": {"workload": "a_i = 5 + 10U, u_i = U, 6D, epsrel 1e-3..1e-7",
+                       "seed": args.seed}, "runs": rows}
+
+
+def config5(args, pg, W):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_api as O
+    from gpuintegration_amd.workloads import MT19937_64
+    g = MT19937_64(args.seed)
+    d = 8
+    cases = [("oscillatory", [float(i + 1) for i in range(d)], [0.0] * d)]
+    a6 = [5.0 + 10.0 * g.uniform() for _ in range(d)]
+    u6 = [0.3 + 0.4 * g.uniform() for _ in range(d)]
+    cases.append(("discontinuous", a6, u6))
+    out = []
+    for fam, a, u in cases:
+        exact = O.exact(fam, d, O.params(a, u))
+        res, dt, st, tr = run_gpu(pg, fam, a, u, args.large_epsrel, 1, 0, pg.FP64,
+                                  mem_model="actual")
+        rep = phase_report(W, fam, d, res, dt, st, tr, 1)
+        out.append(dict(rep, family=fam, abs_err_vs_analytic=abs(res.estimate - exact),
+                        analytic=exact))
+    return {"metric": "memory-bound phases: non-eval HBM GB/s, 8D, epsrel 1e-8",
+            "config": {"workload": "8D cos(sum (i+1) x_i) and 8D discontinuous "
+                                   "(a_i=5+10U, u_i=0.3+0.4U), epsrel %g, memory model: actual"
+                                   % args.large_epsrel, "seed": args.seed},
+            "runs": out}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 5])
+    ap.add_argument("--epsrel", type=float, default=1e-6)
+    ap.add_argument("--mixed-epsrel", type=float, default=1e-5)
+    ap.add_argument("--large-epsrel", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(each integral runs on one device); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import gpuintegration_amd as pg
+    from gpuintegration_amd import workloads as W
+    fn = {1: config1, 2: config2, 3: config3, 5: config5}[args.config]
+    print(json.dumps(fn(args, pg, W)), flush=True)
 
 
 if __name__ == "__main__":

@@ -134,6 +134,7 @@ _SIGS = {
     "pagani_ws_ndim": (ctypes.c_int, [_vp]),
     "pagani_ws_set_device_mem": (ctypes.c_int, [_vp, _sz]),
     "pagani_ws_device_mem": (_sz, [_vp]),
+    "pagani_ws_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
@@ -314,6 +315,12 @@ class Workspace:
 
     def device_mem(self):
         return self.lib.pagani_ws_device_mem(self._ws)
+
+    def set_mem_model(self, model):
+        """0 / "reference": the reference's 8(10d+8)+16 B per region;
+        1 / "actual": this library's own footprint."""
+        m = {"reference": 0, "actual": 1}.get(model, model)
+        self._check(self.lib.pagani_ws_set_mem_model(self._ws, int(m)))
 
     def set_slices(self, s):
         self._check(self.lib.pagani_ws_set_slices(self._ws, int(s)))

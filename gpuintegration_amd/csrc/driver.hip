@@ -47,6 +47,7 @@ struct pagani_ws {
   std::string error;
   std::size_t hbm_total = 0;
   std::size_t mem_model_total = 0;  // 0 = hbm_total
+  int mem_model = PAGANI_MEM_REFERENCE;
   int num_cus = 256;
   int slices_override = 0;
   bool timing = false;
@@ -58,11 +59,15 @@ struct pagani_ws {
   int cur = 0;
   DevBuf<double> est, err0, err1, pest, perr;
   DevBuf<std::uint8_t> dim, active, flags;
-  DevBuf<double> part;
-  DevBuf<std::uint32_t> cnt;
-  DevBuf<std::uint64_t> off;
+  DevBuf<double> part, fpart;
+  DevBuf<std::uint32_t> cnt, fcnt;
+  DevBuf<std::uint64_t> off, foff;
   ph::Scalars* d_scal = nullptr;
   ph::Scalars* h_scal = nullptr;
+  // multi-threshold pass: per-block partials, results (device / pinned host)
+  double* d_mpart = nullptr;
+  double* d_mout = nullptr;
+  double* h_mout = nullptr;
   hipEvent_t ev[8] = {};
   bool compact_pending = false;
   std::vector<TraceRec> trace;
@@ -128,19 +133,28 @@ void release(DevBuf<T>& b)
 int ensure_chunks(pagani_ws* ws, std::size_t n)
 {
   const std::size_t nch = std::max<std::size_t>(ph::num_chunks(n), 1);
-  int rc = ensure(ws, ws->part, nch * ph::kPartDoubles, "chunk sums");
-  if (rc) return rc;
-  rc = ensure(ws, ws->cnt, nch, "chunk counts");
-  if (rc) return rc;
-  return ensure(ws, ws->off, nch, "chunk offsets");
+  const std::size_t nfold = std::max<std::size_t>(ph::num_folds(nch), 1);
+  int rc;
+  if ((rc = ensure(ws, ws->part, nch * ph::kPartDoubles, "chunk sums")) ||
+      (rc = ensure(ws, ws->cnt, nch, "chunk counts")) ||
+      (rc = ensure(ws, ws->off, nch, "chunk offsets")) ||
+      (rc = ensure(ws, ws->fpart, nfold * ph::kPartDoubles, "fold sums")) ||
+      (rc = ensure(ws, ws->fcnt, nfold, "fold counts")))
+    return rc;
+  return ensure(ws, ws->foff, nfold, "fold offsets");
+}
+
+hipError_t launch_finalize(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+{
+  return ph::finalize(ws->stream, ph::num_chunks(n), ws->part.p, counts ? ws->cnt.p : nullptr,
+                      counts ? ws->off.p : nullptr, ws->fpart.p, ws->fcnt.p, ws->foff.p,
+                      ws->d_scal, minmax ? 1 : 0);
 }
 
 // Finalize the chunk records of an n-region pass and read the scalars back.
 int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
 {
-  const std::size_t nch = ph::num_chunks(n);
-  HIP_TRY(ws, ph::finalize(ws->stream, nch, ws->part.p, counts ? ws->cnt.p : nullptr,
-                           counts ? ws->off.p : nullptr, ws->d_scal, minmax ? 1 : 0));
+  HIP_TRY(ws, launch_finalize(ws, n, counts, minmax));
   HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
                              hipMemcpyDeviceToHost, ws->stream));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
@@ -233,24 +247,30 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
 // Heuristic classifier, heuristic_classifier.cuh:89-452 (host logic; the
 // threshold passes run on the device).
 
-std::size_t mem_required(std::size_t n, int d)
+std::size_t mem_required(std::size_t n, int d, int model)
 {
-  // num_doubles_needed / num_ints_needed / device_mem_required_for_full_split
   const std::size_t nd = n * static_cast<std::size_t>(d);
+  if (model == PAGANI_MEM_ACTUAL) {
+    // live bounds (16d) + both children's bounds (32d) + their est/err0/err1
+    // (48) and dim/active/flags (6) + the parents' est/err (16)
+    return 16 * nd + 32 * nd + 48 * n + 6 * n + 16 * n;
+  }
+  // num_doubles_needed / num_ints_needed / device_mem_required_for_full_split
   const std::size_t dbl = 8 * n + 10 * nd;
   const std::size_t ints = 4 * n;
   return 8 * dbl + 4 * ints;
 }
 
-std::size_t free_mem(std::size_t n, int d, std::size_t total)
+std::size_t free_mem(std::size_t n, int d, std::size_t total, int model)
 {
-  const std::size_t occ = mem_required(n, d);
+  const std::size_t occ = mem_required(n, d, model);
   return total > occ ? total - occ : 1;
 }
 
-double mem_ratio(std::size_t n, int d, std::size_t total)
+double mem_ratio(std::size_t n, int d, std::size_t total, int model)
 {
-  return static_cast<double>(mem_required(n, d)) / static_cast<double>(free_mem(n, d, total));
+  return static_cast<double>(mem_required(n, d, model)) /
+         static_cast<double>(free_mem(n, d, total, model));
 }
 
 struct Classifier {
@@ -261,9 +281,10 @@ struct Classifier {
   double max_budget = .25, max_active = .5;
   std::size_t total = 0;
   int d = 0;
+  int model = PAGANI_MEM_REFERENCE;
 
-  Classifier(double rel, double abs_, std::size_t tot, int ndim)
-      : epsrel(rel), epsabs(abs_), total(tot), d(ndim)
+  Classifier(double rel, double abs_, std::size_t tot, int ndim, int mem_model)
+      : epsrel(rel), epsabs(abs_), total(tot), d(ndim), model(mem_model)
   {
     required_digits = static_cast<int>(std::ceil(std::log10(1 / epsrel)));
   }
@@ -303,29 +324,61 @@ struct Classifier {
 
   bool criteria_met(std::size_t n) const
   {
-    const double ratio = mem_ratio(n, d, total);
+    const double ratio = mem_ratio(n, d, total, model);
     if (ratio > 1.) return true;
     return ratio > .1 && estimate_converged();
   }
 };
 
-// One threshold pass: flags = err > thr, count and sum(flag*err).
-int threshold_pass(pagani_ws* ws, const double* err, std::size_t n, double thr,
-                   std::uint8_t* flags, std::size_t& count, double& dot)
+// One run of get_larger_threshold_results (heuristic_classifier.cuh:
+// 327-346).  While apply_threshold fails, the reference raises the
+// threshold to t + |high - t| / 2 with `high` fixed, at most 20 times: the
+// whole sequence is known before the first pass, so a single read of the
+// errors counts all of it (k_multi_threshold).  The sequential loop is then
+// replayed on the host with the counts, with the reference's arithmetic.
+// `last_dot` / `last_thr` are the sum of err > t and t of the last applied
+// threshold.
+int larger_threshold(pagani_ws* ws, const Classifier& c, const double* err, std::size_t n,
+                     pagani_class_res& res, double& last_dot, double& last_thr)
 {
-  HIP_TRY(ws, ph::threshold(ws->stream, n, err, thr, flags, ws->part.p, ws->cnt.p));
-  const int rc = finalize_read(ws, n, true, false);
-  if (rc) return rc;
-  count = ws->h_scal->count;
-  dot = ws->h_scal->sum[3];
+  ph::ThresholdSet T;
+  double t = res.threshold;
+  for (int k = 0; k < ph::kMaxThresholds; ++k) {
+    T.t[k] = t;
+    t += std::fabs(res.range_high - t) * .5;
+  }
+  T.k = ph::kMaxThresholds;
+  HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, T, ws->d_mpart, ws->d_mout));
+  HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * ph::kMaxThresholds * sizeof(double),
+                             hipMemcpyDeviceToHost, ws->stream));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   ws->stats.classify_passes++;
+  res.pass_mem = 0;
+  for (int k = 0; k < ph::kMaxThresholds && !res.pass_mem; ++k) {
+    // apply_threshold at res.threshold (== T.t[k])
+    const std::size_t cnt = static_cast<std::size_t>(ws->h_mout[k]);
+    last_dot = ws->h_mout[ph::kMaxThresholds + k];
+    last_thr = res.threshold;
+    res.threshold_passes++;
+    res.num_active = cnt;
+    res.percent_mem_active = static_cast<double>(static_cast<int>(cnt)) /
+                             static_cast<double>(static_cast<int>(n));
+    res.pass_mem = res.percent_mem_active <= c.max_active;
+    if (!res.pass_mem) {
+      res.range_low = res.threshold;
+      const double diff = std::fabs(res.range_high - res.threshold);
+      res.threshold += diff * .5;
+    }
+  }
   return PAGANI_OK;
 }
 
 // classify, :362-451 (with apply_threshold :282-303,
 // get_larger_threshold_results :327-346, evaluate_error_budget :305-325).
+// The flags of the last applied threshold (*last_thr) are written by the
+// caller with k_threshold_flags when it needs them.
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
-             double iter_fin_err, double tot_fin_err, std::uint8_t* flags, pagani_class_res& res)
+             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr)
 {
   std::memset(&res, 0, sizeof(res));
   int rc = ensure_chunks(ws, n);
@@ -340,31 +393,11 @@ int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, dou
   const double target_error = std::fabs(c.hist[2]) * c.epsrel;
   std::size_t n_inc = 0, n_dec = 0;
   double last_dot = 0.;
-  auto apply = [&]() -> int {
-    std::size_t cnt = 0;
-    const int r = threshold_pass(ws, err, n, res.threshold, flags, cnt, last_dot);
-    if (r) return r;
-    res.threshold_passes++;
-    res.num_active = cnt;
-    res.percent_mem_active = static_cast<double>(static_cast<int>(cnt)) /
-                             static_cast<double>(static_cast<int>(n));
-    res.pass_mem = res.percent_mem_active <= c.max_active;
-    return PAGANI_OK;
-  };
+  *last_thr = res.threshold;
   do {
     if (!res.pass_mem && n_inc <= 20) {
-      res.pass_mem = 0;
-      std::size_t counter = 0;
-      while (!res.pass_mem && counter < 20) {
-        rc = apply();
-        if (rc) return rc;
-        if (!res.pass_mem) {
-          res.range_low = res.threshold;
-          const double diff = std::fabs(res.range_high - res.threshold);
-          res.threshold += diff * .5;
-        }
-        counter++;
-      }
+      rc = larger_threshold(ws, c, err, n, res, last_dot, *last_thr);
+      if (rc) return rc;
       n_inc++;
     }
     if (res.pass_mem) {
@@ -456,7 +489,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
   }
 
   const std::size_t total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
-  Classifier clf(epsrel, epsabs, total, d);
+  Classifier clf(epsrel, epsabs, total, d, ws->mem_model);
   pagani_result cum;
   std::memset(&cum, 0, sizeof cum);
   cum.status = 1;
@@ -494,8 +527,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
                            has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
                            ws->part.p, ws->cnt.p));
-    HIP_TRY(ws, ph::finalize(ws->stream, ph::num_chunks(n), ws->part.p, ws->cnt.p, ws->off.p,
-                             ws->d_scal, 0));
+    HIP_TRY(ws, launch_finalize(ws, n, true, false));
     record(ws, EV_REF1);
     HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
                                hipMemcpyDeviceToHost, ws->stream));
@@ -539,36 +571,38 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         recount = true;
       }
       // heuristic_classify, Workspace.cuh:74-118
-      const bool necessary = mem_ratio(n, d, total) > 1.;
+      const bool necessary = mem_ratio(n, d, total, ws->mem_model) > 1.;
       bool terminate;
       if (!clf.criteria_met(n)) {
         terminate = necessary;
       } else {
         if ((rc = ensure(ws, ws->flags, n, "flags"))) return rc;
         pagani_class_res res;
+        double thr = 0.0;
         record(ws, EV_CLS0);
-        rc = classify(ws, clf, errs, n, iter_err, fin_err, cum.errorest, ws->flags.p, res);
-        record(ws, EV_CLS1);
+        rc = classify(ws, clf, errs, n, iter_err, fin_err, cum.errorest, res, &thr);
         if (rc) return rc;
         const bool ok = res.pass_mem && res.pass_errorest_budget;
         if (ok) {
+          // the accepted threshold's flags replace the active flags
+          // (Workspace.cuh:104-111), with their counts for the compaction
+          HIP_TRY(ws, ph::threshold_flags(ws->stream, n, ws->est.p, errs, thr, ws->flags.p,
+                                          ws->part.p, ws->cnt.p));
+          if ((rc = finalize_read(ws, n, true, false))) return rc;
+          if (ws->h_scal->count != res.num_active)
+            return fail(ws, PAGANI_EHIP, "classifier flag count disagrees with its pass");
           std::swap(ws->active, ws->flags);
-          recount = true;
+          act_est = ws->h_scal->sum[2];
+          m = ws->h_scal->count;
+          fin_est = iter_est - act_est;
+          fin_err = res.finished_errorest;
+          recount = false;
         }
+        record(ws, EV_CLS1);
         terminate = (!ok && necessary) || res.num_active == 0;
-        if (ok) fin_err = res.finished_errorest;
         if (ws->timing) {
           HIP_TRY(ws, hipStreamSynchronize(ws->stream));
           ws->stats.classify_ms += elapsed(ws, EV_CLS0, EV_CLS1);
-        }
-        if (recount) {
-          HIP_TRY(ws, ph::count_flags(ws->stream, n, ws->active.p, ws->est.p, errs, ws->part.p,
-                                      ws->cnt.p));
-          if ((rc = finalize_read(ws, n, true, false))) return rc;
-          act_est = ws->h_scal->sum[2];
-          m = ws->h_scal->count;
-          recount = false;
-          if (ok) fin_est = iter_est - act_est;
         }
       }
       if (recount) {
@@ -612,8 +646,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       const int nxt = 1 - ws->cur;
       record(ws, EV_CMP0);
       HIP_TRY(ws, ph::compact(ws->stream, 1, d, n, m, ws->L[ws->cur].p, ws->H[ws->cur].p,
-                              ws->active.p, ws->est.p, errs, ws->dim.p, ws->off.p, ws->L[nxt].p,
-                              ws->H[nxt].p, ws->pest.p, ws->perr.p, nullptr));
+                              ws->active.p, ws->est.p, errs, ws->dim.p, ws->off.p, ws->foff.p,
+                              ws->L[nxt].p, ws->H[nxt].p, ws->pest.p, ws->perr.p, nullptr));
       record(ws, EV_CMP1);
       ws->compact_pending = ws->timing;
       ws->cur = nxt;
@@ -724,6 +758,14 @@ pagani_ws* pagani_ws_create(int ndim, int device)
     return bail(ws, "hipMalloc(scalars)", e);
   if ((e = hipHostMalloc(&ws->h_scal, sizeof(ph::Scalars), hipHostMallocDefault)) != hipSuccess)
     return bail(ws, "hipHostMalloc(scalars)", e);
+  const std::size_t mpart = std::size_t(ph::kMultiBlocks) * 3 * ph::kMaxThresholds;
+  if ((e = hipMalloc(&ws->d_mpart, mpart * sizeof(double))) != hipSuccess)
+    return bail(ws, "hipMalloc(threshold partials)", e);
+  if ((e = hipMalloc(&ws->d_mout, 2 * ph::kMaxThresholds * sizeof(double))) != hipSuccess)
+    return bail(ws, "hipMalloc(threshold results)", e);
+  if ((e = hipHostMalloc(&ws->h_mout, 2 * ph::kMaxThresholds * sizeof(double),
+                         hipHostMallocDefault)) != hipSuccess)
+    return bail(ws, "hipHostMalloc(threshold results)", e);
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(ws, "hipEventCreate", e);
   return ws;
@@ -748,6 +790,12 @@ void pagani_ws_destroy(pagani_ws* ws)
   release(ws->part);
   release(ws->cnt);
   release(ws->off);
+  release(ws->fpart);
+  release(ws->fcnt);
+  release(ws->foff);
+  if (ws->d_mpart) hipFree(ws->d_mpart);
+  if (ws->d_mout) hipFree(ws->d_mout);
+  if (ws->h_mout) hipHostFree(ws->h_mout);
   if (ws->d_gtab) hipFree(ws->d_gtab);
   if (ws->d_scal) hipFree(ws->d_scal);
   if (ws->h_scal) hipHostFree(ws->h_scal);
@@ -768,6 +816,14 @@ int pagani_ws_set_device_mem(pagani_ws* ws, size_t total_bytes)
 {
   if (!ws) return PAGANI_EBADARG;
   ws->mem_model_total = total_bytes;
+  return PAGANI_OK;
+}
+
+int pagani_ws_set_mem_model(pagani_ws* ws, int model)
+{
+  if (!ws || (model != PAGANI_MEM_REFERENCE && model != PAGANI_MEM_ACTUAL))
+    return fail(ws, PAGANI_EBADARG, "unknown memory model");
+  ws->mem_model = model;
   return PAGANI_OK;
 }
 
@@ -941,7 +997,7 @@ int pagani_filter_dev(pagani_ws* ws, size_t n, const double* L, const double* H,
   *m_out = m;
   if (m == 0) return PAGANI_OK;
   HIP_TRY(ws, ph::compact(ws->stream, 0, ws->ndim, n, m, L, H, active, est, err, dim, ws->off.p,
-                          L2, H2, parent_est, parent_err, dim2));
+                          ws->foff.p, L2, H2, parent_est, parent_err, dim2));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   return PAGANI_OK;
 }
@@ -972,7 +1028,7 @@ int pagani_filter_split_dev(pagani_ws* ws, size_t n, const double* L, const doub
   *m_out = m;
   if (m == 0) return PAGANI_OK;
   HIP_TRY(ws, ph::compact(ws->stream, 1, ws->ndim, n, m, L, H, active, est, err, dim, ws->off.p,
-                          L2, H2, parent_est, parent_err, nullptr));
+                          ws->foff.p, L2, H2, parent_est, parent_err, nullptr));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   return PAGANI_OK;
 }
@@ -1026,7 +1082,7 @@ int pagani_scan_flags_dev(pagani_ws* ws, const uint8_t* flags, size_t n, uint64_
   HIP_TRY(ws, ph::sum_dot(ws->stream, n, nullptr, flags, ws->part.p, ws->cnt.p));
   if ((rc = finalize_read(ws, n, true, false))) return rc;
   *total = ws->h_scal->count;
-  HIP_TRY(ws, ph::scan_flags(ws->stream, n, flags, ws->off.p, offsets));
+  HIP_TRY(ws, ph::scan_flags(ws->stream, n, flags, ws->off.p, ws->foff.p, offsets));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   return PAGANI_OK;
 }
@@ -1039,13 +1095,19 @@ int pagani_classify_dev(pagani_ws* ws, const double* err, size_t n, double epsre
   if (!ws || !err || !flags || !out || n == 0 || !est_hist)
     return fail(ws, PAGANI_EBADARG, "bad classify arguments");
   const std::size_t total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
-  Classifier c(epsrel, 0.0, total, ws->ndim);
+  Classifier c(epsrel, 0.0, total, ws->ndim, ws->mem_model);
   c.hist[0] = est_hist[0];
   c.hist[1] = est_hist[1];
   c.hist[2] = est_hist[2];
   c.iters_collected = 3;
-  return classify(ws, c, err, n, iter_errorest, iter_finished_errorest, total_finished_errorest,
-                  flags, *out);
+  double thr = 0.0;
+  int rc = classify(ws, c, err, n, iter_errorest, iter_finished_errorest, total_finished_errorest,
+                    *out, &thr);
+  if (rc) return rc;
+  // the flags of the last applied threshold (Classification_res::active_flags)
+  HIP_TRY(ws, ph::threshold(ws->stream, n, err, thr, flags, ws->part.p, ws->cnt.p));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
 }
 
 }  // extern "C"

@@ -185,6 +185,194 @@ __global__ void __launch_bounds__(kThreads)
   write_record(se, sr, sae, sar, c, part, cnt);
 }
 
+// flags of the classifier's accepted threshold, with the refine-style
+// record (device_set_true_for_larger_than, heuristic_classifier.cuh:60-87,
+// and the dot products of Workspace.cuh:107-110).
+__global__ void __launch_bounds__(kThreads)
+    k_threshold_flags(std::size_t n, const double* __restrict__ est,
+                      const double* __restrict__ err, double thr,
+                      std::uint8_t* __restrict__ flags, double* __restrict__ part,
+                      std::uint32_t* __restrict__ cnt)
+{
+  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
+  std::uint32_t c = 0;
+#pragma unroll 4
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    if (t < n) {
+      const double e = est[t], r = err[t];
+      const bool a = r > thr;
+      flags[t] = a ? 1 : 0;
+      se = dd_add(se, e);
+      sr = dd_add(sr, r);
+      if (a) {
+        sae = dd_add(sae, e);
+        sar = dd_add(sar, r);
+        ++c;
+      }
+    }
+  }
+  write_record(se, sr, sae, sar, c, part, cnt);
+}
+
+// The candidate thresholds of one run of get_larger_threshold_results
+// (heuristic_classifier.cuh:327-346) in a single read of the errors: per
+// threshold, the count of err > t_k (apply_threshold, :282-303) and the sum
+// of those errors (the dot product of evaluate_error_budget, :305-325).
+// Fixed grid, grid-stride order, fixed reduction trees: deterministic.
+__global__ void __launch_bounds__(kThreads)
+    k_multi_threshold(std::size_t n, const double* __restrict__ err, const ThresholdSet T,
+                      double* __restrict__ bpart)
+{
+  __shared__ DD sh[kThreads / 64];
+  __shared__ std::uint32_t shc[kThreads / 64];
+  std::uint32_t cnt[kMaxThresholds];
+  double sum[kMaxThresholds];
+#pragma unroll
+  for (int k = 0; k < kMaxThresholds; ++k) {
+    cnt[k] = 0;
+    sum[k] = 0.0;
+  }
+  const std::size_t stride = static_cast<std::size_t>(gridDim.x) * kThreads;
+  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + threadIdx.x; t < n;
+       t += stride) {
+    const double r = err[t];
+#pragma unroll
+    for (int k = 0; k < kMaxThresholds; ++k) {
+      const bool a = r > T.t[k];
+      cnt[k] += a ? 1u : 0u;
+      sum[k] += a ? r : 0.0;
+    }
+  }
+  double* out = bpart + static_cast<std::size_t>(blockIdx.x) * 3 * kMaxThresholds;
+#pragma unroll
+  for (int k = 0; k < kMaxThresholds; ++k) {
+    if (k < T.k) {  // uniform
+      const DD s = block_reduce(DD{sum[k], 0.0}, sh);
+      const std::uint32_t c = block_count(cnt[k], shc);
+      if (threadIdx.x == 0) {
+        out[3 * k] = s.hi;
+        out[3 * k + 1] = s.lo;
+        out[3 * k + 2] = static_cast<double>(c);
+      }
+    }
+  }
+}
+
+// Fold the kMultiBlocks partials of k_multi_threshold: thread i folds
+// blocks i, i+256, ... in order, then one fixed tree per threshold.
+__global__ void __launch_bounds__(kThreads)
+    k_multi_reduce(int nblocks, int nthr, const double* __restrict__ bpart,
+                   double* __restrict__ out)
+{
+  __shared__ DD sh[kThreads / 64];
+  DD s[kMaxThresholds];
+  double c[kMaxThresholds];
+#pragma unroll
+  for (int k = 0; k < kMaxThresholds; ++k) {
+    s[k] = DD{0, 0};
+    c[k] = 0.0;
+  }
+  for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+    const double* p = bpart + static_cast<std::size_t>(b) * 3 * kMaxThresholds;
+#pragma unroll
+    for (int k = 0; k < kMaxThresholds; ++k) {
+      if (k < nthr) {
+        s[k] = dd_add(s[k], DD{p[3 * k], p[3 * k + 1]});
+        c[k] += p[3 * k + 2];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxThresholds; ++k) {
+    if (k < nthr) {
+      const DD t = block_reduce(s[k], sh);
+      const DD cc = block_reduce(DD{c[k], 0.0}, sh);
+      if (threadIdx.x == 0) {
+        out[k] = cc.hi + cc.lo;
+        out[nthr + k] = t.hi + t.lo;
+      }
+    }
+  }
+}
+
+// Stage A of finalize: one workgroup folds kFold chunk records (fixed
+// tree), writes the chunks' offsets relative to the fold and one fold
+// record.
+__global__ void __launch_bounds__(kThreads)
+    k_fold(std::size_t nchunks, const double* __restrict__ part,
+           const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+           double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt, int with_minmax)
+{
+  __shared__ DD sh[kThreads / 64];
+  __shared__ std::uint32_t wtot[kThreads / 64];
+  __shared__ double smn[kThreads / 64], smx[kThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const std::size_t c = static_cast<std::size_t>(blockIdx.x) * kFold + threadIdx.x;
+  const bool live = c < nchunks;
+  DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  double mn = HUGE_VAL, mx = -HUGE_VAL;
+  std::uint32_t v = 0;
+  if (live) {
+    const double* p = part + c * kPartDoubles;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] = DD{p[2 * i], p[2 * i + 1]};
+    if (with_minmax) {
+      mn = p[8];
+      mx = p[9];
+    }
+    if (cnt) v = cnt[c];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) s[i] = block_reduce(s[i], sh);
+  if (cnt) {
+    std::uint32_t inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const std::uint32_t y = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += y;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    std::uint32_t wbase = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      if (w < wave) wbase += wtot[w];
+      total += wtot[w];
+    }
+    if (live && off) off[c] = wbase + inc - v;
+    if (threadIdx.x == 0) fcnt[blockIdx.x] = total;
+  }
+  if (with_minmax) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mn = fmin(mn, __shfl_xor(mn, o, 64));
+      mx = fmax(mx, __shfl_xor(mx, o, 64));
+    }
+    if (lane == 0) {
+      smn[wave] = mn;
+      smx[wave] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int w = 1; w < kThreads / 64; ++w) {
+        mn = fmin(mn, smn[w]);
+        mx = fmax(mx, smx[w]);
+      }
+  }
+  if (threadIdx.x == 0) {
+    double* q = fpart + static_cast<std::size_t>(blockIdx.x) * kPartDoubles;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      q[2 * i] = s[i].hi;
+      q[2 * i + 1] = s[i].lo;
+    }
+    q[8] = mn;
+    q[9] = mx;
+  }
+}
+
 // device_set_true_for_larger_than + the reductions of apply_threshold and
 // evaluate_error_budget (heuristic_classifier.cuh:60-87, 282-325).
 __global__ void __launch_bounds__(kThreads)
@@ -383,12 +571,13 @@ __global__ void __launch_bounds__(kThreads)
               const double* __restrict__ H, const std::uint8_t* __restrict__ active,
               const double* __restrict__ est, const double* __restrict__ err,
               const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
-              double* __restrict__ L2, double* __restrict__ H2, double* __restrict__ pest,
-              double* __restrict__ perr, std::uint8_t* __restrict__ dim2)
+              const std::uint64_t* __restrict__ foff, double* __restrict__ L2,
+              double* __restrict__ H2, double* __restrict__ pest, double* __restrict__ perr,
+              std::uint8_t* __restrict__ dim2)
 {
   __shared__ std::uint32_t wsum[kThreads / 64];
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
-  std::uint64_t run = off[blockIdx.x];
+  std::uint64_t run = off[blockIdx.x] + foff[blockIdx.x / kFold];
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
     const bool a = t < n && active[t] == 1;
@@ -462,11 +651,12 @@ __global__ void __launch_bounds__(256)
 
 __global__ void __launch_bounds__(kThreads)
     k_scan_flags(std::size_t n, const std::uint8_t* __restrict__ flags,
-                 const std::uint64_t* __restrict__ off, std::uint64_t* __restrict__ out)
+                 const std::uint64_t* __restrict__ off, const std::uint64_t* __restrict__ foff,
+                 std::uint64_t* __restrict__ out)
 {
   __shared__ std::uint32_t wsum[kThreads / 64];
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
-  std::uint64_t run = off[blockIdx.x];
+  std::uint64_t run = off[blockIdx.x] + foff[blockIdx.x / kFold];
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
     const bool a = t < n && flags[t] != 0;
@@ -544,23 +734,54 @@ hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x, const std::uin
   return hipGetLastError();
 }
 
-hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
-                    const std::uint32_t* cnt, std::uint64_t* off, Scalars* out, int with_minmax)
+hipError_t threshold_flags(hipStream_t s, std::size_t n, const double* est, const double* err,
+                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt)
 {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, nchunks, part, cnt, off, out,
-                     with_minmax);
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_threshold_flags, dim3(static_cast<unsigned>(num_chunks(n))),
+                     dim3(kThreads), 0, s, n, est, err, thr, flags, part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err, const ThresholdSet& T,
+                           double* bpart, double* out)
+{
+  if (T.k < 1 || T.k > kMaxThresholds) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_multi_threshold, dim3(kMultiBlocks), dim3(kThreads), 0, s, n, err, T,
+                     bpart);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_multi_reduce, dim3(1), dim3(kThreads), 0, s, kMultiBlocks, T.k, bpart,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
+                    const std::uint32_t* cnt, std::uint64_t* off, double* fpart,
+                    std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out, int with_minmax)
+{
+  const std::size_t nfold = num_folds(nchunks);
+  if (nfold > 0) {
+    hipLaunchKernelGGL(k_fold, dim3(static_cast<unsigned>(nfold)), dim3(kThreads), 0, s, nchunks,
+                       part, cnt, off, fpart, cnt ? fcnt : nullptr, with_minmax);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, s, nfold, fpart,
+                     cnt ? fcnt : nullptr, cnt ? foff : nullptr, out, with_minmax);
   return hipGetLastError();
 }
 
 hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m,
                    const double* L, const double* H, const std::uint8_t* active,
                    const double* est, const double* err, const std::uint8_t* dim,
-                   const std::uint64_t* off, double* L2, double* H2, double* pest,
-                   double* perr, std::uint8_t* dim2)
+                   const std::uint64_t* off, const std::uint64_t* foff, double* L2, double* H2,
+                   double* pest, double* perr, std::uint8_t* dim2)
 {
   if (n == 0 || m == 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
-                     split, d, n, m, L, H, active, est, err, dim, off, L2, H2, pest, perr, dim2);
+                     split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr,
+                     dim2);
   return hipGetLastError();
 }
 
@@ -573,11 +794,11 @@ hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const
 }
 
 hipError_t scan_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
-                      const std::uint64_t* off, std::uint64_t* out)
+                      const std::uint64_t* off, const std::uint64_t* foff, std::uint64_t* out)
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_scan_flags, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0,
-                     s, n, flags, off, out);
+                     s, n, flags, off, foff, out);
   return hipGetLastError();
 }
 

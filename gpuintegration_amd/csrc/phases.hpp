@@ -17,10 +17,24 @@ namespace pagani {
 namespace phases {
 
 constexpr int kThreads = 256;
-constexpr int kPerThread = 8;
+constexpr int kPerThread = 16;
 constexpr std::size_t kChunk = static_cast<std::size_t>(kThreads) * kPerThread;
+// finalize stage A folds this many chunk records per workgroup
+constexpr std::size_t kFold = 256;
 
 inline std::size_t num_chunks(std::size_t n) { return (n + kChunk - 1) / kChunk; }
+inline std::size_t num_folds(std::size_t nchunks) { return (nchunks + kFold - 1) / kFold; }
+
+// The heuristic classifier's threshold search raises the threshold at most
+// 20 times in a row (heuristic_classifier.cuh:335); all of a run's
+// candidates are known in advance, so one pass evaluates them together.
+constexpr int kMaxThresholds = 20;
+struct ThresholdSet {
+  double t[kMaxThresholds];
+  int k;
+};
+// fixed grid of the multi-threshold pass (deterministic partial order)
+constexpr int kMultiBlocks = 1024;
 
 // Results of finalize(), read back by the host driver.
 struct Scalars {
@@ -59,11 +73,25 @@ hipError_t minmax(hipStream_t s, std::size_t n, const double* x, double* part);
 hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x,
                    const std::uint8_t* flags, double* part, std::uint32_t* cnt);
 
-// Reduce nchunks partial records; chunk offsets (exclusive scan of cnt) into
-// off (may be nullptr); results into *out.
+// flags = err > thr and the refine-style record {est, err, flag*est,
+// flag*err} with the flag count (the classifier's accepted threshold).
+hipError_t threshold_flags(hipStream_t s, std::size_t n, const double* est, const double* err,
+                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt);
+
+// For every threshold t_k of T: count of err > t_k and sum of those err.
+// Writes T.k counts then T.k sums (doubles) to out (device).  bpart holds
+// kMultiBlocks * 3 * kMaxThresholds doubles.
+hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err,
+                           const ThresholdSet& T, double* bpart, double* out);
+
+// Reduce nchunks partial records in two stages: workgroups of kFold chunk
+// records (chunk offsets relative to their fold into off, fold records into
+// fpart/fcnt), then one workgroup over the folds (fold offsets into foff,
+// results into *out).  Counts / offsets are skipped when cnt is nullptr.
+// The global exclusive offset of chunk c is off[c] + foff[c / kFold].
 hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
-                    const std::uint32_t* cnt, std::uint64_t* off, Scalars* out,
-                    int with_minmax);
+                    const std::uint32_t* cnt, std::uint64_t* off, double* fpart,
+                    std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out, int with_minmax);
 
 // Order-preserving compaction of the active regions (chunk offsets from
 // finalize).  split == 0: filter only, bounds at stride m, split axes
@@ -73,8 +101,8 @@ hipError_t finalize(hipStream_t s, std::size_t nchunks, const double* part,
 hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m,
                    const double* L, const double* H, const std::uint8_t* active,
                    const double* est, const double* err, const std::uint8_t* dim,
-                   const std::uint64_t* off, double* L2, double* H2, double* pest,
-                   double* perr, std::uint8_t* dim2);
+                   const std::uint64_t* off, const std::uint64_t* foff, double* L2,
+                   double* H2, double* pest, double* perr, std::uint8_t* dim2);
 
 // Children of all m regions (Sub_region_splitter::split without a filter).
 hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,
@@ -82,7 +110,7 @@ hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const
 
 // Per-element exclusive scan of 0/1 flags using chunk offsets.
 hipError_t scan_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
-                      const std::uint64_t* off, std::uint64_t* out);
+                      const std::uint64_t* off, const std::uint64_t* foff, std::uint64_t* out);
 
 hipError_t fill_u8(hipStream_t s, std::size_t n, std::uint8_t* p, std::uint8_t v);
 

@@ -180,6 +180,16 @@ int pagani_ws_ndim(const pagani_ws* ws);
  * the reference).  0 selects the device's HBM size (the default). */
 int pagani_ws_set_device_mem(pagani_ws* ws, size_t total_bytes);
 size_t pagani_ws_device_mem(const pagani_ws* ws);
+/* Bytes the classifier's memory model charges for splitting n regions:
+ * PAGANI_MEM_REFERENCE (default) is device_mem_required_for_full_split
+ * (heuristic_classifier.cuh:96-133, 8(10d+8)+16 bytes per region);
+ * PAGANI_MEM_ACTUAL is this library's own footprint (the live bounds, the
+ * children's bounds, the children's per-region arrays and the parents'
+ * estimates: 48d+70 bytes per region), which lets a pool grow about 1.6x
+ * larger on the same device before classification is forced. */
+#define PAGANI_MEM_REFERENCE 0
+#define PAGANI_MEM_ACTUAL 1
+int pagani_ws_set_mem_model(pagani_ws* ws, int model);
 /* waves per batch of 64 regions in the evaluation kernel; 0 = automatic */
 int pagani_ws_set_slices(pagani_ws* ws, int slices);
 int pagani_ws_set_timing(pagani_ws* ws, int enable);
