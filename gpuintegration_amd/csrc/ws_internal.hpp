@@ -1,0 +1,262 @@
+// ws_internal.hpp -- the workspace object and the helpers shared by the
+// single-integral driver (driver.hip) and the batched driver (batch.hip).
+// Library-internal; not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pagani.h"
+#include "phases.hpp"
+#include "rule.hpp"
+
+namespace pagani {
+namespace internal {
+
+namespace ph = pagani::phases;
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  std::size_t cap = 0;
+};
+
+struct TraceRec {
+  std::uint64_t n, m;
+  double estimate, errorest;
+};
+
+
+}  // namespace internal
+}  // namespace pagani
+
+struct pagani_ws {
+  int ndim = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string error;
+  std::size_t hbm_total = 0;
+  std::size_t mem_model_total = 0;  // 0 = hbm_total
+  int mem_model = PAGANI_MEM_REFERENCE;
+  int num_cus = 256;
+  int slices_override = 0;
+  bool timing = false;
+  pagani_stats stats{};
+  pagani::RuleTables rule;
+  double* d_gtab = nullptr;
+
+  pagani::internal::DevBuf<double> L[2], H[2];
+  int cur = 0;
+  pagani::internal::DevBuf<double> est, err0, err1, pest, perr;
+  pagani::internal::DevBuf<std::uint8_t> dim, active, flags;
+  pagani::internal::DevBuf<double> part, fpart;
+  pagani::internal::DevBuf<std::uint32_t> cnt, fcnt;
+  pagani::internal::DevBuf<std::uint64_t> off, foff;
+  pagani::phases::Scalars* d_scal = nullptr;
+  pagani::phases::Scalars* h_scal = nullptr;
+  // multi-threshold pass: per-block partials, results (device / pinned host)
+  double* d_mpart = nullptr;
+  double* d_mout = nullptr;
+  double* h_mout = nullptr;
+  hipEvent_t ev[8] = {};
+  bool compact_pending = false;
+  std::vector<pagani::internal::TraceRec> trace;
+  std::size_t live_n = 0;  // regions in L[cur], H[cur] after the last call
+};
+
+namespace pagani {
+namespace internal {
+
+inline int fail(pagani_ws* ws, int code, const std::string& msg)
+{
+  if (ws) ws->error = msg;
+  return code;
+}
+
+inline int hip_fail(pagani_ws* ws, hipError_t e, const char* where)
+{
+  return fail(ws, PAGANI_EHIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(ws, expr)                                  \
+  do {                                                     \
+    const hipError_t e_ = (expr);                          \
+    if (e_ != hipSuccess) return hip_fail(ws, e_, #expr);  \
+  } while (0)
+
+template <class T>
+int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
+{
+  if (n <= b.cap) return PAGANI_OK;
+  if (b.p) {
+    hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+  }
+  std::size_t want = std::max(n, b.cap + b.cap / 2);
+  void* p = nullptr;
+  if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    want = n;
+    if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ws, PAGANI_ENOMEM,
+                  std::string("out of device memory allocating ") + what + " (" +
+                      std::to_string(n * sizeof(T)) + " bytes)");
+    }
+  }
+  b.p = static_cast<T*>(p);
+  b.cap = want;
+  return PAGANI_OK;
+}
+
+template <class T>
+inline void release(DevBuf<T>& b)
+{
+  if (b.p) hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+}
+
+inline int ensure_chunks(pagani_ws* ws, std::size_t n)
+{
+  const std::size_t nch = std::max<std::size_t>(ph::num_chunks(n), 1);
+  const std::size_t nfold = std::max<std::size_t>(ph::num_folds(nch), 1);
+  int rc;
+  if ((rc = ensure(ws, ws->part, nch * ph::kPartDoubles, "chunk sums")) ||
+      (rc = ensure(ws, ws->cnt, nch, "chunk counts")) ||
+      (rc = ensure(ws, ws->off, nch, "chunk offsets")) ||
+      (rc = ensure(ws, ws->fpart, nfold * ph::kPartDoubles, "fold sums")) ||
+      (rc = ensure(ws, ws->fcnt, nfold, "fold counts")))
+    return rc;
+  return ensure(ws, ws->foff, nfold, "fold offsets");
+}
+
+inline hipError_t launch_finalize(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+{
+  return ph::finalize(ws->stream, ph::num_chunks(n), ws->part.p, counts ? ws->cnt.p : nullptr,
+                      counts ? ws->off.p : nullptr, ws->fpart.p, ws->fcnt.p, ws->foff.p,
+                      ws->d_scal, minmax ? 1 : 0);
+}
+
+// Finalize the chunk records of an n-region pass and read the scalars back.
+inline int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+{
+  HIP_TRY(ws, launch_finalize(ws, n, counts, minmax));
+  HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
+                             hipMemcpyDeviceToHost, ws->stream));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  return PAGANI_OK;
+}
+
+enum { EV_EVAL0, EV_EVAL1, EV_REF0, EV_REF1, EV_CMP0, EV_CMP1, EV_CLS0, EV_CLS1 };
+
+inline void record(pagani_ws* ws, int i)
+{
+  if (ws->timing) hipEventRecord(ws->ev[i], ws->stream);
+}
+
+inline double elapsed(pagani_ws* ws, int a, int b)
+{
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, ws->ev[a], ws->ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.0;
+  }
+  return ms;
+}
+
+inline std::size_t mem_required(std::size_t n, int d, int model)
+{
+  const std::size_t nd = n * static_cast<std::size_t>(d);
+  if (model == PAGANI_MEM_ACTUAL) {
+    // live bounds (16d) + both children's bounds (32d) + their est/err0/err1
+    // (48) and dim/active/flags (6) + the parents' est/err (16)
+    return 16 * nd + 32 * nd + 48 * n + 6 * n + 16 * n;
+  }
+  // num_doubles_needed / num_ints_needed / device_mem_required_for_full_split
+  const std::size_t dbl = 8 * n + 10 * nd;
+  const std::size_t ints = 4 * n;
+  return 8 * dbl + 4 * ints;
+}
+
+inline std::size_t free_mem(std::size_t n, int d, std::size_t total, int model)
+{
+  const std::size_t occ = mem_required(n, d, model);
+  return total > occ ? total - occ : 1;
+}
+
+inline double mem_ratio(std::size_t n, int d, std::size_t total, int model)
+{
+  return static_cast<double>(mem_required(n, d, model)) /
+         static_cast<double>(free_mem(n, d, total, model));
+}
+
+struct Classifier {
+  double epsrel = 0, epsabs = 0;
+  int required_digits = 0;
+  double hist[3] = {0, 0, 0};  // uninitialised in the reference; zeroed here
+  std::size_t iters_collected = 0;
+  double max_budget = .25, max_active = .5;
+  std::size_t total = 0;
+  int d = 0;
+  int model = PAGANI_MEM_REFERENCE;
+
+  Classifier(double rel, double abs_, std::size_t tot, int ndim, int mem_model)
+      : epsrel(rel), epsabs(abs_), total(tot), d(ndim), model(mem_model)
+  {
+    required_digits = static_cast<int>(std::ceil(std::log10(1 / epsrel)));
+  }
+
+  void store(double e)
+  {
+    hist[0] = hist[1];
+    hist[1] = hist[2];
+    hist[2] = e;
+    iters_collected++;
+  }
+
+  // sigDigitsSame, :170-203
+  bool sig_digits_same() const
+  {
+    double third = std::fabs(hist[0]), second = std::fabs(hist[1]), first = std::fabs(hist[2]);
+    while (first != 0. && first < 1.) first *= 10;
+    while (second != 0. && second < 1.) second *= 10;
+    while (third != 0. && third < 1.) third *= 10;
+    char a[512] = {0}, b[512] = {0}, c[512] = {0};
+    std::snprintf(a, sizeof a - 1, "%.15f", third);
+    std::snprintf(b, sizeof b - 1, "%.15f", second);
+    std::snprintf(c, sizeof c - 1, "%.15f", first);
+    bool verdict = true;
+    int sig = 0;
+    for (int i = 0; i < required_digits + 1 && sig < required_digits && verdict && i < 500; ++i) {
+      verdict = c[i] == b[i] && b[i] == a[i];
+      sig += (verdict && c[i] != '.') ? 1 : 0;
+    }
+    return verdict;
+  }
+
+  bool estimate_converged() const
+  {
+    return !(iters_collected - 1 < 1 || !sig_digits_same());
+  }
+
+  bool criteria_met(std::size_t n) const
+  {
+    const double ratio = mem_ratio(n, d, total, model);
+    if (ratio > 1.) return true;
+    return ratio > .1 && estimate_converged();
+  }
+};
+
+int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
+             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr);
+
+}  // namespace internal
+}  // namespace pagani

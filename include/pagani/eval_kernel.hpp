@@ -65,26 +65,18 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
   return acc;
 }
 
-}  // namespace detail
-
-// Kernel: regions [blockIdx.x*64, +64) of an n-region SoA pool.
-// blockDim.x == 64 * C.slices.  Dynamic LDS: (slices-1)*5*64 doubles.
+// One wave's share of the evaluation of up to 64 regions, one lane per
+// region t of a SoA pool with row stride `stride`.  Shared by the
+// single-integral and the batched kernels.  `wave` indexes the C.slices
+// waves that split the batch's points (0 when slices == 1); `lows`,
+// `range`, `jac` describe the integration box.
 template <int D, class F, class Real>
-__global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
-    eval_kernel(F f, std::size_t n, const double* __restrict__ L,
-                const double* __restrict__ H, double* __restrict__ est,
-                double* __restrict__ err, std::uint8_t* __restrict__ dim,
-                const pagani_eval_consts C)
+__device__ __forceinline__ void eval_regions_wave(
+    F& f, const double* __restrict__ L, const double* __restrict__ H, std::size_t stride,
+    std::size_t t, bool live, const double* lows, const double* range, double jac,
+    const pagani_eval_consts& C, int wave, int lane, double* part, double* __restrict__ est,
+    double* __restrict__ err, std::uint8_t* __restrict__ dim)
 {
-  extern __shared__ double part[];
-  const int lane = threadIdx.x & 63;
-  // wave index is uniform across the wave: keep it (and everything derived
-  // from it, the slice bounds and the point loop) in scalar registers
-  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
-  const bool live = t0 < n;
-  const std::size_t t = live ? t0 : n - 1;
-
   // Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
   // width in user coordinates.
   double cR[D], hR[D];
@@ -92,16 +84,16 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
   int maxdim = 0;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    const double lo = L[static_cast<std::size_t>(j) * n + t];
-    const double up = lo + H[static_cast<std::size_t>(j) * n + t];
+    const double lo = L[static_cast<std::size_t>(j) * stride + t];
+    const double up = lo + H[static_cast<std::size_t>(j) * stride + t];
     const double r = up - lo;
     vol *= r;
     if (r > maxr) {
       maxr = r;
       maxdim = j;
     }
-    cR[j] = C.lows[j] + (0.5 * (lo + up)) * C.range[j];
-    hR[j] = r * C.range[j];
+    cR[j] = lows[j] + (0.5 * (lo + up)) * range[j];
+    hR[j] = r * range[j];
   }
 
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -114,7 +106,7 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
 #pragma unroll
     for (int j = 0; j < D; ++j) xc[j] = static_cast<Real>(cR[j]);
     const double f0 = detail::call(f, xc);
-    const double base = (f0 * C.jac) * 2 * (1 - C.ratio);
+    const double base = (f0 * jac) * 2 * (1 - C.ratio);
     double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -132,7 +124,7 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
       s1 += fa + fb;
       s2 += fc + fd;
       const double diff = __builtin_fabs(
-          base + C.ratio * (fa * C.jac + fb * C.jac) - (fc * C.jac + fd * C.jac));
+          base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
       if (diff > maxdiff) {
         maxdiff = diff;
         bisect = j;
@@ -178,7 +170,7 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
   // Null-rule error estimate, Sample.cuh:261-289.
   double s[PAGANI_NRULES];
 #pragma unroll
-  for (int r = 0; r < PAGANI_NRULES; ++r) s[r] = rs[r] * C.jac;
+  for (int r = 0; r < PAGANI_NRULES; ++r) s[r] = rs[r] * jac;
   double e1 = 0.0, e2 = 0.0, e3 = 0.0;
 #pragma unroll
   for (int k = 0; k < PAGANI_NSETS; ++k) {
@@ -196,6 +188,78 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     err[t] = er;
     dim[t] = static_cast<std::uint8_t>(bisect);
   }
+}
+
+}  // namespace detail
+
+// Kernel: regions [blockIdx.x*64, +64) of an n-region SoA pool.
+// blockDim.x == 64 * C.slices.  Dynamic LDS: (slices-1)*5*64 doubles.
+template <int D, class F, class Real>
+__global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
+    eval_kernel(F f, std::size_t n, const double* __restrict__ L,
+                const double* __restrict__ H, double* __restrict__ est,
+                double* __restrict__ err, std::uint8_t* __restrict__ dim,
+                const pagani_eval_consts C)
+{
+  extern __shared__ double part[];
+  const int lane = threadIdx.x & 63;
+  // wave index is uniform across the wave: keep it (and everything derived
+  // from it, the slice bounds and the point loop) in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
+  const bool live = t0 < n;
+  const std::size_t t = live ? t0 : n - 1;
+  detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C, wave,
+                                        lane, part, est, err, dim);
+}
+
+// Batched evaluation.  The pool holds one 64-aligned segment per integral
+// (row stride `stride`); a launch covers the integrals of one group that
+// share ndim and integrand type.  Span k of the group starts at workgroup
+// spans[k].wave_begin; each 64-thread workgroup finds its span by a
+// (uniform) binary search, reads its integral's functor and box from the
+// tables, and evaluates up to 64 of the integral's regions.
+struct BatchSpan {
+  std::uint64_t wave_begin;  // first workgroup of this integral
+  std::uint64_t seg_begin;   // first region of its segment
+  std::uint64_t n;           // regions in the segment
+  std::uint32_t item;        // row of the functor / volume tables
+  std::uint32_t pad;
+};
+
+struct BatchVolume {
+  double lows[PAGANI_MAX_DIM];
+  double range[PAGANI_MAX_DIM];
+  double jac;
+  double pad;
+};
+
+template <int D, class F, class Real>
+__global__ void __launch_bounds__(64)
+    eval_batch_kernel(const BatchSpan* __restrict__ spans, int nspans,
+                      const unsigned char* __restrict__ fslots, std::size_t slot_bytes,
+                      const BatchVolume* __restrict__ vols, const double* __restrict__ L,
+                      const double* __restrict__ H, std::size_t stride,
+                      double* __restrict__ est, double* __restrict__ err,
+                      std::uint8_t* __restrict__ dim, const pagani_eval_consts C)
+{
+  int lo = 0, hi = nspans - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (spans[mid].wave_begin <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const BatchSpan sp = spans[lo];
+  const int lane = threadIdx.x & 63;
+  const std::uint64_t local = (blockIdx.x - sp.wave_begin) * 64 + lane;
+  const bool live = local < sp.n;
+  const std::size_t t = sp.seg_begin + (live ? local : sp.n - 1);
+  F f = *reinterpret_cast<const F*>(fslots + static_cast<std::size_t>(sp.item) * slot_bytes);
+  const BatchVolume& V = vols[sp.item];
+  detail::eval_regions_wave<D, F, Real>(f, L, H, stride, t, live, V.lows, V.range, V.jac, C, 0,
+                                        lane, nullptr, est, err, dim);
 }
 
 // Launch the evaluation of args->n regions for integrand f.  Returns 0 or
