@@ -209,7 +209,8 @@ def config1(args, pg, W):
     fam, a, u = W.oscillatory_5d(args.seed)
     d = len(a)
     prm = O.params(a, u)
-    exact = O.exact(fam, d, prm)
+    exact = W.exact(fam, a, u)
+    # CPU baseline: the oracle's sequential Cuhre-equivalent on one core
     O.LIB.orc_set_num_threads(1)
     t0 = time.perf_counter()
     reps = 0
@@ -238,13 +239,11 @@ def config1(args, pg, W):
 
 
 def config3(args, pg, W):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_api as O
     rows = []
     for family in ("corner_peak", "product_peak"):
         fam, a, u = W.mixed_6d(family, args.seed)
         d = len(a)
-        exact = O.exact(fam, d, O.params(a, u))
+        exact = W.exact(fam, a, u)
         for eps in (1e-3, 1e-4, 1e-5, 1e-6, 1e-7):
             for prec, name in ((pg.FP64, "fp64"), (pg.FP32, "fp32")):
                 res, dt, st, tr = run_gpu(pg, fam, a, u, eps, 1, 0, prec)
@@ -261,8 +260,6 @@ def config3(args, pg, W):
 
 
 def config5(args, pg, W):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_api as O
     from gpuintegration_amd.workloads import MT19937_64
     g = MT19937_64(args.seed)
     d = 8
@@ -272,7 +269,7 @@ def config5(args, pg, W):
     cases.append(("discontinuous", a6, u6))
     out = []
     for fam, a, u in cases:
-        exact = O.exact(fam, d, O.params(a, u))
+        exact = W.exact(fam, a, u)
         res, dt, st, tr = run_gpu(pg, fam, a, u, args.large_epsrel, 1, 0, pg.FP64,
                                   mem_model="actual")
         rep = phase_report(W, fam, d, res, dt, st, tr, 1)

@@ -112,6 +112,43 @@ def gaussian_exact(a, u):
     return v
 
 
+def exact(family, a, u):
+    """Closed-form integral over the unit cube of a Genz family member (the
+    forms of genz_1abs_5d.cuh etc.; every one is a product over axes except
+    the oscillatory phase and the corner peak's inclusion-exclusion)."""
+    d = len(a)
+    if family == "oscillatory":
+        # Re[e^{i 2 pi u_1} prod_j (e^{i a_j} - 1) / (i a_j)]
+        z = complex(math.cos(2 * math.pi * u[0]), math.sin(2 * math.pi * u[0]))
+        for aj in a:
+            z *= complex(math.sin(aj) / aj, (1.0 - math.cos(aj)) / aj)
+        return z.real
+    if family == "product_peak":
+        v = 1.0
+        for aj, uj in zip(a, u):
+            v *= aj * (math.atan(aj * (1.0 - uj)) + math.atan(aj * uj))
+        return v
+    if family == "corner_peak":
+        s = 0.0
+        for c in range(1 << d):
+            t = 1.0 + sum(a[j] for j in range(d) if (c >> j) & 1)
+            s += (-1.0 if bin(c).count("1") & 1 else 1.0) / t
+        return s / (math.factorial(d) * math.prod(a))
+    if family == "gaussian":
+        return gaussian_exact(a, u)
+    if family == "c0":
+        v = 1.0
+        for aj, uj in zip(a, u):
+            v *= (2.0 - math.exp(-aj * uj) - math.exp(-aj * (1.0 - uj))) / aj
+        return v
+    if family == "discontinuous":
+        v = 1.0
+        for aj, uj in zip(a, u):
+            v *= (math.exp(aj * min(max(uj, 0.0), 1.0)) - 1.0) / aj
+        return v
+    raise ValueError(f"unknown family {family!r}")
+
+
 def eval_flops_per_point(family, d):
     """Algorithmic FLOPs per rule point: point map (1 FMA per axis), the
     integrand, five weight FMAs.  exp is charged 24, cos 28, a division 4."""

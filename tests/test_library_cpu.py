@@ -98,3 +98,14 @@ def test_workload_generators():
     assert {len(x[1]) for x in b} == set(range(3, 9))
     assert all(1e-6 <= x[3] <= 1e-3 for x in b)
     assert W.eval_flops_per_region("gaussian", 8) == 90770
+
+
+def test_workload_closed_forms_match_oracle():
+    """The package's analytic values (used by the benchmark to report error)
+    agree with the oracle's closed forms, which are pinned to the reference's
+    constants and to brute-force quadrature."""
+    from gpuintegration_amd import workloads as W
+    for fam, a, u, _ in W.batch(96, seed=7):
+        want = O.exact(fam, len(a), O.params(a, u))
+        got = W.exact(fam, a, u)
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-300), (fam, len(a))
