@@ -63,6 +63,17 @@ class _Genz(ctypes.Structure):
     ]
 
 
+class _BatchItem(ctypes.Structure):
+    _fields_ = [
+        ("f", _Genz),
+        ("epsrel", ctypes.c_double),
+        ("epsabs", ctypes.c_double),
+        ("precision", ctypes.c_int),
+        ("relerr_classification", ctypes.c_int),
+        ("init_k", ctypes.c_size_t),
+    ]
+
+
 class EvalConsts(ctypes.Structure):
     _fields_ = [
         ("w", (ctypes.c_double * NRULES) * NSETS),
@@ -168,6 +179,17 @@ _SIGS = {
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _vp, ctypes.POINTER(ClassRes)]),
     "do_integration_from_c": (ctypes.c_int, [_dp]),
+    "pagani_batch_create": (_vp, [ctypes.c_int, ctypes.c_int]),
+    "pagani_batch_destroy": (None, [_vp]),
+    "pagani_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "pagani_batch_set_device_mem": (ctypes.c_int, [_vp, _sz]),
+    "pagani_batch_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_batch_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_batch_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
+    "pagani_batch_reset_stats": (ctypes.c_int, [_vp]),
+    "pagani_integrate_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_BatchItem), _sz,
+                                              ctypes.POINTER(_Result),
+                                              ctypes.POINTER(ctypes.c_int)]),
 }
 
 _LIB = None
@@ -445,6 +467,81 @@ class Workspace:
                                                  float(total_finished_errorest), _ptr(flags),
                                                  ctypes.byref(res)))
         return {k: getattr(res, k) for k, _ in ClassRes._fields_}
+
+
+class Batch:
+    """Owns a pagani_batch: independent Genz integrals integrated from one call
+    by `workers` host threads with their own streams and workspaces."""
+
+    def __init__(self, device=-1, workers=1):
+        self.lib = load()
+        self._b = self.lib.pagani_batch_create(int(device), int(workers))
+        if not self._b:
+            why = self.lib.pagani_batch_last_error(None)
+            raise PaganiError(EHIP, f"pagani_batch_create failed: "
+                                    f"{why.decode() if why else 'unknown'}")
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self.lib.pagani_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = self.lib.pagani_batch_last_error(self._b)
+            raise PaganiError(rc, msg.decode() if msg else "")
+        return rc
+
+    def set_device_mem(self, nbytes):
+        self._check(self.lib.pagani_batch_set_device_mem(self._b, int(nbytes)))
+
+    def set_mem_model(self, model):
+        m = {"reference": 0, "actual": 1}.get(model, model)
+        self._check(self.lib.pagani_batch_set_mem_model(self._b, int(m)))
+
+    def set_timing(self, on=True):
+        self._check(self.lib.pagani_batch_set_timing(self._b, 1 if on else 0))
+
+    def stats(self):
+        s = Stats()
+        self._check(self.lib.pagani_batch_get_stats(self._b, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self.lib.pagani_batch_reset_stats(self._b))
+
+    def integrate(self, items, precision=FP64, relerr=True, init_k=0, epsabs=1e-20):
+        """items: sequence of (family, a, u, epsrel) or (family, a, u, epsrel,
+        epsabs).  Returns (results, codes); raises on a hard error (a negative
+        code other than ENOMEM, which still fills the item's result)."""
+        n = len(items)
+        arr = (_BatchItem * max(n, 1))()
+        for i, it in enumerate(items):
+            fam, a, u, eps = it[:4]
+            arr[i].f = genz(fam, a, u)
+            arr[i].epsrel = float(eps)
+            arr[i].epsabs = float(it[4]) if len(it) > 4 else float(epsabs)
+            arr[i].precision = int(precision)
+            arr[i].relerr_classification = 1 if relerr else 0
+            arr[i].init_k = int(init_k)
+        res = (_Result * max(n, 1))()
+        codes = (ctypes.c_int * max(n, 1))()
+        rc = self.lib.pagani_integrate_batch(self._b, arr, n, res, codes)
+        if rc < 0 and rc != ENOMEM:
+            self._check(rc)
+        return [IntegrationResult._from(res[i]) for i in range(n)], list(codes)[:n]
 
 
 def do_integration_from_c():

@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "driver.hip": [],
     "genz_eval.hip": [],
     "interop.hip": [],
+    "batch.hip": [],
 }
 CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"]}
 

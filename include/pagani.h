@@ -232,6 +232,44 @@ int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx,
                         const pagani_region_list* init, size_t init_k,
                         pagani_result* out);
 
+/* ---- batches of independent integrals ---------------------------------- */
+/* The reference integrates a set of integrands with a host loop of
+ * Workspace::integrate calls (new_time_and_call.cuh:156-167, driven by
+ * randomized_integrands.cu:80-105).  A batch runs the same integrals from one
+ * call: `workers` host threads, each with its own stream and one workspace
+ * per dimension, take items largest-first (by dimension, then epsrel), so
+ * one integral's host round trips overlap another's kernels.  Each item's
+ * result is the one pagani_integrate_genz returns for it on a workspace whose
+ * memory-model total is the batch's (set_device_mem; default: device HBM /
+ * workers, so the concurrent pools fit together) -- independent of the
+ * order and interleaving. */
+typedef struct pagani_batch_item {
+  pagani_genz f;
+  double epsrel, epsabs;
+  int precision;             /* PAGANI_FP64 or PAGANI_FP32 */
+  int relerr_classification; /* 1 = the reference's default */
+  size_t init_k;             /* 0 = default initial split */
+} pagani_batch_item;
+
+typedef struct pagani_batch pagani_batch;
+
+/* NULL on failure (reason from pagani_batch_last_error(NULL)); workers in
+ * [1, 16] */
+pagani_batch* pagani_batch_create(int device, int workers);
+void pagani_batch_destroy(pagani_batch* b);
+const char* pagani_batch_last_error(const pagani_batch* b);
+int pagani_batch_set_device_mem(pagani_batch* b, size_t total_bytes);
+int pagani_batch_set_mem_model(pagani_batch* b, int model);
+int pagani_batch_set_timing(pagani_batch* b, int enable);
+/* statistics summed over all workspaces of the batch */
+int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out);
+int pagani_batch_reset_stats(pagani_batch* b);
+/* Integrate count items; results[i] and codes[i] (the return code of the
+ * item's integrate call) are filled for every item.  Returns PAGANI_OK, or
+ * the first negative code of an item (the other items still run). */
+int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items,
+                           size_t count, pagani_result* results, int* codes);
+
 /* ---- phases on caller-owned device arrays --------------------------------- */
 /* Sub_regions::uniform_split / create_uniform_split, Sub_regions.cuh:11-36,
  * 165-185: k^ndim boxes into L, H (capacity k^ndim regions). */

@@ -27,6 +27,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <utility>
 
 #include "pagani.h"
@@ -34,11 +35,24 @@
 namespace pagani {
 namespace detail {
 
+// Integrands that depend on x only through x - u may say so with
+// `double origin(int i) const` (= u_i) and `centered(t...)` (= f(u + t)).
+// The kernel then subtracts u from each region's centre once and maps
+// points straight to t = x - u.
+template <class F, class = void>
+struct has_origin : std::false_type {};
+template <class F>
+struct has_origin<F, std::void_t<decltype(std::declval<const F&>().origin(0))>>
+    : std::true_type {};
+
 template <class F, class Real, int D, std::size_t... I>
 __device__ __forceinline__ double call_at(F& f, const Real (&x)[D],
                                           std::index_sequence<I...>)
 {
-  return static_cast<double>(f(x[I]...));
+  if constexpr (has_origin<F>::value)
+    return static_cast<double>(f.centered(x[I]...));
+  else
+    return static_cast<double>(f(x[I]...));
 }
 
 template <class F, class Real, int D>
@@ -94,6 +108,7 @@ __device__ __forceinline__ void eval_regions_wave(
     }
     cR[j] = lows[j] + (0.5 * (lo + up)) * range[j];
     hR[j] = r * range[j];
+    if constexpr (has_origin<F>::value) cR[j] -= f.origin(j);
   }
 
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};

@@ -5,6 +5,11 @@
 // (a, u).  Real = double for the FP64 path, float for the mixed-precision
 // path (FP32 integrand, FP64 accumulation).  Parameters are pre-processed on
 // the host (a^2, 1/a^2, 2*pi*u_0) so the device does no divisions by a.
+//
+// The families that depend on x only through x - u (product peak, Gaussian,
+// C0) also offer the "centred" form: origin(i) = u_i and centered(t...) =
+// f(u + t).  The evaluation kernel then folds u into each region's centre
+// once, so a rule point costs one FMA per axis fewer.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,6 +26,8 @@ __device__ __forceinline__ double cos_(double x) { return ::cos(x); }
 __device__ __forceinline__ float cos_(float x) { return ::cosf(x); }
 __device__ __forceinline__ double fabs_(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float fabs_(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double fmax_(double x, double y) { return __builtin_fmax(x, y); }
+__device__ __forceinline__ float fmax_(float x, float y) { return __builtin_fmaxf(x, y); }
 }  // namespace math
 
 // cos(2 pi u_0 + sum a_i x_i)
@@ -54,6 +61,17 @@ struct ProductPeak {
       const Real t = x[i] - u[i];
       den *= ia2[i] + t * t;
     }
+    return Real(1) / den;
+  }
+  // f(u + t): the evaluation kernel hands over t = x - u directly
+  __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  template <class... T>
+  __device__ Real centered(T... ts) const
+  {
+    const Real t[D] = {static_cast<Real>(ts)...};
+    Real den = 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) den *= ia2[i] + t[i] * t[i];
     return Real(1) / den;
   }
 };
@@ -94,6 +112,16 @@ struct Gaussian {
     }
     return math::exp_(-s);
   }
+  __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  template <class... T>
+  __device__ Real centered(T... ts) const
+  {
+    const Real t[D] = {static_cast<Real>(ts)...};
+    Real s = a2[0] * (t[0] * t[0]);
+#pragma unroll
+    for (int i = 1; i < D; ++i) s += a2[i] * (t[i] * t[i]);
+    return math::exp_(-s);
+  }
 };
 
 // exp(-sum a_i |x_i - u_i|)
@@ -110,6 +138,16 @@ struct C0 {
     for (int i = 0; i < D; ++i) s += a[i] * math::fabs_(x[i] - u[i]);
     return math::exp_(-s);
   }
+  __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  template <class... T>
+  __device__ Real centered(T... ts) const
+  {
+    const Real t[D] = {static_cast<Real>(ts)...};
+    Real s = a[0] * math::fabs_(t[0]);
+#pragma unroll
+    for (int i = 1; i < D; ++i) s += a[i] * math::fabs_(t[i]);
+    return math::exp_(-s);
+  }
 };
 
 // 0 if any x_i > u_i, else exp(sum a_i x_i)
@@ -121,15 +159,17 @@ struct Discontinuous {
   __device__ Real operator()(X... xs) const
   {
     const Real x[D] = {static_cast<Real>(xs)...};
-    bool out = false;
-    Real s = 0;
+    // "some x_i > u_i" as max_i (x_i - u_i) > 0: one compare instead of a
+    // chain of lane masks
+    Real m = x[0] - u[0];
+    Real s = a[0] * x[0];
 #pragma unroll
-    for (int i = 0; i < D; ++i) {
-      out = out || (x[i] > u[i]);
+    for (int i = 1; i < D; ++i) {
+      m = math::fmax_(m, x[i] - u[i]);
       s += a[i] * x[i];
     }
     const Real v = math::exp_(s);
-    return out ? Real(0) : v;
+    return m > Real(0) ? Real(0) : v;
   }
 };
 

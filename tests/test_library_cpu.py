@@ -74,6 +74,15 @@ def test_create_without_device_fails_cleanly(pg):
         pg.Workspace(8)
     assert not lib.pagani_ws_create(1, 0)  # bad ndim
     assert b"ndim" in lib.pagani_create_error()
+    assert not lib.pagani_batch_create(0, 1)
+    assert lib.pagani_batch_last_error(None)
+    assert not lib.pagani_batch_create(0, 0)  # bad worker count
+    assert b"workers" in lib.pagani_batch_last_error(None)
+    with pytest.raises(pg.PaganiError):
+        pg.Batch(0, 2)
+    # null handles are rejected, not dereferenced
+    assert lib.pagani_integrate_batch(None, None, 0, None, None) == pg.EBADARG
+    assert lib.pagani_batch_get_stats(None, None) == pg.EBADARG
 
 
 def test_null_workspace_is_rejected(pg):
