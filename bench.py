@@ -29,6 +29,11 @@ sys.path.insert(0, ROOT)
 METRIC = "regions/s + integrals/s to epsrel 1e-6 (8D Genz), mixed vs FP64, 1 GPU"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per region of the 8D Gaussian evaluation kernel, from two
+# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over `bench.py
+# --eval-only`, each scaled by what the counter reads for a 1 GiB device
+# copy: 129.09 B read + 17.00 B written (algorithmic: 128 + 17).
+EVAL_HBM_BYTES_PER_REGION_8D = 146.09
 
 
 def _sync():
@@ -175,7 +180,11 @@ def config2(args, pg, W):
             "flops_per_region": W.eval_flops_per_region(fam, d),
             "eval_launches": st["eval_launches"],
             "avg_launch_ms": st["eval_ms"] / max(st["eval_launches"], 1),
-            "traffic": None,
+            "avg_regions_per_launch": regions / max(st["eval_launches"], 1),
+            "traffic": EVAL_HBM_BYTES_PER_REGION_8D * regions / max(st["eval_launches"], 1),
+            "traffic_unit": "bytes per launch (counter-measured per-region HBM bytes x "
+                            "average regions per launch)",
+            "algorithmic_bytes_per_launch": (16 * d + 17) * regions / max(st["eval_launches"], 1),
         },
     }
     if not args.no_mixed:
@@ -330,6 +339,46 @@ def config4(args, pg, W):
     }
 
 
+def eval_only(args, pg, W):
+    """The evaluation kernel in isolation, for hardware-counter runs: one
+    device-to-device copy of known size (the reference point that scales
+    FETCH_SIZE / WRITE_SIZE to bytes), then `--eval-launches` evaluations of
+    the 8D Gaussian over a uniform k^8 split of the unit cube."""
+    import torch
+    fam, a, u = W.gaussian_8d(args.seed)
+    d = len(a)
+    k = args.eval_k
+    n = k ** d
+    dev = torch.device("cuda:0")
+    L = torch.empty(n * d, dtype=torch.float64, device=dev)
+    H = torch.empty(n * d, dtype=torch.float64, device=dev)
+    est = torch.empty(n, dtype=torch.float64, device=dev)
+    err = torch.empty(n, dtype=torch.float64, device=dev)
+    dim = torch.empty(n, dtype=torch.uint8, device=dev)
+    ws = pg.Workspace(d, 0)
+    ws.uniform_split(k, L, H)
+    nbytes = args.copy_mib << 20
+    src = torch.zeros(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.eval_launches):
+        ws.eval_genz(fam, a, u, n, L, H, est, err, dim)
+    dt = time.perf_counter() - t0
+    ws.close()
+    return {
+        "mode": "eval-only", "kernel": "pagani::eval_kernel<8, Gaussian<double,8>, double>",
+        "regions_per_launch": n, "launches": args.eval_launches,
+        "algorithmic_bytes_per_launch": n * (16 * d + 17),
+        "flops_per_launch": n * W.eval_flops_per_region(fam, d),
+        "copy_bytes": nbytes,
+        "avg_launch_ms_host": dt / args.eval_launches * 1e3,
+        "tflops_host_timed": n * W.eval_flops_per_region(fam, d) * args.eval_launches / dt / 1e12,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -337,6 +386,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="time the evaluation kernel alone (for counter runs)")
+    ap.add_argument("--eval-k", type=int, default=6)
+    ap.add_argument("--eval-launches", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--workers", type=int, default=2)
     ap.add_argument("--epsrel", type=float, default=1e-6)
     ap.add_argument("--mixed-epsrel", type=float, default=1e-5)
@@ -353,6 +407,8 @@ def main():
     import gpuintegration_amd as pg
     from gpuintegration_amd import workloads as W
     fn = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}[args.config]
+    if args.eval_only:
+        fn = eval_only
     print(json.dumps(fn(args, pg, W)), flush=True)
 
 
