@@ -234,9 +234,25 @@ __global__ void __launch_bounds__(kThreads)
     cnt[k] = 0;
     sum[k] = 0.0;
   }
+  // Grid-stride order; four loads are issued before their values are
+  // consumed (in the same order) so more bytes are in flight per wave.
   const std::size_t stride = static_cast<std::size_t>(gridDim.x) * kThreads;
-  for (std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + threadIdx.x; t < n;
-       t += stride) {
+  std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  for (; t + 3 * stride < n; t += 4 * stride) {
+    double r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r[i] = err[t + i * stride];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int k = 0; k < kMaxThresholds; ++k) {
+        const bool a = r[i] > T.t[k];
+        cnt[k] += a ? 1u : 0u;
+        sum[k] += a ? r[i] : 0.0;
+      }
+    }
+  }
+  for (; t < n; t += stride) {
     const double r = err[t];
 #pragma unroll
     for (int k = 0; k < kMaxThresholds; ++k) {
@@ -260,40 +276,37 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Fold the kMultiBlocks partials of k_multi_threshold: thread i folds
-// blocks i, i+256, ... in order, then one fixed tree per threshold.
+// Fold the kMultiBlocks partials of k_multi_threshold: eight lanes per
+// threshold, lane j folds blocks j, j+8, j+16, ... in order, then a fixed
+// butterfly over the eight.
+constexpr int kMultiLanes = 8;
+static_assert(kMaxThresholds * kMultiLanes <= kThreads, "one group of lanes per threshold");
+
 __global__ void __launch_bounds__(kThreads)
     k_multi_reduce(int nblocks, int nthr, const double* __restrict__ bpart,
                    double* __restrict__ out)
 {
-  __shared__ DD sh[kThreads / 64];
-  DD s[kMaxThresholds];
-  double c[kMaxThresholds];
-#pragma unroll
-  for (int k = 0; k < kMaxThresholds; ++k) {
-    s[k] = DD{0, 0};
-    c[k] = 0.0;
-  }
-  for (int b = threadIdx.x; b < nblocks; b += kThreads) {
-    const double* p = bpart + static_cast<std::size_t>(b) * 3 * kMaxThresholds;
-#pragma unroll
-    for (int k = 0; k < kMaxThresholds; ++k) {
-      if (k < nthr) {
-        s[k] = dd_add(s[k], DD{p[3 * k], p[3 * k + 1]});
-        c[k] += p[3 * k + 2];
-      }
+  const int k = threadIdx.x / kMultiLanes, j = threadIdx.x % kMultiLanes;
+  DD s{0.0, 0.0};
+  double c = 0.0;  // counts are integers below 2^53: exact
+  if (k < nthr) {
+    for (int b = j; b < nblocks; b += kMultiLanes) {
+      const double* p = bpart + static_cast<std::size_t>(b) * 3 * kMaxThresholds + 3 * k;
+      s = dd_add(s, DD{p[0], p[1]});
+      c += p[2];
     }
   }
 #pragma unroll
-  for (int k = 0; k < kMaxThresholds; ++k) {
-    if (k < nthr) {
-      const DD t = block_reduce(s[k], sh);
-      const DD cc = block_reduce(DD{c[k], 0.0}, sh);
-      if (threadIdx.x == 0) {
-        out[k] = cc.hi + cc.lo;
-        out[nthr + k] = t.hi + t.lo;
-      }
-    }
+  for (int off = kMultiLanes / 2; off > 0; off >>= 1) {
+    DD o;
+    o.hi = __shfl_xor(s.hi, off, 64);
+    o.lo = __shfl_xor(s.lo, off, 64);
+    s = dd_add(s, o);
+    c += __shfl_xor(c, off, 64);
+  }
+  if (k < nthr && j == 0) {
+    out[k] = c;
+    out[nthr + k] = s.hi + s.lo;
   }
 }
 
