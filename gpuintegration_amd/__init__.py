@@ -22,6 +22,7 @@ MIN_DIM, MAX_DIM = 2, 8
 NSETS, NRULES, MAX_SLICES = 9, 5, 8
 FP64, FP32 = 64, 32
 OK, NOT_CONVERGED, EBADARG, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, -1, -2, -3, -4
+SUSPENDED = 2
 
 # Genz family codes (include/pagani.h)
 FAMILY = {
@@ -179,6 +180,17 @@ _SIGS = {
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _vp, ctypes.POINTER(ClassRes)]),
     "do_integration_from_c": (ctypes.c_int, [_dp]),
+    "pagani_ws_set_iteration_limit": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "pagani_ws_suspended": (ctypes.c_int, [_vp]),
+    "pagani_integrate_cb_resume": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(_Result)]),
+    "pagani_integrate_genz_resume": (ctypes.c_int, [_vp, ctypes.POINTER(_Genz), ctypes.c_int,
+                                                    ctypes.POINTER(_Result)]),
+    "pagani_ws_save_state": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "pagani_ws_load_state": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "pagani_ws_write_regions_csv": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "pagani_ws_set_snapshot_iter": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "pagani_ws_snapshot_regions": (ctypes.c_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(_vp)]),
     "pagani_batch_create": (_vp, [ctypes.c_int, ctypes.c_int]),
     "pagani_batch_destroy": (None, [_vp]),
     "pagani_batch_last_error": (ctypes.c_char_p, [_vp]),
@@ -390,6 +402,47 @@ class Workspace:
         if rc < 0 and rc != ENOMEM:
             self._check(rc)
         return IntegrationResult._from(res)
+
+    # -- suspend / checkpoint / resume ----------------------------------------
+    def set_iteration_limit(self, k):
+        """Suspend integrate/resume calls after k iterations (0: no limit)."""
+        self._check(self.lib.pagani_ws_set_iteration_limit(self._ws, int(k)))
+
+    def suspended(self):
+        return bool(self.lib.pagani_ws_suspended(self._ws))
+
+    def resume_genz(self, family, a, u=None, precision=FP64):
+        """Continue the held run; returns (result, suspended_again)."""
+        g = genz(family, a, u)
+        res = _Result()
+        rc = self.lib.pagani_integrate_genz_resume(self._ws, ctypes.byref(g), int(precision),
+                                                   ctypes.byref(res))
+        if rc < 0 and rc != ENOMEM:
+            self._check(rc)
+        return IntegrationResult._from(res), rc == SUSPENDED
+
+    def save_state(self, path):
+        self._check(self.lib.pagani_ws_save_state(self._ws, os.fsencode(path)))
+
+    def load_state(self, path):
+        self._check(self.lib.pagani_ws_load_state(self._ws, os.fsencode(path)))
+
+    def write_regions_csv(self, path):
+        self._check(self.lib.pagani_ws_write_regions_csv(self._ws, os.fsencode(path)))
+
+    def set_snapshot_iter(self, it):
+        """Keep a copy of the list evaluated in iteration `it` when no region
+        has finished by then (None: never)."""
+        v = (1 << 64) - 1 if it is None else int(it)
+        self._check(self.lib.pagani_ws_set_snapshot_iter(self._ws, v))
+
+    def snapshot_size(self):
+        n = _sz(0)
+        L = _vp()
+        H = _vp()
+        self._check(self.lib.pagani_ws_snapshot_regions(self._ws, ctypes.byref(n),
+                                                        ctypes.byref(L), ctypes.byref(H)))
+        return n.value
 
     # -- phases on torch device tensors ----------------------------------------
     def uniform_split(self, k, L, H):

@@ -37,6 +37,7 @@ HIP_SOURCES = {
     "genz_eval.hip": [],
     "interop.hip": [],
     "batch.hip": [],
+    "checkpoint.hip": [],
 }
 CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"]}
 

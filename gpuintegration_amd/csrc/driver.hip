@@ -218,27 +218,69 @@ int check_volume(pagani_ws* ws, const double* lows, const double* highs)
   return PAGANI_OK;
 }
 
+// Why a run stopped early at an iteration boundary: the per-call iteration
+// limit (pagani_ws_set_iteration_limit).
+void suspend(pagani_ws* ws, const LoopState& st)
+{
+  ws->susp = st;
+  ws->susp.held = true;
+}
+
+// The iteration loop, Workspace.cuh:148-358 / :362-494.  `resume` continues
+// a suspended or loaded run instead of starting one (then epsrel, epsabs,
+// lows, highs, relerr, init and init_k are taken from it).
 int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel, double epsabs,
-                   const double* lows, const double* highs, int relerr,
-                   const pagani_region_list* init, std::size_t init_k, pagani_result* out)
+                   const double* lows_in, const double* highs_in, int relerr,
+                   const pagani_region_list* init, std::size_t init_k, const LoopState* resume,
+                   const pagani_genz* genz, int precision, pagani_result* out)
 {
   if (!ws || !eval || !out) return fail(ws, PAGANI_EBADARG, "null argument");
-  if (!(epsrel >= 0.0) || !(epsabs >= 0.0) || std::isnan(epsrel))
-    return fail(ws, PAGANI_EBADARG, "epsrel and epsabs must be >= 0");
-  int rc = check_volume(ws, lows, highs);
-  if (rc) return rc;
-  const auto wall0 = std::chrono::steady_clock::now();
   const int d = ws->ndim;
+  LoopState st;
+  if (resume) {
+    st = *resume;
+  } else {
+    if (!(epsrel >= 0.0) || !(epsabs >= 0.0) || std::isnan(epsrel))
+      return fail(ws, PAGANI_EBADARG, "epsrel and epsabs must be >= 0");
+    int rc0 = check_volume(ws, lows_in, highs_in);
+    if (rc0) return rc0;
+    st.epsrel = epsrel;
+    st.epsabs = epsabs;
+    st.relerr = relerr;
+    for (int j = 0; j < d; ++j) {
+      st.lows[j] = lows_in ? lows_in[j] : 0.0;
+      st.highs[j] = highs_in ? highs_in[j] : 1.0;
+    }
+    st.user_regions = init != nullptr;
+    st.mem_model = ws->mem_model;
+    st.mem_total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
+    if (genz) st.genz = *genz;
+    st.precision = precision;
+    std::memset(&st.cum, 0, sizeof st.cum);
+    st.cum.status = 1;
+    st.cum.lastPhase = -1;
+  }
+  epsrel = st.epsrel;
+  epsabs = st.epsabs;
+  relerr = st.relerr;
+  const double* lows = st.lows;
+  const double* highs = st.highs;
+  const bool user_regions = st.user_regions != 0;
+  int rc = PAGANI_OK;
+  const auto wall0 = std::chrono::steady_clock::now();
   std::memset(out, 0, sizeof(*out));
   out->status = 1;
   out->lastPhase = -1;
-  ws->trace.clear();
+  if (!resume) ws->trace.clear();
+  ws->susp.held = false;
+  ws->live_evaluated = false;
   harvest_compact(ws);
 
   // initial region list
   std::size_t n = 0;
-  const bool user_regions = init != nullptr;
-  if (init) {
+  if (resume) {
+    n = st.n;
+  } else if (init) {
     n = init->n;
     if (n == 0 || !init->L || !init->H) return fail(ws, PAGANI_EBADARG, "empty region list");
     if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
@@ -259,16 +301,41 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     HIP_TRY(ws, ph::uniform_split(ws->stream, d, k, n, ws->L[ws->cur].p, ws->H[ws->cur].p));
   }
 
-  const std::size_t total = ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
-  Classifier clf(epsrel, epsabs, total, d, ws->mem_model);
-  pagani_result cum;
-  std::memset(&cum, 0, sizeof cum);
-  cum.status = 1;
-  cum.lastPhase = -1;
-  bool has_parents = false;
+  const std::size_t total = st.mem_total;
+  Classifier clf(epsrel, epsabs, total, d, st.mem_model);
+  clf.hist[0] = st.hist[0];
+  clf.hist[1] = st.hist[1];
+  clf.hist[2] = st.hist[2];
+  clf.iters_collected = st.iters_collected;
+  pagani_result cum = st.cum;
+  bool has_parents = st.has_parents != 0;
   int status_code = PAGANI_OK;
+  const int mem_model = st.mem_model;
+  std::uint64_t iters_this_call = 0;
+  double* errs = ws->err0.p;
 
-  for (std::size_t it = 0; it < 700 && n > 0; it++) {
+  std::uint64_t it = st.it;
+  for (; it < 700 && n > 0; it++) {
+    if (ws->iter_limit && iters_this_call == ws->iter_limit) {
+      st.it = it;
+      st.n = n;
+      st.has_parents = has_parents ? 1 : 0;
+      st.cum = cum;
+      st.hist[0] = clf.hist[0];
+      st.hist[1] = clf.hist[1];
+      st.hist[2] = clf.hist[2];
+      st.iters_collected = clf.iters_collected;
+      suspend(ws, st);
+      ws->live_n = n;
+      HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+      harvest_compact(ws);
+      ws->stats.wall_ms += std::chrono::duration<double, std::milli>(
+                               std::chrono::steady_clock::now() - wall0)
+                               .count();
+      *out = cum;
+      return PAGANI_SUSPENDED;
+    }
+    iters_this_call++;
     if ((rc = ensure(ws, ws->est, n, "estimates")) || (rc = ensure(ws, ws->err0, n, "errors")) ||
         (rc = ensure(ws, ws->err1, n, "errors")) || (rc = ensure(ws, ws->dim, n, "split axes")) ||
         (rc = ensure(ws, ws->active, n, "flags")) || (rc = ensure_chunks(ws, n)))
@@ -293,7 +360,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, n);
 
     // two-level error + relerr flags + the four region sums + counts
-    double* errs = has_parents ? ws->err1.p : ws->err0.p;
+    errs = has_parents ? ws->err1.p : ws->err0.p;
     record(ws, EV_REF0);
     HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
                            has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
@@ -317,6 +384,18 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     std::size_t m = ws->h_scal->count;
     if (user_regions) cum.iters++;
 
+    // predict_split's snapshot of the list just evaluated, Workspace.cuh:244-248
+    if (it == ws->snapshot_iter && cum.nregions == 0) {
+      if ((rc = ensure(ws, ws->snapL, n * d, "snapshot")) ||
+          (rc = ensure(ws, ws->snapH, n * d, "snapshot")))
+        return rc;
+      HIP_TRY(ws, hipMemcpyAsync(ws->snapL.p, ws->L[ws->cur].p, n * d * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ws->stream));
+      HIP_TRY(ws, hipMemcpyAsync(ws->snapH.p, ws->H[ws->cur].p, n * d * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ws->stream));
+      ws->snap_n = n;
+    }
+
     // accuracy_reached, PaganiUtils.cuh:387-394
     const double tot_est = std::fabs(cum.estimate + iter_est);
     const double tot_err = cum.errorest + iter_err;
@@ -326,7 +405,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       cum.errorest += iter_err;
       cum.status = 0;
       cum.nregions += n;
-      goto done;
+      goto done_evaluated;
     }
     clf.store(cum.estimate + iter_est);
     {
@@ -342,7 +421,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         recount = true;
       }
       // heuristic_classify, Workspace.cuh:74-118
-      const bool necessary = mem_ratio(n, d, total, ws->mem_model) > 1.;
+      const bool necessary = mem_ratio(n, d, total, mem_model) > 1.;
       bool terminate;
       if (!clf.criteria_met(n)) {
         terminate = necessary;
@@ -387,7 +466,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         cum.estimate += iter_est;
         cum.errorest += iter_err;
         cum.nregions += n;
-        goto done;
+        goto done_evaluated;
       }
       if (m > 0) {
         const int nxt = 1 - ws->cur;
@@ -402,7 +481,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
           cum.errorest += iter_err;
           cum.nregions += n;
           status_code = rc;
-          goto done;
+          goto done_evaluated;
         }
       }
       ws->trace.push_back({n, m, cum.estimate + iter_est, tot_err});
@@ -428,6 +507,11 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     if (!user_regions) cum.iters++;
   }
   cum.nregions += n;
+  goto done;
+done_evaluated:
+  // the live list is the one evaluated last: its est / err / dim are valid
+  ws->live_evaluated = true;
+  ws->live_err1 = errs == ws->err1.p;
 done:
   ws->live_n = n;
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
@@ -677,7 +761,7 @@ int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double ep
   if (!ws) return PAGANI_EBADARG;
   if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
   return integrate_impl(ws, eval, ctx, epsrel, epsabs, lows, highs, relerr_classification, init,
-                        init_k, out);
+                        init_k, nullptr, nullptr, 0, out);
 }
 
 int pagani_integrate_genz(pagani_ws* ws, const pagani_genz* f, int precision, double epsrel,
@@ -688,8 +772,65 @@ int pagani_integrate_genz(pagani_ws* ws, const pagani_genz* f, int precision, do
   int rc = check_genz(ws, f, precision);
   if (rc) return rc;
   GenzCtx ctx{*f, precision};
-  return pagani_integrate_cb(ws, genz_trampoline, &ctx, epsrel, epsabs, lows, highs,
-                             relerr_classification, nullptr, init_k, out);
+  if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+  return integrate_impl(ws, genz_trampoline, &ctx, epsrel, epsabs, lows, highs,
+                        relerr_classification, nullptr, init_k, nullptr, f, precision, out);
+}
+
+int pagani_ws_set_iteration_limit(pagani_ws* ws, uint64_t iterations)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->iter_limit = iterations;
+  return PAGANI_OK;
+}
+
+int pagani_ws_suspended(const pagani_ws* ws) { return ws ? (ws->susp.held ? 1 : 0) : 0; }
+
+int pagani_integrate_cb_resume(pagani_ws* ws, pagani_eval_fn eval, void* ctx, pagani_result* out)
+{
+  if (!ws) return PAGANI_EBADARG;
+  if (!ws->susp.held) return fail(ws, PAGANI_EBADARG, "no suspended run to resume");
+  if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+  return integrate_impl(ws, eval, ctx, 0, 0, nullptr, nullptr, 0, nullptr, 0, &ws->susp,
+                        nullptr, 0, out);
+}
+
+int pagani_integrate_genz_resume(pagani_ws* ws, const pagani_genz* f, int precision,
+                                 pagani_result* out)
+{
+  if (!ws) return PAGANI_EBADARG;
+  int rc = check_genz(ws, f, precision);
+  if (rc) return rc;
+  if (!ws->susp.held) return fail(ws, PAGANI_EBADARG, "no suspended run to resume");
+  const pagani_genz& g = ws->susp.genz;
+  if (g.family != 0) {
+    bool same = g.family == f->family && g.ndim == f->ndim && ws->susp.precision == precision;
+    for (int j = 0; same && j < f->ndim; ++j)
+      same = g.a[j] == f->a[j] && g.u[j] == f->u[j];
+    if (!same) return fail(ws, PAGANI_EBADARG, "integrand differs from the suspended run's");
+  }
+  GenzCtx ctx{*f, precision};
+  if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+  return integrate_impl(ws, genz_trampoline, &ctx, 0, 0, nullptr, nullptr, 0, nullptr, 0,
+                        &ws->susp, nullptr, 0, out);
+}
+
+int pagani_ws_set_snapshot_iter(pagani_ws* ws, uint64_t iteration)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->snapshot_iter = iteration;
+  ws->snap_n = 0;
+  return PAGANI_OK;
+}
+
+int pagani_ws_snapshot_regions(const pagani_ws* ws, size_t* n, const double** L,
+                               const double** H)
+{
+  if (!ws || !n || !L || !H) return PAGANI_EBADARG;
+  *n = ws->snap_n;
+  *L = ws->snap_n ? ws->snapL.p : nullptr;
+  *H = ws->snap_n ? ws->snapH.p : nullptr;
+  return PAGANI_OK;
 }
 
 int pagani_uniform_split_dev(pagani_ws* ws, size_t k, double* L, double* H)

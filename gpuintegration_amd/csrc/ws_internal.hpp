@@ -32,6 +32,29 @@ struct TraceRec {
   double estimate, errorest;
 };
 
+// Everything the iteration loop carries from one iteration to the next,
+// besides the region list itself (L[cur], H[cur]: n regions) and the
+// parents' estimates (pest: n/2 when has_parents).  A run suspended at an
+// iteration boundary keeps it in the workspace; a checkpoint file holds it
+// with the arrays.
+struct LoopState {
+  bool held = false;
+  std::uint64_t it = 0;  // index of the next iteration
+  std::uint64_t n = 0;   // regions to evaluate next
+  int has_parents = 0;
+  int user_regions = 0;  // Sub_regions& overload bookkeeping
+  int relerr = 1;
+  double epsrel = 0, epsabs = 0;
+  double lows[PAGANI_MAX_DIM] = {}, highs[PAGANI_MAX_DIM] = {};
+  pagani_result cum{};
+  double hist[3] = {0, 0, 0};  // classifier's last three estimates
+  std::uint64_t iters_collected = 0;
+  int mem_model = PAGANI_MEM_REFERENCE;
+  std::uint64_t mem_total = 0;
+  // the Genz descriptor when the run integrates one (family 0: a callback)
+  pagani_genz genz{};
+  int precision = 0;
+};
 
 }  // namespace internal
 }  // namespace pagani
@@ -68,6 +91,18 @@ struct pagani_ws {
   bool compact_pending = false;
   std::vector<pagani::internal::TraceRec> trace;
   std::size_t live_n = 0;  // regions in L[cur], H[cur] after the last call
+  // est / err0-or-err1 / dim hold the live list's values (a finished call
+  // whose last iteration evaluated that list)
+  bool live_evaluated = false;
+  bool live_err1 = false;
+  // suspension and checkpoints
+  std::uint64_t iter_limit = 0;  // iterations per call before suspending; 0 = none
+  pagani::internal::LoopState susp;
+  // snapshot of the region list evaluated in iteration snapshot_iter when no
+  // region had finished by then (Workspace.cuh:244-248, predict_split)
+  std::uint64_t snapshot_iter = ~std::uint64_t(0);
+  pagani::internal::DevBuf<double> snapL, snapH;
+  std::size_t snap_n = 0;
 };
 
 namespace pagani {

@@ -40,6 +40,7 @@ extern "C" {
 /* status codes */
 #define PAGANI_OK 0
 #define PAGANI_NOT_CONVERGED 1
+#define PAGANI_SUSPENDED 2 /* iteration limit reached; the run can be resumed */
 #define PAGANI_EBADARG (-1)
 #define PAGANI_EHIP (-2)
 #define PAGANI_ENOMEM (-3)
@@ -231,6 +232,40 @@ int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx,
                         const double* highs, int relerr_classification,
                         const pagani_region_list* init, size_t init_k,
                         pagani_result* out);
+
+/* ---- suspend, checkpoint, resume ----------------------------------------- */
+/* No reference counterpart beyond the in-memory region-list copy
+ * (Sub_regions::take_snapshot / load_snapshot, Sub_regions.cuh:220-239): a
+ * long run can stop at an iteration boundary, be written to a file and be
+ * continued later with exactly the result of the uninterrupted run.
+ *
+ * With a limit k > 0, an integrate or resume call returns PAGANI_SUSPENDED
+ * after k iterations, with *out holding the running sums (status 1); the
+ * workspace keeps the loop state until its next integrate call. */
+int pagani_ws_set_iteration_limit(pagani_ws* ws, uint64_t iterations);
+int pagani_ws_suspended(const pagani_ws* ws); /* 1 when a run is held */
+int pagani_integrate_cb_resume(pagani_ws* ws, pagani_eval_fn eval, void* ctx,
+                               pagani_result* out);
+/* The integrand must be the suspended run's (checked for Genz descriptors). */
+int pagani_integrate_genz_resume(pagani_ws* ws, const pagani_genz* f, int precision,
+                                 pagani_result* out);
+/* Checkpoint file of the held run (layout in checkpoint.hip's header
+ * comment: a versioned header, the loop state, the trace, the SoA region
+ * list and the parents' estimates, and a checksum).  Loading verifies the
+ * size, the dimension and the checksum, then holds the run for a resume. */
+int pagani_ws_save_state(pagani_ws* ws, const char* path);
+int pagani_ws_load_state(pagani_ws* ws, const char* path);
+/* The live region list as CSV, one row per region in the column order of
+ * Snapshot<NDIM>::Save (quad.h:107-118): estimate, error, upper and lower
+ * bound per axis, split axis, -1. */
+int pagani_ws_write_regions_csv(pagani_ws* ws, const char* path);
+/* predict_split (Workspace.cuh:244-248): copy the region list evaluated in
+ * iteration `iteration` when no region has finished by then (the reference
+ * uses 15); UINT64_MAX (the default) disables.  The copy stays in the
+ * workspace (device SoA, *n regions) until the next snapshot. */
+int pagani_ws_set_snapshot_iter(pagani_ws* ws, uint64_t iteration);
+int pagani_ws_snapshot_regions(const pagani_ws* ws, size_t* n, const double** L,
+                               const double** H);
 
 /* ---- batches of independent integrals ---------------------------------- */
 /* The reference integrates a set of integrands with a host loop of

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <cstdint>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -69,7 +71,11 @@ struct Sub_regions {
   explicit Sub_regions(std::size_t partitions_per_axis) { uniform_split(partitions_per_axis); }
   Sub_regions(const Sub_regions&) = delete;
   Sub_regions& operator=(const Sub_regions&) = delete;
-  ~Sub_regions() { free_device(); }
+  ~Sub_regions()
+  {
+    free_device();
+    free_snapshot();
+  }
 
   // k boxes per axis (create_uniform_split, Sub_regions.cuh:11-36)
   void uniform_split(std::size_t k)
@@ -130,11 +136,50 @@ struct Sub_regions {
     }
   }
 
+  // take_snapshot / load_snapshot (Sub_regions.cuh:220-239): one device
+  // copy of the list, swapped back in by load_snapshot
+  void take_snapshot() { set_snapshot(dLeftCoord, dLength, size); }
+
+  void set_snapshot(const T* L, const T* H, std::size_t n)
+  {
+    free_snapshot();
+    snapshot_size = n;
+    if (n == 0) return;
+    if (hipMalloc(&snapshot_dLeftCoord, n * ndim * sizeof(T)) != hipSuccess ||
+        hipMalloc(&snapshot_dLength, n * ndim * sizeof(T)) != hipSuccess) {
+      free_snapshot();
+      throw std::bad_alloc();
+    }
+    copy(snapshot_dLeftCoord, L, n * ndim, hipMemcpyDeviceToDevice);
+    copy(snapshot_dLength, H, n * ndim, hipMemcpyDeviceToDevice);
+  }
+
+  void load_snapshot()
+  {
+    free_device();
+    dLeftCoord = snapshot_dLeftCoord;
+    dLength = snapshot_dLength;
+    size = snapshot_size;
+    snapshot_dLeftCoord = snapshot_dLength = nullptr;
+    snapshot_size = 0;
+  }
+
   T* dLeftCoord = nullptr;
   T* dLength = nullptr;
   std::size_t size = 0;
+  T* snapshot_dLeftCoord = nullptr;
+  T* snapshot_dLength = nullptr;
+  std::size_t snapshot_size = 0;
 
  private:
+  void free_snapshot()
+  {
+    if (snapshot_dLeftCoord) (void)hipFree(snapshot_dLeftCoord);
+    if (snapshot_dLength) (void)hipFree(snapshot_dLength);
+    snapshot_dLeftCoord = snapshot_dLength = nullptr;
+    snapshot_size = 0;
+  }
+
   static void copy(T* dst, const T* src, std::size_t count, hipMemcpyKind kind)
   {
     if (hipMemcpy(dst, src, count * sizeof(T), kind) != hipSuccess)
@@ -174,6 +219,8 @@ class Workspace {
   {
     IntegT f = integrand;
     pagani_result r;
+    // this overload has no caller-owned list to keep a predict_split snapshot in
+    pagani_ws_set_snapshot_iter(ws_, ~std::uint64_t(0));
     const int rc = pagani_integrate_cb(ws_, &pagani::eval_callback<static_cast<int>(ndim), IntegT>,
                                        &f, epsrel, epsabs, vol.lows, vol.highs,
                                        relerr_classification ? 1 : 0, nullptr, 0, &r);
@@ -194,6 +241,9 @@ class Workspace {
     IntegT f = integrand;
     pagani_result r;
     pagani_region_list init{subregions.size, subregions.dLeftCoord, subregions.dLength};
+    // predict_split: the list of iteration 15 if no region finished before
+    // (Workspace.cuh:244-248)
+    pagani_ws_set_snapshot_iter(ws_, predict_split ? 15 : ~std::uint64_t(0));
     const int rc = pagani_integrate_cb(ws_, &pagani::eval_callback<static_cast<int>(ndim), IntegT>,
                                        &f, epsrel, epsabs, vol.lows, vol.highs,
                                        relerr_classification ? 1 : 0, &init, 0, &r);
@@ -202,6 +252,10 @@ class Workspace {
     const double *L = nullptr, *H = nullptr;
     pagani::throw_on(pagani_ws_live_regions(ws_, &n, &L, &H), ws_, "live regions");
     subregions.assign_device(L, H, n);
+    if (predict_split) {
+      pagani::throw_on(pagani_ws_snapshot_regions(ws_, &n, &L, &H), ws_, "snapshot");
+      if (n) subregions.set_snapshot(L, H, n);
+    }
     return pagani::to_result(r);
   }
 

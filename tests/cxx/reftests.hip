@@ -148,6 +148,40 @@ int run_split(int k, double epsrel, double epsabs, std::size_t mem_total, pagani
   return 0;
 }
 
+// predict_split (Workspace.cuh:244-248): the list evaluated in iteration 15
+// is kept in the caller's Sub_regions when no region finished before it;
+// load_snapshot (Sub_regions.cuh:232-239) then restarts from that list.
+template <int D, class F>
+int run_predict_split(int k, double epsrel, double epsabs, int relerr, pagani_result* out,
+                      std::size_t* snap_n, std::uint64_t* trace_n, std::uint64_t* trace_m,
+                      std::size_t cap, std::size_t* ntrace, pagani_result* out2)
+{
+  Workspace<double, D> pagani;
+  quad::Volume<double, D> vol;
+  Sub_regions<double, D> regions(static_cast<std::size_t>(k));
+  numint::integration_result r = pagani.template integrate<F, true>(
+      F{}, regions, epsrel, epsabs, vol, relerr != 0);
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->nregions = r.nregions;
+  out->status = r.status;
+  out->iters = r.iters;
+  *snap_n = regions.snapshot_size;
+  *ntrace = pagani_ws_trace(pagani.handle(), cap, trace_n, trace_m, nullptr, nullptr);
+  std::memset(out2, 0, sizeof(*out2));
+  if (regions.snapshot_size) {
+    regions.load_snapshot();
+    if (regions.size != *snap_n || regions.snapshot_size != 0) return -3;
+    r = pagani.integrate(F{}, regions, epsrel, epsabs, vol, relerr != 0);
+    out2->estimate = r.estimate;
+    out2->errorest = r.errorest;
+    out2->nregions = r.nregions;
+    out2->status = r.status;
+    out2->iters = r.iters;
+  }
+  return 0;
+}
+
 // RegionSampling.cu: evaluate every region of a k^D uniform split once.
 template <int D, class F>
 int sample(int k, double* est_host, double* err_host, std::size_t cap)
@@ -220,6 +254,22 @@ extern "C" int reftest_integrate_split(const char* name, int k, double epsrel, d
       return run_split<2, XY>(k, epsrel, epsabs, mem_total, out, final_n);
     if (!std::strcmp(name, "F_4_5D"))
       return run_split<5, F_4_5D>(k, epsrel, epsabs, mem_total, out, final_n);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}
+
+extern "C" int reftest_predict_split(const char* name, int k, double epsrel, double epsabs,
+                                     int relerr, pagani_result* out, std::size_t* snap_n,
+                                     std::uint64_t* trace_n, std::uint64_t* trace_m,
+                                     std::size_t cap, std::size_t* ntrace, pagani_result* out2)
+{
+  if (!name || !out || !snap_n || !trace_n || !trace_m || !ntrace || !out2 || k < 1) return -1;
+  try {
+    if (!std::strcmp(name, "F_4_5D"))
+      return run_predict_split<5, F_4_5D>(k, epsrel, epsabs, relerr, out, snap_n, trace_n,
+                                          trace_m, cap, ntrace, out2);
   } catch (const std::exception&) {
     return -2;
   }
