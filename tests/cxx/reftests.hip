@@ -288,3 +288,224 @@ extern "C" int reftest_sample(const char* name, int k, double* est, double* err,
   }
   return -1;
 }
+
+// ---------------------------------------------------------------------------
+// Interpolation tables (include/pagani/interp.hpp), driven like the
+// reference's Interpolation1D/2D/3D.cu: evaluate on the device at given
+// points, optionally through clamp().
+
+#include <fstream>
+
+#include "pagani/interp.hpp"
+#include "pagani/str_to_doubles.hh"
+
+namespace {
+
+template <class T>
+struct DevVec {
+  T* p = nullptr;
+  explicit DevVec(std::size_t n)
+  {
+    if (n && hipMalloc(&p, n * sizeof(T)) != hipSuccess) throw std::bad_alloc();
+  }
+  ~DevVec()
+  {
+    if (p) (void)hipFree(p);
+  }
+  void put(const T* h, std::size_t n)
+  {
+    if (n && hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("copy");
+  }
+  void get(T* h, std::size_t n) const
+  {
+    if (n && hipMemcpy(h, p, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("copy");
+  }
+};
+
+__global__ void k_interp1(quad::Interp1D f, const double* x, double* out, std::size_t n,
+                          int clamp)
+{
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = clamp ? f.clamp(x[i]) : f(x[i]);
+}
+
+__global__ void k_interp2(quad::Interp2D f, const double* x, const double* y, double* out,
+                          std::size_t n, int clamp)
+{
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = clamp ? f.clamp(x[i], y[i]) : f(x[i], y[i]);
+}
+
+__global__ void k_interp3(quad::Interp3D f, const double* x, const double* y, const double* z,
+                          double* out, std::size_t n, int clamp)
+{
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) out[i] = clamp ? f.clamp(x[i], y[i], z[i]) : f(x[i], y[i], z[i]);
+}
+
+unsigned blocks_for(std::size_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+// an integrand holding a table, as the reference's model integrands do
+struct TableIntegrand {
+  quad::Interp2D table;
+  __device__ double operator()(double x, double y) const { return table(x, y); }
+};
+
+// memory-usage check (interpolator_mem_usage.cu): an integrand with both
+// kinds of table, copied into every launch
+struct TwoTables {
+  quad::Interp1D a;
+  quad::Interp2D b;
+  __device__ double operator()(double x, double y) const { return a(x) * b(x, y); }
+};
+
+bool load_table(const char* path, quad::Interp2D& t)
+{
+  std::ifstream in(path);
+  if (!in) return false;
+  in >> t;
+  return static_cast<bool>(in) || in.eof();
+}
+
+}  // namespace
+
+extern "C" int reftest_parse_doubles(const char* line, double* out, std::size_t cap,
+                                     std::size_t* n)
+{
+  try {
+    const std::vector<double> v = str_to_doubles(line);
+    *n = v.size();
+    for (std::size_t i = 0; i < v.size() && i < cap; ++i) out[i] = v[i];
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+extern "C" int reftest_interp1d(const double* xs, const double* zs, std::size_t cols,
+                                const double* px, std::size_t n, int clamp, double* out)
+{
+  try {
+    quad::Interp1D f(xs, zs, cols);
+    DevVec<double> dx(n), dout(n);
+    dx.put(px, n);
+    hipLaunchKernelGGL(k_interp1, dim3(blocks_for(n)), dim3(256), 0, 0, f, dx.p, dout.p, n,
+                       clamp);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    dout.get(out, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+extern "C" int reftest_interp2d(const double* xs, const double* ys, const double* zs,
+                                std::size_t cols, std::size_t rows, const double* px,
+                                const double* py, std::size_t n, int clamp, double* out)
+{
+  try {
+    quad::Interp2D f(xs, ys, zs, cols, rows);
+    DevVec<double> dx(n), dy(n), dout(n);
+    dx.put(px, n);
+    dy.put(py, n);
+    hipLaunchKernelGGL(k_interp2, dim3(blocks_for(n)), dim3(256), 0, 0, f, dx.p, dy.p, dout.p,
+                       n, clamp);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    dout.get(out, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+extern "C" int reftest_interp3d(const double* xs, const double* ys, const double* zs,
+                                const double* vs, std::size_t nx, std::size_t ny, std::size_t nz,
+                                const double* px, const double* py, const double* pz,
+                                std::size_t n, int clamp, double* out)
+{
+  try {
+    quad::Interp3D f(xs, ys, zs, vs, nx, ny, nz);
+    DevVec<double> dx(n), dy(n), dz(n), dout(n);
+    dx.put(px, n);
+    dy.put(py, n);
+    dz.put(pz, n);
+    hipLaunchKernelGGL(k_interp3, dim3(blocks_for(n)), dim3(256), 0, 0, f, dx.p, dy.p, dz.p,
+                       dout.p, n, clamp);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    dout.get(out, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// A table read from a stream-format file, evaluated at n points.
+extern "C" int reftest_interp2d_file(const char* path, const double* px, const double* py,
+                                     std::size_t n, double* out)
+{
+  try {
+    quad::Interp2D f;
+    if (!load_table(path, f)) return -3;
+    DevVec<double> dx(n), dy(n), dout(n);
+    dx.put(px, n);
+    dy.put(py, n);
+    hipLaunchKernelGGL(k_interp2, dim3(blocks_for(n)), dim3(256), 0, 0, f, dx.p, dy.p, dout.p,
+                       n, 0);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    dout.get(out, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// PAGANI over the table's own rectangle, with the table inside the integrand.
+extern "C" int reftest_integrate_table_file(const char* path, const double* lows,
+                                            const double* highs, double epsrel, double epsabs,
+                                            pagani_result* out)
+{
+  try {
+    TableIntegrand f;
+    if (!load_table(path, f.table)) return -3;
+    Workspace<double, 2> pagani;
+    quad::Volume<double, 2> vol(lows, highs);
+    const numint::integration_result r = pagani.integrate(f, epsrel, epsabs, vol);
+    out->estimate = r.estimate;
+    out->errorest = r.errorest;
+    out->nregions = r.nregions;
+    out->status = r.status;
+    out->iters = r.iters;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// Device memory in use before and after `runs` integrations of an integrand
+// holding two tables (interpolator_mem_usage.cu): copies must not leak.
+extern "C" int reftest_interp_mem_usage(int runs, std::size_t* free_before,
+                                        std::size_t* free_after, double* last)
+{
+  try {
+    const double xs1[5] = {0., .25, .5, .75, 1.};
+    const double zs1[5] = {1., 1.5, 2., 2.5, 3.};
+    const double xs2[3] = {0., .5, 1.};
+    const double ys2[2] = {0., 1.};
+    const double zs2[6] = {1., 2., 3., 4., 5., 6.};
+    Workspace<double, 2> pagani;
+    quad::Volume<double, 2> vol;
+    TwoTables f{quad::Interp1D(xs1, zs1, 5), quad::Interp2D(xs2, ys2, zs2, 3, 2)};
+    (void)pagani.integrate(f, 1e-6, 1e-12, vol);  // warm up the pools
+    std::size_t total = 0;
+    if (hipMemGetInfo(free_before, &total) != hipSuccess) return -2;
+    numint::integration_result r{};
+    for (int i = 0; i < runs; ++i) r = pagani.integrate(f, 1e-6, 1e-12, vol);
+    if (hipMemGetInfo(free_after, &total) != hipSuccess) return -2;
+    *last = r.estimate;
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
