@@ -391,7 +391,7 @@ def main():
     ap.add_argument("--eval-k", type=int, default=6)
     ap.add_argument("--eval-launches", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)
-    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--workers", type=int, default=1)
     ap.add_argument("--epsrel", type=float, default=1e-6)
     ap.add_argument("--mixed-epsrel", type=float, default=1e-5)
     ap.add_argument("--large-epsrel", type=float, default=1e-8)
