@@ -101,6 +101,8 @@ def phase_report(W, fam, d, res, dt, st, tr, steps):
         "phase_ms_per_integral": {k: st[k] / steps for k in
                                   ("eval_ms", "refine_ms", "compact_ms", "classify_ms")},
         "classify_passes_per_integral": st["classify_passes"] / steps,
+        "alloc_ms_per_integral": st["alloc_ms"] / steps,
+        "allocations_per_integral": st["allocations"] / steps,
     }
 
 

@@ -109,6 +109,8 @@ class Stats(ctypes.Structure):
         ("classify_passes", ctypes.c_uint64),
         ("max_regions", ctypes.c_uint64),
         ("integrals", ctypes.c_uint64),
+        ("alloc_ms", ctypes.c_double),
+        ("allocations", ctypes.c_uint64),
     ]
 
     def as_dict(self):

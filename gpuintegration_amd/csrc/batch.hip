@@ -189,6 +189,8 @@ int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
       out->classify_passes += s.classify_passes;
       out->max_regions = std::max(out->max_regions, s.max_regions);
       out->integrals += s.integrals;
+      out->alloc_ms += s.alloc_ms;
+      out->allocations += s.allocations;
     }
   return PAGANI_OK;
 }

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -125,16 +126,22 @@ inline int hip_fail(pagani_ws* ws, hipError_t e, const char* where)
     if (e_ != hipSuccess) return hip_fail(ws, e_, #expr);  \
   } while (0)
 
+// Grow a buffer to at least n elements (contents are not kept: every
+// buffer is rewritten before it is read).  Region lists roughly double per
+// iteration, so a buffer grows to twice the request: it then fits the next
+// iteration as well and is reallocated every other iteration at most.  If
+// that much does not fit, exactly n is tried.
 template <class T>
 int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
 {
   if (n <= b.cap) return PAGANI_OK;
+  const auto t0 = std::chrono::steady_clock::now();
   if (b.p) {
-    hipFree(b.p);
+    (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
   }
-  std::size_t want = std::max(n, b.cap + b.cap / 2);
+  std::size_t want = 2 * n;
   void* p = nullptr;
   if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
     (void)hipGetLastError();
@@ -148,6 +155,9 @@ int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
   }
   b.p = static_cast<T*>(p);
   b.cap = want;
+  ws->stats.alloc_ms +=
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  ws->stats.allocations++;
   return PAGANI_OK;
 }
 

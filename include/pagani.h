@@ -139,6 +139,8 @@ typedef struct pagani_stats {
   uint64_t classify_passes;
   uint64_t max_regions;
   uint64_t integrals;
+  double alloc_ms;      /* host time spent growing device buffers */
+  uint64_t allocations; /* device buffers (re)allocated */
 } pagani_stats;
 
 /* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
