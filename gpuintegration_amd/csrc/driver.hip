@@ -101,34 +101,82 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
 // threshold passes run on the device).
 
 
-// One run of get_larger_threshold_results (heuristic_classifier.cuh:
-// 327-346).  While apply_threshold fails, the reference raises the
-// threshold to t + |high - t| / 2 with `high` fixed, at most 20 times: the
-// whole sequence is known before the first pass, so a single read of the
-// errors counts all of it (k_multi_threshold).  The sequential loop is then
-// replayed on the host with the counts, with the reference's arithmetic.
-// `last_dot` / `last_thr` are the sum of err > t and t of the last applied
-// threshold.
-int larger_threshold(pagani_ws* ws, const Classifier& c, const double* err, std::size_t n,
-                     pagani_class_res& res, double& last_dot, double& last_thr)
-{
-  ph::ThresholdSet T;
-  double t = res.threshold;
-  for (int k = 0; k < ph::kMaxThresholds; ++k) {
-    T.t[k] = t;
-    t += std::fabs(res.range_high - t) * .5;
+// The classifier's threshold search (heuristic_classifier.cuh:282-451)
+// needs, per candidate threshold t, the number of errors above t and their
+// sum.  Every candidate the search visits is a fixed function of earlier
+// ones, so when a candidate has not been measured yet, one pass measures it
+// together with the candidates the search would visit next:
+//   - the rest of a get_larger_threshold_results run from t: t_{k+1} =
+//     t_k + |high - t_k| / 2 (:327-346), and
+//   - the lowerings that follow when t passes the memory test but not the
+//     error budget: t_{j+1} = t_j - |low - t_j| / 2 (:404-410).
+// The search is then replayed on the host with the reference's arithmetic,
+// reading measured values from the cache; a candidate outside it costs
+// another pass.  Decisions are those of the sequential search.
+constexpr int kRaiseLimit = 20;  // :335
+
+struct ThresholdCache {
+  std::vector<double> t, dot;
+  std::vector<std::uint64_t> cnt;
+  bool find(double x, std::uint64_t& c, double& d) const
+  {
+    for (std::size_t i = 0; i < t.size(); ++i)
+      if (t[i] == x) {
+        c = cnt[i];
+        d = dot[i];
+        return true;
+      }
+    return false;
   }
-  T.k = ph::kMaxThresholds;
+};
+
+int measure_threshold(pagani_ws* ws, const double* err, std::size_t n, double t, double low,
+                      double high, ThresholdCache& cache, std::uint64_t& c, double& d)
+{
+  if (cache.find(t, c, d)) return PAGANI_OK;
+  ph::ThresholdSet T;
+  int k = 0;
+  double x = t;
+  for (int i = 0; i < kRaiseLimit && k < ph::kMaxThresholds; ++i) {
+    T.t[k++] = x;
+    x += std::fabs(high - x) * .5;
+  }
+  x = t;
+  while (k < ph::kMaxThresholds) {
+    x -= std::fabs(low - x) * .5;
+    T.t[k++] = x;
+  }
+  T.k = k;
   HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, T, ws->d_mpart, ws->d_mout));
-  HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * ph::kMaxThresholds * sizeof(double),
+  HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * k * sizeof(double),
                              hipMemcpyDeviceToHost, ws->stream));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   ws->stats.classify_passes++;
+  for (int i = 0; i < k; ++i) {
+    cache.t.push_back(T.t[i]);
+    cache.cnt.push_back(static_cast<std::uint64_t>(ws->h_mout[i]));
+    cache.dot.push_back(ws->h_mout[k + i]);
+  }
+  c = static_cast<std::uint64_t>(ws->h_mout[0]);
+  d = ws->h_mout[k];
+  return PAGANI_OK;
+}
+
+// get_larger_threshold_results (:327-346) with apply_threshold (:282-303).
+// `last_dot` / `last_thr` are the sum of err > t and t of the last applied
+// threshold.
+int larger_threshold(pagani_ws* ws, const Classifier& c, const double* err, std::size_t n,
+                     ThresholdCache& cache, pagani_class_res& res, double& last_dot,
+                     double& last_thr)
+{
   res.pass_mem = 0;
-  for (int k = 0; k < ph::kMaxThresholds && !res.pass_mem; ++k) {
-    // apply_threshold at res.threshold (== T.t[k])
-    const std::size_t cnt = static_cast<std::size_t>(ws->h_mout[k]);
-    last_dot = ws->h_mout[ph::kMaxThresholds + k];
+  for (int k = 0; k < kRaiseLimit && !res.pass_mem; ++k) {
+    std::uint64_t cnt = 0;
+    double dot = 0.0;
+    const int rc = measure_threshold(ws, err, n, res.threshold, res.range_low, res.range_high,
+                                     cache, cnt, dot);
+    if (rc) return rc;
+    last_dot = dot;
     last_thr = res.threshold;
     res.threshold_passes++;
     res.num_active = cnt;
@@ -144,10 +192,9 @@ int larger_threshold(pagani_ws* ws, const Classifier& c, const double* err, std:
   return PAGANI_OK;
 }
 
-// classify, :362-451 (with apply_threshold :282-303,
-// get_larger_threshold_results :327-346, evaluate_error_budget :305-325).
-// The flags of the last applied threshold (*last_thr) are written by the
-// caller with k_threshold_flags when it needs them.
+// classify, :362-451 (with evaluate_error_budget :305-325).  The flags of
+// the last applied threshold (*last_thr) are written by the caller with
+// k_threshold_flags when it needs them.
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
              double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr)
 {
@@ -165,9 +212,10 @@ int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, dou
   std::size_t n_inc = 0, n_dec = 0;
   double last_dot = 0.;
   *last_thr = res.threshold;
+  ThresholdCache cache;
   do {
     if (!res.pass_mem && n_inc <= 20) {
-      rc = larger_threshold(ws, c, err, n, res, last_dot, *last_thr);
+      rc = larger_threshold(ws, c, err, n, cache, res, last_dot, *last_thr);
       if (rc) return rc;
       n_inc++;
     }

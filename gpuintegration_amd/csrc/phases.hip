@@ -221,21 +221,23 @@ __global__ void __launch_bounds__(kThreads)
 // threshold, the count of err > t_k (apply_threshold, :282-303) and the sum
 // of those errors (the dot product of evaluate_error_budget, :305-325).
 // Fixed grid, grid-stride order, fixed reduction trees: deterministic.
+template <int K>
 __global__ void __launch_bounds__(kThreads)
     k_multi_threshold(std::size_t n, const double* __restrict__ err, const ThresholdSet T,
                       double* __restrict__ bpart)
 {
   __shared__ DD sh[kThreads / 64];
   __shared__ std::uint32_t shc[kThreads / 64];
-  std::uint32_t cnt[kMaxThresholds];
-  double sum[kMaxThresholds];
+  std::uint32_t cnt[K];
+  double sum[K];
 #pragma unroll
-  for (int k = 0; k < kMaxThresholds; ++k) {
+  for (int k = 0; k < K; ++k) {
     cnt[k] = 0;
     sum[k] = 0.0;
   }
   // Grid-stride order; four loads are issued before their values are
   // consumed (in the same order) so more bytes are in flight per wave.
+  // Unused candidates (k >= T.k) hold +inf and count nothing.
   const std::size_t stride = static_cast<std::size_t>(gridDim.x) * kThreads;
   std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + threadIdx.x;
   for (; t + 3 * stride < n; t += 4 * stride) {
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
 #pragma unroll
-      for (int k = 0; k < kMaxThresholds; ++k) {
+      for (int k = 0; k < K; ++k) {
         const bool a = r[i] > T.t[k];
         cnt[k] += a ? 1u : 0u;
         sum[k] += a ? r[i] : 0.0;
@@ -255,7 +257,7 @@ __global__ void __launch_bounds__(kThreads)
   for (; t < n; t += stride) {
     const double r = err[t];
 #pragma unroll
-    for (int k = 0; k < kMaxThresholds; ++k) {
+    for (int k = 0; k < K; ++k) {
       const bool a = r > T.t[k];
       cnt[k] += a ? 1u : 0u;
       sum[k] += a ? r : 0.0;
@@ -263,7 +265,7 @@ __global__ void __launch_bounds__(kThreads)
   }
   double* out = bpart + static_cast<std::size_t>(blockIdx.x) * 3 * kMaxThresholds;
 #pragma unroll
-  for (int k = 0; k < kMaxThresholds; ++k) {
+  for (int k = 0; k < K; ++k) {
     if (k < T.k) {  // uniform
       const DD s = block_reduce(DD{sum[k], 0.0}, sh);
       const std::uint32_t c = block_count(cnt[k], shc);
@@ -276,10 +278,10 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Fold the kMultiBlocks partials of k_multi_threshold: eight lanes per
-// threshold, lane j folds blocks j, j+8, j+16, ... in order, then a fixed
-// butterfly over the eight.
-constexpr int kMultiLanes = 8;
+// Fold the kMultiBlocks partials of k_multi_threshold: four lanes per
+// threshold, lane j folds blocks j, j+4, j+8, ... in order, then a fixed
+// butterfly over the four.
+constexpr int kMultiLanes = 4;
 static_assert(kMaxThresholds * kMultiLanes <= kThreads, "one group of lanes per threshold");
 
 __global__ void __launch_bounds__(kThreads)
@@ -760,8 +762,14 @@ hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err, cons
                            double* bpart, double* out)
 {
   if (T.k < 1 || T.k > kMaxThresholds) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_multi_threshold, dim3(kMultiBlocks), dim3(kThreads), 0, s, n, err, T,
-                     bpart);
+  ThresholdSet U = T;
+  for (int k = U.k; k < kMaxThresholds; ++k) U.t[k] = __builtin_inf();
+  if (U.k <= kMaxThresholds / 2)
+    hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds / 2>, dim3(kMultiBlocks), dim3(kThreads),
+                       0, s, n, err, U, bpart);
+  else
+    hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds>, dim3(kMultiBlocks), dim3(kThreads), 0,
+                       s, n, err, U, bpart);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_multi_reduce, dim3(1), dim3(kThreads), 0, s, kMultiBlocks, T.k, bpart,

@@ -25,10 +25,10 @@ constexpr std::size_t kFold = 256;
 inline std::size_t num_chunks(std::size_t n) { return (n + kChunk - 1) / kChunk; }
 inline std::size_t num_folds(std::size_t nchunks) { return (nchunks + kFold - 1) / kFold; }
 
-// The heuristic classifier's threshold search raises the threshold at most
-// 20 times in a row (heuristic_classifier.cuh:335); all of a run's
-// candidates are known in advance, so one pass evaluates them together.
-constexpr int kMaxThresholds = 20;
+// Candidate thresholds measured by one pass of the classifier probe: the
+// search's next 20 raises (heuristic_classifier.cuh:335 caps a run at 20)
+// and the next 20 lowerings, all computable before the pass.
+constexpr int kMaxThresholds = 40;
 struct ThresholdSet {
   double t[kMaxThresholds];
   int k;
