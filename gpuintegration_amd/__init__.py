@@ -182,6 +182,8 @@ _SIGS = {
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            _vp, ctypes.POINTER(ClassRes)]),
     "do_integration_from_c": (ctypes.c_int, [_dp]),
+    "pagani_ws_footprint": (_sz, [_vp]),
+    "pagani_ws_trim": (ctypes.c_int, [_vp]),
     "pagani_ws_set_iteration_limit": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "pagani_ws_suspended": (ctypes.c_int, [_vp]),
     "pagani_integrate_cb_resume": (ctypes.c_int, [_vp, _vp, _vp, ctypes.POINTER(_Result)]),
@@ -404,6 +406,14 @@ class Workspace:
         if rc < 0 and rc != ENOMEM:
             self._check(rc)
         return IntegrationResult._from(res)
+
+    def footprint(self):
+        """Device bytes held by the region pool."""
+        return self.lib.pagani_ws_footprint(self._ws)
+
+    def trim(self):
+        """Release the region pool (it regrows on the next call)."""
+        self._check(self.lib.pagani_ws_trim(self._ws))
 
     # -- suspend / checkpoint / resume ----------------------------------------
     def set_iteration_limit(self, k):

@@ -234,8 +234,17 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
       }
       return;
     }
+    // Items come in decreasing dimension, so once a worker moves on to a
+    // smaller d it never uses the previous workspace again in this call: its
+    // pool is released, and idle pools never crowd out the running one.
+    int last_d = 0;
     for (std::size_t k; (k = next.fetch_add(1)) < count;) {
       const std::size_t i = order[k];
+      const int d = items[i].f.ndim;
+      if (last_d && d != last_d && last_d >= PAGANI_MIN_DIM && last_d <= PAGANI_MAX_DIM &&
+          b->ws[w][last_d])
+        (void)pagani_ws_trim(b->ws[w][last_d]);
+      last_d = d;
       codes[i] = run_item(b, w, items[i], &results[i]);
     }
   };

@@ -825,6 +825,28 @@ int pagani_integrate_genz(pagani_ws* ws, const pagani_genz* f, int precision, do
                         relerr_classification, nullptr, init_k, nullptr, f, precision, out);
 }
 
+size_t pagani_ws_footprint(const pagani_ws* ws)
+{
+  if (!ws) return 0;
+  std::size_t bytes = 0;
+  for_each_pool(const_cast<pagani_ws*>(ws), [&](auto& b) { bytes += b.cap * sizeof(*b.p); });
+  return bytes;
+}
+
+int pagani_ws_trim(pagani_ws* ws)
+{
+  if (!ws) return PAGANI_EBADARG;
+  if (ws->susp.held) return fail(ws, PAGANI_EBADARG, "a suspended run holds the region pool");
+  if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  harvest_compact(ws);
+  for_each_pool(ws, [](auto& b) { release(b); });
+  ws->live_n = 0;
+  ws->live_evaluated = false;
+  ws->snap_n = 0;
+  return PAGANI_OK;
+}
+
 int pagani_ws_set_iteration_limit(pagani_ws* ws, uint64_t iterations)
 {
   if (!ws) return PAGANI_EBADARG;

@@ -128,9 +128,11 @@ inline int hip_fail(pagani_ws* ws, hipError_t e, const char* where)
 
 // Grow a buffer to at least n elements (contents are not kept: every
 // buffer is rewritten before it is read).  Region lists roughly double per
-// iteration, so a buffer grows to twice the request: it then fits the next
-// iteration as well and is reallocated every other iteration at most.  If
-// that much does not fit, exactly n is tried.
+// iteration, so while memory is plentiful a buffer grows to twice the
+// request: it then fits the next iteration as well and is reallocated every
+// other iteration at most.  Once twice the request would take more than an
+// eighth of the free device memory, buffers are sized exactly, so headroom
+// kept by one buffer never starves the next.
 template <class T>
 int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
 {
@@ -141,7 +143,13 @@ int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
     b.p = nullptr;
     b.cap = 0;
   }
-  std::size_t want = 2 * n;
+  std::size_t want = n;
+  std::size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+    if (2 * n * sizeof(T) <= free_b / 8) want = 2 * n;
+  } else {
+    (void)hipGetLastError();
+  }
   void* p = nullptr;
   if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) {
     (void)hipGetLastError();
@@ -164,9 +172,35 @@ int ensure(pagani_ws* ws, DevBuf<T>& b, std::size_t n, const char* what)
 template <class T>
 inline void release(DevBuf<T>& b)
 {
-  if (b.p) hipFree(b.p);
+  if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
+}
+
+// Every growable buffer of the workspace, for trimming and accounting.
+template <class F>
+inline void for_each_pool(pagani_ws* ws, F&& f)
+{
+  f(ws->L[0]);
+  f(ws->L[1]);
+  f(ws->H[0]);
+  f(ws->H[1]);
+  f(ws->est);
+  f(ws->err0);
+  f(ws->err1);
+  f(ws->pest);
+  f(ws->perr);
+  f(ws->dim);
+  f(ws->active);
+  f(ws->flags);
+  f(ws->part);
+  f(ws->fpart);
+  f(ws->cnt);
+  f(ws->fcnt);
+  f(ws->off);
+  f(ws->foff);
+  f(ws->snapL);
+  f(ws->snapH);
 }
 
 inline int ensure_chunks(pagani_ws* ws, std::size_t n)

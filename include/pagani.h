@@ -235,6 +235,13 @@ int pagani_integrate_cb(pagani_ws* ws, pagani_eval_fn eval, void* ctx,
                         const pagani_region_list* init, size_t init_k,
                         pagani_result* out);
 
+/* Device bytes held by the workspace's region pool, and their release (the
+ * pool regrows on the next call).  The reference allocates and frees its
+ * buffers inside every integrate call; a workspace keeps them between calls
+ * unless trimmed.  Trimming fails while a suspended run holds the pool. */
+size_t pagani_ws_footprint(const pagani_ws* ws);
+int pagani_ws_trim(pagani_ws* ws);
+
 /* ---- suspend, checkpoint, resume ----------------------------------------- */
 /* No reference counterpart beyond the in-memory region-list copy
  * (Sub_regions::take_snapshot / load_snapshot, Sub_regions.cuh:220-239): a

@@ -101,3 +101,20 @@ def test_batch_bad_item_fails_alone(pg, gpu):
         assert b"family" in lib.pagani_batch_last_error(b._b)
         assert all(codes[i] >= 0 for i in range(len(items)) if i != 3)
         assert res[3].status == 1
+
+
+def test_trim_releases_pool(pg, gpu):
+    fam, a, u, eps = _items(4, seed=13)[0]
+    with pg.Workspace(len(a), 0) as ws:
+        r1 = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20)
+        assert ws.footprint() > 0
+        ws.trim()
+        assert ws.footprint() == 0
+        r2 = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20)
+        # a suspended run keeps its pool
+        ws.set_iteration_limit(1)
+        ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20)
+        if ws.suspended():
+            with pytest.raises(pg.PaganiError):
+                ws.trim()
+    assert (r1.estimate, r1.errorest, r1.nregions) == (r2.estimate, r2.errorest, r2.nregions)
