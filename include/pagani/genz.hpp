@@ -20,6 +20,37 @@
 
 namespace pagani {
 namespace math {
+
+// e^x for x <= 0, the argument range of the Gaussian and C0 families (their
+// exponent is minus a sum of non-negative terms).  x = n ln2 + r with
+// n = rint(x / ln2) and |r| <= ln2/2, r from a two-part ln2 (Cody-Waite);
+// e^r by a degree-11 polynomial; 2^n by ldexp, which also rounds
+// subnormal results and gives 0 below about -745.1.  The coefficients
+// interpolate e^r at the 12 Chebyshev nodes of [-ln2/2, ln2/2] (computed in
+// extended precision, then rounded to double; the check that recomputes
+// them is in the tests): max polynomial error 0.14 ulp.  With no overflow
+// to guard against, the range handling is one clamp, which also maps -inf
+// to 0 (and NaN to 0: the families never produce NaN from finite input).
+__device__ __forceinline__ double exp_neg(double x)
+{
+  constexpr double kLog2e = 0x1.71547652b82fep+0;
+  constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
+  constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
+  constexpr double c[12] = {
+      0x1.0000000000000p+0,  0x1.0000000000000p+0,  0x1.0000000000011p-1, 0x1.555555555555fp-3,
+      0x1.555555554f0c7p-5,  0x1.111111110e636p-7,  0x1.6c16c187face9p-10, 0x1.a01a01b4a99f9p-13,
+      0x1.a01991b378c3ap-16, 0x1.71ddeea4df7bbp-19, 0x1.28b3fce70e98ep-22, 0x1.af6d18013a587p-26};
+  x = __builtin_fmax(x, -1.0e4);
+  const double n = __builtin_rint(x * kLog2e);
+  double r = __builtin_fma(-n, kLn2Hi, x);
+  r = __builtin_fma(-n, kLn2Lo, r);
+  double p = c[11];
+#pragma unroll
+  for (int k = 10; k >= 0; --k) p = __builtin_fma(p, r, c[k]);
+  return __builtin_ldexp(p, static_cast<int>(n));
+}
+__device__ __forceinline__ float exp_neg(float x) { return ::expf(x); }
+
 __device__ __forceinline__ double exp_(double x) { return ::exp(x); }
 __device__ __forceinline__ float exp_(float x) { return ::expf(x); }
 __device__ __forceinline__ double cos_(double x) { return ::cos(x); }
@@ -110,7 +141,7 @@ struct Gaussian {
       const Real t = x[i] - u[i];
       s += a2[i] * (t * t);
     }
-    return math::exp_(-s);
+    return math::exp_neg(-s);
   }
   __device__ double origin(int i) const { return static_cast<double>(u[i]); }
   template <class... T>
@@ -120,7 +151,7 @@ struct Gaussian {
     Real s = a2[0] * (t[0] * t[0]);
 #pragma unroll
     for (int i = 1; i < D; ++i) s += a2[i] * (t[i] * t[i]);
-    return math::exp_(-s);
+    return math::exp_neg(-s);
   }
 };
 
@@ -136,7 +167,7 @@ struct C0 {
     Real s = 0;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += a[i] * math::fabs_(x[i] - u[i]);
-    return math::exp_(-s);
+    return math::exp_neg(-s);
   }
   __device__ double origin(int i) const { return static_cast<double>(u[i]); }
   template <class... T>
@@ -146,7 +177,7 @@ struct C0 {
     Real s = a[0] * math::fabs_(t[0]);
 #pragma unroll
     for (int i = 1; i < D; ++i) s += a[i] * math::fabs_(t[i]);
-    return math::exp_(-s);
+    return math::exp_neg(-s);
   }
 };
 

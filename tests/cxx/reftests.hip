@@ -509,3 +509,35 @@ extern "C" int reftest_interp_mem_usage(int runs, std::size_t* free_before,
     return -1;
   }
 }
+
+// ---------------------------------------------------------------------------
+// The non-positive-argument exponential of the Gaussian and C0 families
+// (pagani::math::exp_neg) beside the device library's exp.
+
+#include "pagani/genz.hpp"
+
+namespace {
+__global__ void k_exp_neg(const double* x, double* mine, double* lib, std::size_t n)
+{
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    mine[i] = pagani::math::exp_neg(x[i]);
+    lib[i] = ::exp(x[i]);
+  }
+}
+}  // namespace
+
+extern "C" int reftest_exp_neg(const double* x, std::size_t n, double* mine, double* lib)
+{
+  try {
+    DevVec<double> dx(n), dm(n), dl(n);
+    dx.put(x, n);
+    hipLaunchKernelGGL(k_exp_neg, dim3(blocks_for(n)), dim3(256), 0, 0, dx.p, dm.p, dl.p, n);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    dm.get(mine, n);
+    dl.get(lib, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
