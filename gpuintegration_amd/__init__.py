@@ -111,6 +111,10 @@ class Stats(ctypes.Structure):
         ("integrals", ctypes.c_uint64),
         ("alloc_ms", ctypes.c_double),
         ("allocations", ctypes.c_uint64),
+        ("device_ms", ctypes.c_double),
+        ("device_iterations", ctypes.c_uint64),
+        ("device_regions", ctypes.c_uint64),
+        ("device_rounds", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -150,6 +154,7 @@ _SIGS = {
     "pagani_ws_device_mem": (_sz, [_vp]),
     "pagani_ws_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_device_loop": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "pagani_ws_reset_stats": (ctypes.c_int, [_vp]),
@@ -359,6 +364,10 @@ class Workspace:
         1 / "actual": this library's own footprint."""
         m = {"reference": 0, "actual": 1}.get(model, model)
         self._check(self.lib.pagani_ws_set_mem_model(self._ws, int(m)))
+
+    def set_device_loop(self, on=True):
+        """Run small iterations on the device in rounds (default on)."""
+        self._check(self.lib.pagani_ws_set_device_loop(self._ws, 1 if on else 0))
 
     def set_slices(self, s):
         self._check(self.lib.pagani_ws_set_slices(self._ws, int(s)))

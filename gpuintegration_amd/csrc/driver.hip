@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "pagani.h"
+#include "pagani/eval_kernel.hpp"
 #include "pagani_ext.h"
 #include "phases.hpp"
 #include "rule.hpp"
@@ -44,14 +45,11 @@ void harvest_compact(pagani_ws* ws)
 
 int choose_slices(const pagani_ws* ws, std::size_t n)
 {
-  if (ws->slices_override > 0) return std::min(ws->slices_override, PAGANI_MAX_SLICES);
-  const std::size_t batches = (n + 63) / 64;
-  const std::size_t target = static_cast<std::size_t>(ws->num_cus) * 4 * 4;
-  const int generic = ws->rule.feval - ws->rule.set_begin[3];
-  int s = 1;
-  while (s < PAGANI_MAX_SLICES && batches * s < target && generic / (2 * s) >= 16) s *= 2;
-  return s;
+  return choose_slices_n(n, ws->num_cus, ws->slices_override,
+                         ws->rule.feval - ws->rule.set_begin[3]);
 }
+
+void set_slices(pagani_eval_consts& C, int S);
 
 void make_consts(const pagani_ws* ws, const double* lows, const double* highs, std::size_t n,
                  pagani_eval_consts& C)
@@ -77,22 +75,20 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
   C.ndim = d;
   C.feval = R.feval;
   for (int s = 0; s <= PAGANI_NSETS; ++s) C.set_begin[s] = R.set_begin[s];
-  const int S = choose_slices(ws, n);
+  set_slices(C, choose_slices(ws, n));
+}
+
+// The slice count of a launch and its split of the generic points (the
+// split itself is pagani::slice_range; slice_begin records it).
+void set_slices(pagani_eval_consts& C, int S)
+{
   C.slices = S;
-  const int P0 = R.set_begin[3], F = R.feval, G = F - P0;
-  for (int i = 0; i <= PAGANI_MAX_SLICES; ++i) C.slice_begin[i] = F;
-  C.slice_begin[0] = P0;
-  if (S == 1) {
-    C.slice_begin[1] = F;
-  } else {
-    // wave 0 also evaluates the 1+4d points of sets 0-2
-    const int share0 = std::max(0, F / S - P0);
-    const int rest = G - share0;
-    C.slice_begin[1] = P0 + share0;
-    for (int w = 1; w < S; ++w) {
-      const int sh = rest / (S - 1) + ((w - 1) < rest % (S - 1) ? 1 : 0);
-      C.slice_begin[w + 1] = C.slice_begin[w] + sh;
-    }
+  for (int i = 0; i <= PAGANI_MAX_SLICES; ++i) C.slice_begin[i] = C.feval;
+  for (int w = 0; w < S; ++w) {
+    int pa, pb;
+    pagani::slice_range(S, w, C.set_begin[3], C.feval, pa, pb);
+    C.slice_begin[w] = pa;
+    C.slice_begin[w + 1] = pb;
   }
 }
 
@@ -274,6 +270,167 @@ void suspend(pagani_ws* ws, const LoopState& st)
   ws->susp.held = true;
 }
 
+// Grow a buffer to at least `need` elements keeping its first `keep`.
+template <class T>
+int grow_keep(pagani_ws* ws, DevBuf<T>& b, std::size_t need, std::size_t keep, const char* what)
+{
+  if (need <= b.cap) return PAGANI_OK;
+  DevBuf<T> nb;
+  int rc = ensure(ws, nb, need, what);
+  if (rc) return rc;
+  if (keep && b.p)
+    HIP_TRY(ws, hipMemcpyAsync(nb.p, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, ws->stream));
+  HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+  release(b);
+  b = nb;
+  return PAGANI_OK;
+}
+
+// What a device-resident segment ended with (see run_device_segment).
+enum DevOutcome { kSegHost, kSegDone, kSegEmpty, kSegLimit };
+
+// The loop variables the host loop and a device segment hand to each other.
+struct LoopVars {
+  std::uint64_t it;
+  std::size_t n;
+  bool has_parents;
+  pagani_result* cum;
+  Classifier* clf;
+};
+
+// Run iterations on the device, in rounds of 2, 4, ..., 64 queued
+// iterations with one host round trip each, until the device loop stops.
+// Each iteration: evaluation, refinement, the reductions, k_decide, filter
+// + split, all with fixed launch shapes for kDevLoopMax regions, reading
+// the counts from the DevLoop descriptors.  On return the loop variables
+// describe where the host continues; with kSegHost the iteration `v.it`
+// has been evaluated and reduced (its sums are in d_scal) and the host
+// finishes it.
+int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
+                       double epsabs, int relerr, bool user_regions, const double* lows,
+                       const double* highs, LoopVars& v, DevOutcome& outcome)
+{
+  const int d = ws->ndim;
+  const std::size_t cap = ph::kDevLoopMax;
+  const int cur = ws->cur;
+  int rc;
+  // the live list and the parents' estimates are kept; everything else is
+  // sized for the capacity
+  if ((rc = grow_keep(ws, ws->L[cur], cap * d, v.n * d, "region bounds")) ||
+      (rc = grow_keep(ws, ws->H[cur], cap * d, v.n * d, "region bounds")) ||
+      (rc = grow_keep(ws, ws->pest, cap / 2, v.has_parents ? v.n / 2 : 0, "parent estimates")) ||
+      (rc = ensure(ws, ws->L[1 - cur], cap * d, "region bounds")) ||
+      (rc = ensure(ws, ws->H[1 - cur], cap * d, "region bounds")) ||
+      (rc = ensure(ws, ws->perr, cap / 2, "parent errors")) ||
+      (rc = ensure(ws, ws->est, cap, "estimates")) || (rc = ensure(ws, ws->err0, cap, "errors")) ||
+      (rc = ensure(ws, ws->err1, cap, "errors")) || (rc = ensure(ws, ws->dim, cap, "split axes")) ||
+      (rc = ensure(ws, ws->active, cap, "flags")) || (rc = ensure_chunks(ws, cap)))
+    return rc;
+
+  DevLoop& h = *ws->h_loop;
+  std::memset(&h, 0, sizeof(h));
+  h.it.n = v.n;
+  h.it.cur = static_cast<std::uint32_t>(cur);
+  h.it.run = 1;
+  h.it.has_parents = v.has_parents ? 1u : 0u;
+  h.it.slices = static_cast<std::uint32_t>(choose_slices(ws, v.n));
+  h.num_cus = ws->num_cus;
+  h.slices_override = ws->slices_override;
+  h.generic = ws->rule.feval - ws->rule.set_begin[3];
+  h.n_cap = cap;
+  h.mem_total = v.clf->total;
+  h.epsrel = epsrel;
+  h.epsabs = epsabs;
+  h.d = d;
+  h.mem_model = v.clf->model;
+  h.relerr = relerr;
+  h.user_regions = user_regions ? 1 : 0;
+  h.iter = v.it;
+  h.cum_est = v.cum->estimate;
+  h.cum_err = v.cum->errorest;
+  h.cum_nregions = v.cum->nregions;
+  h.cum_iters = v.cum->iters;
+  h.cum_status = v.cum->status;
+  h.status = kDevRun;
+  for (int i = 0; i < 3; ++i) h.hist[i] = v.clf->hist[i];
+  h.iters_collected = v.clf->iters_collected;
+
+  pagani_eval_args args{};
+  args.n = cap;
+  args.L = ws->L[0].p;
+  args.H = ws->H[0].p;
+  args.L2 = ws->L[1].p;
+  args.H2 = ws->H[1].p;
+  args.est = ws->est.p;
+  args.err = ws->err0.p;
+  args.dim = ws->dim.p;
+  args.stream = ws->stream;
+  args.dev = &ws->d_loop->it;
+  make_consts(ws, lows, highs, cap, args.consts);
+
+  for (int queued = 2;; queued = std::min(queued * 2, kDevTrace)) {
+    // A round's launches have as many waves per workgroup as its first
+    // iteration's slice count; k_decide pauses the round (it.run = 0 with
+    // status kDevRun) before an iteration that would need more.
+    h.it.run = 1;
+    h.launch_slices = static_cast<std::int32_t>(h.it.slices);
+    set_slices(args.consts, h.launch_slices);
+    HIP_TRY(ws, hipMemcpyAsync(ws->d_loop, ws->h_loop, sizeof(DevLoop), hipMemcpyHostToDevice,
+                               ws->stream));
+    if (ws->timing) record(ws, EV_DEV0);
+    for (int q = 0; q < queued; ++q) {
+      rc = eval(ctx, &args);
+      if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
+      HIP_TRY(ws, ph::refine_dev(ws->stream, cap, &ws->d_loop->it, ws->est.p, ws->err0.p,
+                                 ws->err1.p, ws->pest.p, epsrel, relerr, ws->active.p,
+                                 ws->part.p, ws->cnt.p));
+      HIP_TRY(ws, ph::finalize_dev(ws->stream, cap, &ws->d_loop->it, ws->part.p, ws->cnt.p,
+                                   ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p, ws->d_scal));
+      HIP_TRY(ws, launch_decide(ws->stream, ws->d_loop, ws->d_scal));
+      HIP_TRY(ws, ph::compact_dev(ws->stream, cap, &ws->d_loop->cp, d, ws->L[0].p, ws->H[0].p,
+                                  ws->L[1].p, ws->H[1].p, ws->active.p, ws->est.p, ws->err0.p,
+                                  ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,
+                                  ws->perr.p));
+    }
+    if (ws->timing) record(ws, EV_DEV1);
+    HIP_TRY(ws, hipMemcpyAsync(ws->h_loop, ws->d_loop, sizeof(DevLoop), hipMemcpyDeviceToHost,
+                               ws->stream));
+    HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+    if (ws->timing) ws->stats.device_ms += elapsed(ws, EV_DEV0, EV_DEV1);
+    ws->stats.device_rounds++;
+    ws->stats.device_iterations += h.iterations;
+    ws->stats.device_regions += h.evaluated;
+    ws->stats.regions_evaluated += h.evaluated;
+    ws->stats.eval_launches += h.iterations;
+    ws->stats.iterations += h.iterations;
+    ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, h.max_n);
+    for (std::uint32_t i = 0; i < h.ntrace; ++i) ws->trace.push_back(h.trace[i]);
+    h.ntrace = 0;
+    h.evaluated = 0;
+    h.iterations = 0;
+    if (h.status != kDevRun) break;
+  }
+
+  v.it = h.iter;
+  v.n = h.it.n;
+  v.has_parents = h.it.has_parents != 0;
+  ws->cur = static_cast<int>(h.it.cur);
+  v.cum->estimate = h.cum_est;
+  v.cum->errorest = h.cum_err;
+  v.cum->nregions = h.cum_nregions;
+  v.cum->iters = h.cum_iters;
+  v.cum->status = h.cum_status;
+  for (int i = 0; i < 3; ++i) v.clf->hist[i] = h.hist[i];
+  v.clf->iters_collected = h.iters_collected;
+  switch (h.status) {
+    case kDevDone: outcome = kSegDone; break;
+    case kDevDoneEmpty: outcome = kSegEmpty; break;
+    case kDevLimit: outcome = kSegLimit; break;
+    default: outcome = kSegHost; break;
+  }
+  return PAGANI_OK;
+}
+
 // The iteration loop, Workspace.cuh:148-358 / :362-494.  `resume` continues
 // a suspended or loaded run instead of starting one (then epsrel, epsabs,
 // lows, highs, relerr, init and init_k are taken from it).
@@ -383,47 +540,72 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       *out = cum;
       return PAGANI_SUSPENDED;
     }
-    iters_this_call++;
-    if ((rc = ensure(ws, ws->est, n, "estimates")) || (rc = ensure(ws, ws->err0, n, "errors")) ||
-        (rc = ensure(ws, ws->err1, n, "errors")) || (rc = ensure(ws, ws->dim, n, "split axes")) ||
-        (rc = ensure(ws, ws->active, n, "flags")) || (rc = ensure_chunks(ws, n)))
-      return rc;
+    // Small lists far from the memory budget: iterate on the device until
+    // it needs the host (or finishes).
+    bool reduced = false;  // this iteration's sums are already in d_scal
+    if (ws->device_loop && !ws->iter_limit && ws->snapshot_iter == ~std::uint64_t(0) &&
+        n <= ph::kDevLoopMax && mem_ratio(n, d, total, mem_model) <= .1) {
+      LoopVars v{it, n, has_parents, &cum, &clf};
+      DevOutcome outcome;
+      if ((rc = run_device_segment(ws, eval, ctx, epsrel, epsabs, relerr, user_regions, lows,
+                                   highs, v, outcome)))
+        return rc;
+      it = v.it;
+      n = v.n;
+      has_parents = v.has_parents;
+      if (outcome == kSegDone) {
+        errs = has_parents ? ws->err1.p : ws->err0.p;
+        goto done_evaluated;
+      }
+      if (outcome == kSegEmpty || outcome == kSegLimit) break;
+      reduced = true;
+    }
+    if (!reduced) {
+      iters_this_call++;
+      if ((rc = ensure(ws, ws->est, n, "estimates")) || (rc = ensure(ws, ws->err0, n, "errors")) ||
+          (rc = ensure(ws, ws->err1, n, "errors")) || (rc = ensure(ws, ws->dim, n, "split axes")) ||
+          (rc = ensure(ws, ws->active, n, "flags")) || (rc = ensure_chunks(ws, n)))
+        return rc;
 
-    // evaluate (Cubature_rules::apply_cubature_integration_rules)
-    pagani_eval_args args;
-    args.n = n;
-    args.L = ws->L[ws->cur].p;
-    args.H = ws->H[ws->cur].p;
-    args.est = ws->est.p;
-    args.err = ws->err0.p;
-    args.dim = ws->dim.p;
-    args.stream = ws->stream;
-    make_consts(ws, lows, highs, n, args.consts);
-    record(ws, EV_EVAL0);
-    rc = eval(ctx, &args);
-    if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
-    record(ws, EV_EVAL1);
-    ws->stats.regions_evaluated += n;
-    ws->stats.eval_launches++;
-    ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, n);
+      // evaluate (Cubature_rules::apply_cubature_integration_rules)
+      pagani_eval_args args{};  // dev = NULL: an ordinary launch
+      args.n = n;
+      args.L = ws->L[ws->cur].p;
+      args.H = ws->H[ws->cur].p;
+      args.est = ws->est.p;
+      args.err = ws->err0.p;
+      args.dim = ws->dim.p;
+      args.stream = ws->stream;
+      make_consts(ws, lows, highs, n, args.consts);
+      record(ws, EV_EVAL0);
+      rc = eval(ctx, &args);
+      if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
+      record(ws, EV_EVAL1);
+      ws->stats.regions_evaluated += n;
+      ws->stats.eval_launches++;
+      ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, n);
 
-    // two-level error + relerr flags + the four region sums + counts
+      // two-level error + relerr flags + the four region sums + counts
+      errs = has_parents ? ws->err1.p : ws->err0.p;
+      record(ws, EV_REF0);
+      HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
+                             has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
+                             ws->part.p, ws->cnt.p));
+      HIP_TRY(ws, launch_finalize(ws, n, true, false));
+      record(ws, EV_REF1);
+    }
     errs = has_parents ? ws->err1.p : ws->err0.p;
-    record(ws, EV_REF0);
-    HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
-                           has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
-                           ws->part.p, ws->cnt.p));
-    HIP_TRY(ws, launch_finalize(ws, n, true, false));
-    record(ws, EV_REF1);
     HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
                                hipMemcpyDeviceToHost, ws->stream));
     HIP_TRY(ws, hipStreamSynchronize(ws->stream));
-    if (ws->timing) {
-      ws->stats.eval_ms += elapsed(ws, EV_EVAL0, EV_EVAL1);
-      ws->stats.refine_ms += elapsed(ws, EV_REF0, EV_REF1);
+    if (!reduced) {
+      if (ws->timing) {
+        ws->stats.eval_ms += elapsed(ws, EV_EVAL0, EV_EVAL1);
+        ws->stats.refine_ms += elapsed(ws, EV_REF0, EV_REF1);
+      }
+      ws->stats.iterations++;
     }
     harvest_compact(ws);
-    ws->stats.iterations++;
 
     const double iter_est = ws->h_scal->sum[0];
     const double iter_err = ws->h_scal->sum[1];
@@ -672,6 +854,10 @@ pagani_ws* pagani_ws_create(int ndim, int device)
   if ((e = hipHostMalloc(&ws->h_mout, 2 * ph::kMaxThresholds * sizeof(double),
                          hipHostMallocDefault)) != hipSuccess)
     return bail(ws, "hipHostMalloc(threshold results)", e);
+  if ((e = hipMalloc(&ws->d_loop, sizeof(DevLoop))) != hipSuccess)
+    return bail(ws, "hipMalloc(device loop state)", e);
+  if ((e = hipHostMalloc(&ws->h_loop, sizeof(DevLoop), hipHostMallocDefault)) != hipSuccess)
+    return bail(ws, "hipHostMalloc(device loop state)", e);
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(ws, "hipEventCreate", e);
   return ws;
@@ -702,6 +888,8 @@ void pagani_ws_destroy(pagani_ws* ws)
   if (ws->d_mpart) hipFree(ws->d_mpart);
   if (ws->d_mout) hipFree(ws->d_mout);
   if (ws->h_mout) hipHostFree(ws->h_mout);
+  if (ws->d_loop) hipFree(ws->d_loop);
+  if (ws->h_loop) hipHostFree(ws->h_loop);
   if (ws->d_gtab) hipFree(ws->d_gtab);
   if (ws->d_scal) hipFree(ws->d_scal);
   if (ws->h_scal) hipHostFree(ws->h_scal);
@@ -737,6 +925,13 @@ size_t pagani_ws_device_mem(const pagani_ws* ws)
 {
   if (!ws) return 0;
   return ws->mem_model_total ? ws->mem_model_total : ws->hbm_total;
+}
+
+int pagani_ws_set_device_loop(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->device_loop = enable != 0;
+  return PAGANI_OK;
 }
 
 int pagani_ws_set_slices(pagani_ws* ws, int slices)
@@ -923,7 +1118,7 @@ int pagani_eval_genz_dev(pagani_ws* ws, const pagani_genz* f, int precision, siz
   if (n == 0) return PAGANI_OK;
   if (!L || !H || !est || !err || !dim) return fail(ws, PAGANI_EBADARG, "null device pointer");
   if ((rc = check_volume(ws, lows, highs))) return rc;
-  pagani_eval_args args;
+  pagani_eval_args args{};  // dev = NULL: an ordinary launch
   args.n = n;
   args.L = L;
   args.H = H;

@@ -113,11 +113,12 @@ __global__ void __launch_bounds__(256) k_uniform_split(int d, std::size_t k, std
 
 // RefineError (Phases.cuh:219-268) and the relerr flag; children t and t+n/2
 // share parent t mod n/2.
-__global__ void __launch_bounds__(kThreads)
-    k_refine(std::size_t n, const double* __restrict__ est, const double* err_in,
-             double* err_out, const double* __restrict__ pest, double epsrel, int relerr,
-             std::uint8_t* __restrict__ active, double* __restrict__ part,
-             std::uint32_t* __restrict__ cnt)
+__device__ __forceinline__ void refine_chunk(std::size_t n, const double* __restrict__ est,
+                                             const double* err_in, double* err_out,
+                                             const double* __restrict__ pest, double epsrel,
+                                             int relerr, std::uint8_t* __restrict__ active,
+                                             double* __restrict__ part,
+                                             std::uint32_t* __restrict__ cnt)
 {
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
   const std::size_t h = n / 2;
@@ -158,6 +159,32 @@ __global__ void __launch_bounds__(kThreads)
     }
   }
   write_record(se, sr, sae, sar, c, part, cnt);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_refine(std::size_t n, const double* __restrict__ est, const double* err_in,
+             double* err_out, const double* __restrict__ pest, double epsrel, int relerr,
+             std::uint8_t* __restrict__ active, double* __restrict__ part,
+             std::uint32_t* __restrict__ cnt)
+{
+  refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt);
+}
+
+// The same for the device-resident loop: n and whether there are parents
+// come from the iteration descriptor (no parents: the raw errors are
+// refined in place, as in the host-driven loop's first iteration).
+__global__ void __launch_bounds__(kThreads)
+    k_refine_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ est,
+                 double* err0, double* err1, const double* __restrict__ pest, double epsrel,
+                 int relerr, std::uint8_t* __restrict__ active, double* __restrict__ part,
+                 std::uint32_t* __restrict__ cnt)
+{
+  if (!it->run) return;
+  const std::size_t n = it->n;
+  if (blockIdx.x >= num_chunks(n)) return;
+  const bool parents = it->has_parents != 0;
+  refine_chunk(n, est, err0, parents ? err1 : err0, parents ? pest : nullptr, epsrel, relerr,
+               active, part, cnt);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -315,10 +342,11 @@ __global__ void __launch_bounds__(kThreads)
 // Stage A of finalize: one workgroup folds kFold chunk records (fixed
 // tree), writes the chunks' offsets relative to the fold and one fold
 // record.
-__global__ void __launch_bounds__(kThreads)
-    k_fold(std::size_t nchunks, const double* __restrict__ part,
-           const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
-           double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt, int with_minmax)
+__device__ __forceinline__ void fold_block(std::size_t nchunks, const double* __restrict__ part,
+                                           const std::uint32_t* __restrict__ cnt,
+                                           std::uint64_t* __restrict__ off,
+                                           double* __restrict__ fpart,
+                                           std::uint32_t* __restrict__ fcnt, int with_minmax)
 {
   __shared__ DD sh[kThreads / 64];
   __shared__ std::uint32_t wtot[kThreads / 64];
@@ -386,6 +414,25 @@ __global__ void __launch_bounds__(kThreads)
     q[8] = mn;
     q[9] = mx;
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_fold(std::size_t nchunks, const double* __restrict__ part,
+           const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+           double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt, int with_minmax)
+{
+  fold_block(nchunks, part, cnt, off, fpart, fcnt, with_minmax);
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_fold_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ part,
+               const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+               double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt)
+{
+  if (!it->run) return;
+  const std::size_t nchunks = num_chunks(it->n);
+  if (blockIdx.x >= num_folds(nchunks)) return;
+  fold_block(nchunks, part, cnt, off, fpart, fcnt, 0);
 }
 
 // device_set_true_for_larger_than + the reductions of apply_threshold and
@@ -480,10 +527,10 @@ __global__ void __launch_bounds__(kThreads)
 // tree combines the threads.
 constexpr int kFinThreads = 1024;
 
-__global__ void __launch_bounds__(kFinThreads)
-    k_finalize(std::size_t nchunks, const double* __restrict__ part,
-               const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
-               Scalars* __restrict__ out, int with_minmax)
+__device__ __forceinline__ void finalize_block(std::size_t nchunks, const double* __restrict__ part,
+                                               const std::uint32_t* __restrict__ cnt,
+                                               std::uint64_t* __restrict__ off,
+                                               Scalars* __restrict__ out, int with_minmax)
 {
   __shared__ std::uint64_t wtot[kFinThreads / 64];
   __shared__ DD tree[kFinThreads];
@@ -556,6 +603,23 @@ __global__ void __launch_bounds__(kFinThreads)
   }
 }
 
+__global__ void __launch_bounds__(kFinThreads)
+    k_finalize(std::size_t nchunks, const double* __restrict__ part,
+               const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+               Scalars* __restrict__ out, int with_minmax)
+{
+  finalize_block(nchunks, part, cnt, off, out, with_minmax);
+}
+
+__global__ void __launch_bounds__(kFinThreads)
+    k_finalize_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ fpart,
+                   const std::uint32_t* __restrict__ fcnt, std::uint64_t* __restrict__ foff,
+                   Scalars* __restrict__ out)
+{
+  if (!it->run) return;
+  finalize_block(num_folds(num_chunks(it->n)), fpart, fcnt, foff, out, 0);
+}
+
 // Chunk-local exclusive position of the active element t (ballot within the
 // wave, wave totals through LDS).  Returns the block total.
 __device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
@@ -581,14 +645,13 @@ __device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
 // alignRegions (Sub_region_filter.cuh:8-47) and, with split, the bisection
 // of divideIntervalsGPU (Sub_region_splitter.cuh:8-46) fused: the active
 // region t at compacted index idx produces children idx and m+idx.
-__global__ void __launch_bounds__(kThreads)
-    k_compact(int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
-              const double* __restrict__ H, const std::uint8_t* __restrict__ active,
-              const double* __restrict__ est, const double* __restrict__ err,
-              const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
-              const std::uint64_t* __restrict__ foff, double* __restrict__ L2,
-              double* __restrict__ H2, double* __restrict__ pest, double* __restrict__ perr,
-              std::uint8_t* __restrict__ dim2)
+__device__ __forceinline__ void compact_chunk(
+    int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
+    const double* __restrict__ H, const std::uint8_t* __restrict__ active,
+    const double* __restrict__ est, const double* __restrict__ err,
+    const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
+    const std::uint64_t* __restrict__ foff, double* __restrict__ L2, double* __restrict__ H2,
+    double* __restrict__ pest, double* __restrict__ perr, std::uint8_t* __restrict__ dim2)
 {
   __shared__ std::uint32_t wsum[kThreads / 64];
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
@@ -634,6 +697,38 @@ __global__ void __launch_bounds__(kThreads)
     }
     run += tot;
   }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_compact(int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
+              const double* __restrict__ H, const std::uint8_t* __restrict__ active,
+              const double* __restrict__ est, const double* __restrict__ err,
+              const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
+              const std::uint64_t* __restrict__ foff, double* __restrict__ L2,
+              double* __restrict__ H2, double* __restrict__ pest, double* __restrict__ perr,
+              std::uint8_t* __restrict__ dim2)
+{
+  compact_chunk(split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr, dim2);
+}
+
+// The device-resident loop's filter + split: the descriptor says whether it
+// runs, the counts, which pair holds the parents (the children go to the
+// other) and which error array the refinement wrote.
+__global__ void __launch_bounds__(kThreads)
+    k_compact_dev(const DevCompact* __restrict__ cp, int d, double* L0, double* H0, double* L1,
+                  double* H1, const std::uint8_t* __restrict__ active,
+                  const double* __restrict__ est, const double* err0, const double* err1,
+                  const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
+                  const std::uint64_t* __restrict__ foff, double* __restrict__ pest,
+                  double* __restrict__ perr)
+{
+  if (!cp->run) return;
+  const std::size_t n = cp->n, m = cp->m;
+  if (blockIdx.x >= num_chunks(n)) return;
+  const bool c1 = cp->cur != 0;
+  compact_chunk(1, d, n, m, c1 ? L1 : L0, c1 ? H1 : H0, active, est,
+                cp->has_parents ? err1 : err0, dim, off, foff, c1 ? L0 : L1, c1 ? H0 : H1, pest,
+                perr, nullptr);
 }
 
 __global__ void __launch_bounds__(256)
@@ -803,6 +898,43 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
   hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
                      split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr,
                      dim2);
+  return hipGetLastError();
+}
+
+hipError_t refine_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
+                      const double* est, double* err0, double* err1, const double* pest,
+                      double epsrel, int relerr, std::uint8_t* active, double* part,
+                      std::uint32_t* cnt)
+{
+  hipLaunchKernelGGL(k_refine_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))),
+                     dim3(kThreads), 0, s, it, est, err0, err1, pest, epsrel, relerr, active,
+                     part, cnt);
+  return hipGetLastError();
+}
+
+hipError_t finalize_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
+                        const double* part, const std::uint32_t* cnt, std::uint64_t* off,
+                        double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out)
+{
+  const std::size_t nfold = num_folds(num_chunks(n_cap));
+  hipLaunchKernelGGL(k_fold_dev, dim3(static_cast<unsigned>(nfold)), dim3(kThreads), 0, s, it,
+                     part, cnt, off, fpart, fcnt);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finalize_dev, dim3(1), dim3(kFinThreads), 0, s, it, fpart, fcnt, foff,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, int d,
+                       double* L0, double* H0, double* L1, double* H1,
+                       const std::uint8_t* active, const double* est, const double* err0,
+                       const double* err1, const std::uint8_t* dim, const std::uint64_t* off,
+                       const std::uint64_t* foff, double* pest, double* perr)
+{
+  hipLaunchKernelGGL(k_compact_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))),
+                     dim3(kThreads), 0, s, cp, d, L0, H0, L1, H1, active, est, err0, err1, dim,
+                     off, foff, pest, perr);
   return hipGetLastError();
 }
 

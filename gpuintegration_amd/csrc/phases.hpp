@@ -13,6 +13,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "pagani.h"
+
 namespace pagani {
 namespace phases {
 
@@ -22,8 +24,26 @@ constexpr std::size_t kChunk = static_cast<std::size_t>(kThreads) * kPerThread;
 // finalize stage A folds this many chunk records per workgroup
 constexpr std::size_t kFold = 256;
 
-inline std::size_t num_chunks(std::size_t n) { return (n + kChunk - 1) / kChunk; }
-inline std::size_t num_folds(std::size_t nchunks) { return (nchunks + kFold - 1) / kFold; }
+__host__ __device__ inline std::size_t num_chunks(std::size_t n)
+{
+  return (n + kChunk - 1) / kChunk;
+}
+__host__ __device__ inline std::size_t num_folds(std::size_t nchunks)
+{
+  return (nchunks + kFold - 1) / kFold;
+}
+
+// The device-resident loop handles lists of at most one fold of chunks, so
+// its finalize is a single fold plus the final workgroup.
+constexpr std::size_t kDevLoopMax = kChunk * kFold;
+
+// Filter + split descriptor of the device-resident loop (device memory):
+// the parents' count n and pair, the children's count m, which error array
+// holds the refined errors.
+struct DevCompact {
+  std::uint64_t n, m;
+  std::uint32_t cur, run, has_parents, pad;
+};
 
 // Candidate thresholds measured by one pass of the classifier probe: the
 // search's next 20 raises (heuristic_classifier.cuh:335 caps a run at 20)
@@ -49,6 +69,22 @@ constexpr int kPartDoubles = 10;
 
 hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, double* L,
                          double* H);
+
+// Device-resident loop variants (lists of at most kDevLoopMax regions):
+// fixed launch shapes for n_cap regions, the actual count and selections
+// read from the descriptors; a descriptor with run == 0 makes them no-ops.
+hipError_t refine_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
+                      const double* est, double* err0, double* err1, const double* pest,
+                      double epsrel, int relerr, std::uint8_t* active, double* part,
+                      std::uint32_t* cnt);
+hipError_t finalize_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
+                        const double* part, const std::uint32_t* cnt, std::uint64_t* off,
+                        double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out);
+hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, int d,
+                       double* L0, double* H0, double* L1, double* H1,
+                       const std::uint8_t* active, const double* est, const double* err0,
+                       const double* err1, const std::uint8_t* dim, const std::uint64_t* off,
+                       const std::uint64_t* foff, double* pest, double* perr);
 
 // RefineError + relerr classification (parent_est == nullptr: iteration 0,
 // err_out may equal err_in).  Partial sums: {est, err_out, act*est, act*err_out}.

@@ -33,6 +33,41 @@ struct TraceRec {
   double estimate, errorest;
 };
 
+// State of the device-resident loop (device memory, mirrored in pinned host
+// memory between rounds).  k_decide (devloop.hip) runs the host loop's
+// per-iteration decisions on it; see integrate_impl for when the loop runs
+// on the device.
+constexpr int kDevTrace = 64;  // trace records per round (a round is at most 64 iterations)
+enum : std::int32_t {
+  kDevRun = 0,        // every queued iteration committed; more may follow
+  kDevDone = 1,       // converged: the last list was evaluated
+  kDevDoneEmpty = 2,  // no region stayed active: the list is empty
+  kDevHost = 3,       // the host finishes iteration `iter` from its scalars
+  kDevLimit = 4,      // the iteration cap was reached after a split
+};
+
+struct DevLoop {
+  pagani_dev_iter it;          // the next iteration
+  pagani::phases::DevCompact cp;  // this iteration's filter + split
+  // fixed for the run
+  std::uint64_t n_cap, mem_total;
+  double epsrel, epsabs;
+  std::int32_t d, mem_model, relerr, user_regions;
+  // the loop's running state
+  std::uint64_t iter;
+  double cum_est, cum_err;
+  std::uint64_t cum_nregions, cum_iters;
+  std::int32_t cum_status, status;
+  double hist[3];
+  std::uint64_t iters_collected;
+  // the evaluation's slice choice (choose_slices) and this round's launch
+  std::int32_t num_cus, slices_override, generic, launch_slices;
+  // per round
+  std::uint64_t evaluated, iterations, max_n;
+  std::uint32_t ntrace, pad;
+  TraceRec trace[kDevTrace];
+};
+
 // Everything the iteration loop carries from one iteration to the next,
 // besides the region list itself (L[cur], H[cur]: n regions) and the
 // parents' estimates (pest: n/2 when has_parents).  A run suspended at an
@@ -88,7 +123,11 @@ struct pagani_ws {
   double* d_mpart = nullptr;
   double* d_mout = nullptr;
   double* h_mout = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};
+  // device-resident loop
+  bool device_loop = true;
+  pagani::internal::DevLoop* d_loop = nullptr;
+  pagani::internal::DevLoop* h_loop = nullptr;
   bool compact_pending = false;
   std::vector<pagani::internal::TraceRec> trace;
   std::size_t live_n = 0;  // regions in L[cur], H[cur] after the last call
@@ -234,7 +273,7 @@ inline int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
   return PAGANI_OK;
 }
 
-enum { EV_EVAL0, EV_EVAL1, EV_REF0, EV_REF1, EV_CMP0, EV_CMP1, EV_CLS0, EV_CLS1 };
+enum { EV_EVAL0, EV_EVAL1, EV_REF0, EV_REF1, EV_CMP0, EV_CMP1, EV_CLS0, EV_CLS1, EV_DEV0, EV_DEV1 };
 
 inline void record(pagani_ws* ws, int i)
 {
@@ -251,7 +290,21 @@ inline double elapsed(pagani_ws* ws, int a, int b)
   return ms;
 }
 
-inline std::size_t mem_required(std::size_t n, int d, int model)
+// Waves per batch of 64 regions for an n-region evaluation: more waves
+// share a batch while there are too few batches to give every CU four
+// waves per SIMD and each wave would still get at least 16 generic points.
+__host__ __device__ inline int choose_slices_n(std::size_t n, int num_cus, int override_,
+                                               int generic)
+{
+  if (override_ > 0) return override_ < PAGANI_MAX_SLICES ? override_ : PAGANI_MAX_SLICES;
+  const std::size_t batches = (n + 63) / 64;
+  const std::size_t target = static_cast<std::size_t>(num_cus) * 4 * 4;
+  int s = 1;
+  while (s < PAGANI_MAX_SLICES && batches * s < target && generic / (2 * s) >= 16) s *= 2;
+  return s;
+}
+
+__host__ __device__ inline std::size_t mem_required(std::size_t n, int d, int model)
 {
   const std::size_t nd = n * static_cast<std::size_t>(d);
   if (model == PAGANI_MEM_ACTUAL) {
@@ -265,13 +318,14 @@ inline std::size_t mem_required(std::size_t n, int d, int model)
   return 8 * dbl + 4 * ints;
 }
 
-inline std::size_t free_mem(std::size_t n, int d, std::size_t total, int model)
+__host__ __device__ inline std::size_t free_mem(std::size_t n, int d, std::size_t total,
+                                                int model)
 {
   const std::size_t occ = mem_required(n, d, model);
   return total > occ ? total - occ : 1;
 }
 
-inline double mem_ratio(std::size_t n, int d, std::size_t total, int model)
+__host__ __device__ inline double mem_ratio(std::size_t n, int d, std::size_t total, int model)
 {
   return static_cast<double>(mem_required(n, d, model)) /
          static_cast<double>(free_mem(n, d, total, model));
@@ -336,6 +390,9 @@ struct Classifier {
 
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
              double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr);
+
+// k_decide launcher (devloop.hip): one iteration's decisions on the device.
+hipError_t launch_decide(hipStream_t s, DevLoop* loop, const pagani::phases::Scalars* sc);
 
 }  // namespace internal
 }  // namespace pagani

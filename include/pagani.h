@@ -100,7 +100,23 @@ typedef struct pagani_eval_consts {
   int slice_begin[PAGANI_MAX_SLICES + 1]; /* generic-point slice bounds     */
 } pagani_eval_consts;
 
-/* One evaluation request: regions [0, n) of the SoA pool. */
+/* Iteration descriptor of the device-resident loop, in device memory: the
+ * regions to evaluate, which SoA pair holds them, whether the iteration
+ * runs at all (a loop that has stopped turns the queued iterations into
+ * no-ops). */
+typedef struct pagani_dev_iter {
+  uint64_t n;
+  uint32_t cur;         /* 0: (L, H), 1: (L2, H2) */
+  uint32_t run;         /* 0: skip */
+  uint32_t has_parents; /* the refinement has parents' estimates */
+  uint32_t slices;      /* waves per batch of 64 regions (see consts.slices) */
+} pagani_dev_iter;
+
+/* One evaluation request: regions [0, n) of the SoA pool.  When dev is not
+ * NULL the request belongs to the device-resident loop: n is an upper bound,
+ * the count and the pair (L, H) or (L2, H2) are read from *dev on the
+ * device, and the launch must not depend on them (pagani::eval_regions
+ * handles both forms). */
 typedef struct pagani_eval_args {
   size_t n;
   const double* L;
@@ -110,6 +126,9 @@ typedef struct pagani_eval_args {
   uint8_t* dim;
   void* stream; /* hipStream_t */
   pagani_eval_consts consts;
+  const pagani_dev_iter* dev;
+  const double* L2;
+  const double* H2;
 } pagani_eval_args;
 
 /* An evaluation callback launches the region-evaluation kernel for one
@@ -141,6 +160,10 @@ typedef struct pagani_stats {
   uint64_t integrals;
   double alloc_ms;      /* host time spent growing device buffers */
   uint64_t allocations; /* device buffers (re)allocated */
+  double device_ms;     /* time of the device-resident loop's rounds (timing on) */
+  uint64_t device_iterations; /* iterations run by the device-resident loop */
+  uint64_t device_regions;    /* regions it evaluated (included in regions_evaluated) */
+  uint64_t device_rounds;     /* its host round trips */
 } pagani_stats;
 
 /* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
@@ -193,6 +216,13 @@ size_t pagani_ws_device_mem(const pagani_ws* ws);
 #define PAGANI_MEM_REFERENCE 0
 #define PAGANI_MEM_ACTUAL 1
 int pagani_ws_set_mem_model(pagani_ws* ws, int model);
+/* Device-resident loop (on by default): while the region list is at most
+ * 2^20 regions and the memory model is below a tenth of its budget,
+ * iterations are decided on the device and enqueued in rounds of growing
+ * length, with one host round trip per round instead of one per
+ * iteration.  Results are identical either way; the host takes over for
+ * the heuristic classifier, the error-budget fix and larger lists. */
+int pagani_ws_set_device_loop(pagani_ws* ws, int enable);
 /* waves per batch of 64 regions in the evaluation kernel; 0 = automatic */
 int pagani_ws_set_slices(pagani_ws* ws, int slices);
 int pagani_ws_set_timing(pagani_ws* ws, int enable);

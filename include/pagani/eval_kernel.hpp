@@ -33,6 +33,33 @@
 #include "pagani.h"
 
 namespace pagani {
+
+// The generic points [set_begin[3], feval) split over `slices` waves of a
+// batch: wave 0 also evaluates the centre and the axis points of sets 0-2,
+// so it takes a smaller share; the rest is spread evenly.  Host and device
+// use this one definition, so a slice count always means the same split.
+__host__ __device__ inline void slice_range(int slices, int wave, int p0, int feval, int& pa,
+                                            int& pb)
+{
+  if (slices <= 1) {
+    pa = p0;
+    pb = feval;
+    return;
+  }
+  const int generic = feval - p0;
+  const int share0 = feval / slices - p0 > 0 ? feval / slices - p0 : 0;
+  const int rest = generic - share0;
+  const int q = rest / (slices - 1), r = rest % (slices - 1);
+  if (wave == 0) {
+    pa = p0;
+    pb = p0 + share0;
+    return;
+  }
+  const int w = wave - 1;
+  pa = p0 + share0 + w * q + (w < r ? w : r);
+  pb = p0 + share0 + (w + 1) * q + (w + 1 < r ? w + 1 : r);
+}
+
 namespace detail {
 
 // Integrands that depend on x only through x - u may say so with
@@ -80,16 +107,16 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
 }
 
 // One wave's share of the evaluation of up to 64 regions, one lane per
-// region t of a SoA pool with row stride `stride`.  Shared by the
-// single-integral and the batched kernels.  `wave` indexes the C.slices
-// waves that split the batch's points (0 when slices == 1); `lows`,
-// `range`, `jac` describe the integration box.
+// region t of a SoA pool with row stride `stride`.  `wave` indexes the
+// `slices` waves that split the batch's points (0 when slices == 1); with
+// more than one, every wave of the workgroup passes two barriers per call.
+// `lows`, `range`, `jac` describe the integration box.
 template <int D, class F, class Real>
 __device__ __forceinline__ void eval_regions_wave(
     F& f, const double* __restrict__ L, const double* __restrict__ H, std::size_t stride,
     std::size_t t, bool live, const double* lows, const double* range, double jac,
-    const pagani_eval_consts& C, int wave, int lane, double* part, double* __restrict__ est,
-    double* __restrict__ err, std::uint8_t* __restrict__ dim)
+    const pagani_eval_consts& C, int slices, int wave, int lane, double* part,
+    double* __restrict__ est, double* __restrict__ err, std::uint8_t* __restrict__ dim)
 {
   // Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
   // width in user coordinates.
@@ -151,9 +178,9 @@ __device__ __forceinline__ void eval_regions_wave(
   }
 
   // Sets 3..8 from the generator table, this wave's slice of the points.
-  const int pa = C.slice_begin[wave];
-  const int pb = C.slice_begin[wave + 1];
   const int p0 = C.set_begin[3];
+  int pa, pb;
+  slice_range(slices, wave, p0, C.feval, pa, pb);
 #pragma unroll
   for (int s = 3; s < PAGANI_NSETS; ++s) {
     const int a = pa > C.set_begin[s] ? pa : C.set_begin[s];
@@ -167,19 +194,23 @@ __device__ __forceinline__ void eval_regions_wave(
     }
   }
 
-  if (C.slices > 1) {
+  if (slices > 1) {
     if (wave > 0) {
 #pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r)
         part[((wave - 1) * PAGANI_NRULES + r) * 64 + lane] = rs[r];
     }
     __syncthreads();
-    if (wave > 0) return;
-    for (int w = 1; w < C.slices; ++w) {
+    if (wave == 0) {
+      for (int w = 1; w < slices; ++w) {
 #pragma unroll
-      for (int r = 0; r < PAGANI_NRULES; ++r)
-        rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
+        for (int r = 0; r < PAGANI_NRULES; ++r)
+          rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
+      }
     }
+    // the partials may be overwritten by the workgroup's next batch
+    __syncthreads();
+    if (wave > 0) return;
   }
 
   // Null-rule error estimate, Sample.cuh:261-289.
@@ -207,74 +238,53 @@ __device__ __forceinline__ void eval_regions_wave(
 
 }  // namespace detail
 
-// Kernel: regions [blockIdx.x*64, +64) of an n-region SoA pool.
-// blockDim.x == 64 * C.slices.  Dynamic LDS: (slices-1)*5*64 doubles.
+// Kernel: workgroups of 64 * S threads step over the batches of 64
+// regions of an n-region SoA pool (a launch with ceil(n/64) workgroups does
+// one batch each).  For a host-driven launch `it` is NULL and n, the pool
+// (L0, H0) and S = C.slices come from the arguments.  In the device-
+// resident loop they come from *it (n, pair, slices; nothing if !it->run)
+// and the launch is fixed: its workgroups may hold more waves than the
+// iteration's slice count, and the extra waves only keep the barriers
+// company.  Both forms run this one kernel, so a given n and slice count
+// give the same bits either way.  Dynamic LDS: (waves - 1)*5*64 doubles.
 template <int D, class F, class Real>
 __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
-    eval_kernel(F f, std::size_t n, const double* __restrict__ L,
-                const double* __restrict__ H, double* __restrict__ est,
-                double* __restrict__ err, std::uint8_t* __restrict__ dim,
-                const pagani_eval_consts C)
+    eval_kernel(F f, const pagani_dev_iter* __restrict__ it, std::size_t n_arg,
+                const double* __restrict__ L0, const double* __restrict__ H0,
+                const double* __restrict__ L1, const double* __restrict__ H1,
+                double* __restrict__ est, double* __restrict__ err,
+                std::uint8_t* __restrict__ dim, const pagani_eval_consts C)
 {
   extern __shared__ double part[];
+  std::size_t n = n_arg;
+  const double* L = L0;
+  const double* H = H0;
+  int slices = C.slices;
+  if (it) {
+    if (!it->run) return;
+    n = it->n;
+    if (it->cur) {
+      L = L1;
+      H = H1;
+    }
+    slices = static_cast<int>(it->slices);
+  }
   const int lane = threadIdx.x & 63;
   // wave index is uniform across the wave: keep it (and everything derived
   // from it, the slice bounds and the point loop) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
-  const bool live = t0 < n;
-  const std::size_t t = live ? t0 : n - 1;
-  detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C, wave,
-                                        lane, part, est, err, dim);
-}
-
-// Batched evaluation.  The pool holds one 64-aligned segment per integral
-// (row stride `stride`); a launch covers the integrals of one group that
-// share ndim and integrand type.  Span k of the group starts at workgroup
-// spans[k].wave_begin; each 64-thread workgroup finds its span by a
-// (uniform) binary search, reads its integral's functor and box from the
-// tables, and evaluates up to 64 of the integral's regions.
-struct BatchSpan {
-  std::uint64_t wave_begin;  // first workgroup of this integral
-  std::uint64_t seg_begin;   // first region of its segment
-  std::uint64_t n;           // regions in the segment
-  std::uint32_t item;        // row of the functor / volume tables
-  std::uint32_t pad;
-};
-
-struct BatchVolume {
-  double lows[PAGANI_MAX_DIM];
-  double range[PAGANI_MAX_DIM];
-  double jac;
-  double pad;
-};
-
-template <int D, class F, class Real>
-__global__ void __launch_bounds__(64)
-    eval_batch_kernel(const BatchSpan* __restrict__ spans, int nspans,
-                      const unsigned char* __restrict__ fslots, std::size_t slot_bytes,
-                      const BatchVolume* __restrict__ vols, const double* __restrict__ L,
-                      const double* __restrict__ H, std::size_t stride,
-                      double* __restrict__ est, double* __restrict__ err,
-                      std::uint8_t* __restrict__ dim, const pagani_eval_consts C)
-{
-  int lo = 0, hi = nspans - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (spans[mid].wave_begin <= blockIdx.x)
-      lo = mid;
-    else
-      hi = mid - 1;
+  for (std::size_t b = blockIdx.x; b * 64 < n; b += gridDim.x) {
+    if (wave < slices) {
+      const std::size_t t0 = b * 64 + lane;
+      const bool live = t0 < n;
+      const std::size_t t = live ? t0 : n - 1;
+      detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C,
+                                            slices, wave, lane, part, est, err, dim);
+    } else if (slices > 1) {
+      __syncthreads();
+      __syncthreads();
+    }
   }
-  const BatchSpan sp = spans[lo];
-  const int lane = threadIdx.x & 63;
-  const std::uint64_t local = (blockIdx.x - sp.wave_begin) * 64 + lane;
-  const bool live = local < sp.n;
-  const std::size_t t = sp.seg_begin + (live ? local : sp.n - 1);
-  F f = *reinterpret_cast<const F*>(fslots + static_cast<std::size_t>(sp.item) * slot_bytes);
-  const BatchVolume& V = vols[sp.item];
-  detail::eval_regions_wave<D, F, Real>(f, L, H, stride, t, live, V.lows, V.range, V.jac, C, 0,
-                                        lane, nullptr, est, err, dim);
 }
 
 // Launch the evaluation of args->n regions for integrand f.  Returns 0 or
@@ -285,15 +295,23 @@ int eval_regions(const F& f, const pagani_eval_args* args)
   static_assert(D >= PAGANI_MIN_DIM && D <= PAGANI_MAX_DIM, "unsupported ndim");
   if (args->consts.ndim != D) return PAGANI_EBADARG;
   if (args->n == 0) return PAGANI_OK;
-  const int slices = args->consts.slices;
-  if (slices < 1 || slices > PAGANI_MAX_SLICES) return PAGANI_EBADARG;
-  const std::size_t blocks = (args->n + 63) / 64;
+  // waves per workgroup: the slice count (an upper bound of the device
+  // loop's counts for its launches)
+  const int waves = args->consts.slices;
+  if (waves < 1 || waves > PAGANI_MAX_SLICES) return PAGANI_EBADARG;
+  const std::size_t batches = (args->n + 63) / 64;
+  std::size_t blocks = batches;
+  if (args->dev) {
+    // enough workgroups to fill the device, each stepping over batches
+    const std::size_t cap = 4096 / static_cast<std::size_t>(waves);
+    blocks = batches < cap ? batches : cap;
+  }
   const std::size_t lds =
-      slices > 1 ? static_cast<std::size_t>(slices - 1) * PAGANI_NRULES * 64 * sizeof(double)
-                 : 0;
+      waves > 1 ? static_cast<std::size_t>(waves - 1) * PAGANI_NRULES * 64 * sizeof(double) : 0;
   hipLaunchKernelGGL((eval_kernel<D, F, Real>), dim3(static_cast<unsigned>(blocks)),
-                     dim3(64 * slices), lds, static_cast<hipStream_t>(args->stream), f,
-                     args->n, args->L, args->H, args->est, args->err, args->dim,
+                     dim3(64 * waves), lds, static_cast<hipStream_t>(args->stream), f, args->dev,
+                     args->n, args->L, args->H, args->dev ? args->L2 : args->L,
+                     args->dev ? args->H2 : args->H, args->est, args->err, args->dim,
                      args->consts);
   return hipGetLastError() == hipSuccess ? PAGANI_OK : PAGANI_EHIP;
 }

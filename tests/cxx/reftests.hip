@@ -195,7 +195,7 @@ int sample(int k, double* est_host, double* err_host, std::size_t cap)
   if (hipMalloc(&est, n * sizeof(double)) != hipSuccess ||
       hipMalloc(&err, n * sizeof(double)) != hipSuccess || hipMalloc(&dim, n) != hipSuccess)
     return -3;
-  pagani_eval_args args;
+  pagani_eval_args args{};  // dev = NULL: an ordinary launch
   args.n = n;
   args.L = regions.dLeftCoord;
   args.H = regions.dLength;

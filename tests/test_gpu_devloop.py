@@ -1,0 +1,80 @@
+"""The device-resident loop (iterations decided on the device in rounds)
+against the host-driven loop: identical results, traces and region counts,
+with fewer host round trips.
+"""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # family, d, epsrel: small lists that stay on the device throughout, and
+    # larger ones that hand over to the host (classifier, budget fix)
+    ("gaussian", 3, 1e-6),
+    ("gaussian", 5, 1e-6),
+    ("gaussian", 8, 1e-4),
+    ("oscillatory", 5, 1e-3),
+    ("oscillatory", 6, 1e-7),
+    ("product_peak", 4, 1e-6),
+    ("corner_peak", 6, 1e-6),
+    ("c0", 5, 1e-5),
+    ("discontinuous", 4, 1e-4),
+    ("discontinuous", 6, 1e-5),
+]
+
+
+def _params(fam, d, seed):
+    from gpuintegration_amd.workloads import MT19937_64
+    g = MT19937_64(seed)
+    if fam == "oscillatory":
+        a = [1.0 + 0.5 * g.uniform() for _ in range(d)]
+    elif fam == "gaussian":
+        a = [10.0 * (1.0 + 0.05 * (2.0 * g.uniform() - 1.0)) for _ in range(d)]
+    else:
+        a = [5.0 + 10.0 * g.uniform() for _ in range(d)]
+    u = [0.3 + 0.4 * g.uniform() for _ in range(d)]
+    return a, u
+
+
+def _run(pg, fam, a, u, eps, device, mem=None, relerr=True):
+    with pg.Workspace(len(a), 0) as ws:
+        ws.set_device_loop(device)
+        if mem:
+            ws.set_device_mem(mem)
+        r = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20, relerr=relerr)
+        return r, ws.trace(), ws.stats()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-{c[1]}d-{c[2]:g}" for c in CASES])
+def test_device_loop_matches_host_loop(pg, gpu, case):
+    fam, d, eps = case
+    a, u = _params(fam, d, seed=d * 7 + len(fam))
+    for mem in (None, int(16e9)):
+        rh, th, sh = _run(pg, fam, a, u, eps, False, mem)
+        rd, td, sd = _run(pg, fam, a, u, eps, True, mem)
+        assert (rd.estimate, rd.errorest, rd.nregions, rd.iters, rd.status) == \
+               (rh.estimate, rh.errorest, rh.nregions, rh.iters, rh.status)
+        for k in ("n", "m", "estimate", "errorest"):
+            assert list(td[k]) == list(th[k]), k
+        assert sd["regions_evaluated"] == sh["regions_evaluated"]
+        assert sd["iterations"] == sh["iterations"]
+        assert sh["device_iterations"] == 0
+        assert sd["device_iterations"] > 0
+        assert sd["device_rounds"] <= sd["device_iterations"]
+
+
+def test_device_loop_without_relerr_classification(pg, gpu):
+    a, u = _params("gaussian", 4, seed=3)
+    rh, th, _ = _run(pg, "gaussian", a, u, 1e-6, False, relerr=False)
+    rd, td, sd = _run(pg, "gaussian", a, u, 1e-6, True, relerr=False)
+    assert (rd.estimate, rd.errorest, rd.nregions, rd.iters) == \
+           (rh.estimate, rh.errorest, rh.nregions, rh.iters)
+    assert list(td["n"]) == list(th["n"])
+    assert sd["device_iterations"] > 0
+
+
+def test_device_loop_round_trips(pg, gpu):
+    # a run that stays small: far fewer host round trips than iterations
+    a, u = _params("gaussian", 5, seed=11)
+    r, tr, st = _run(pg, "gaussian", a, u, 1e-6, True)
+    assert st["device_iterations"] >= 4
+    assert st["device_rounds"] < st["device_iterations"]
