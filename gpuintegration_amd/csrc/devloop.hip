@@ -105,9 +105,12 @@ __global__ void __launch_bounds__(64) k_decide(DevLoop* __restrict__ s,
     s->it.has_parents = 1;
     s->it.slices = static_cast<std::uint32_t>(
         choose_slices_n(2 * m, s->num_cus, s->slices_override, s->generic));
-    // the round's launches have launch_slices waves per workgroup: an
-    // iteration that needs more waits for the next round
-    if (static_cast<std::int32_t>(s->it.slices) > s->launch_slices) s->it.run = 0;
+    // the round's launches have launch_slices waves per workgroup and
+    // grids for round_cap regions: an iteration that needs another slice
+    // count (idle waves would take the place of working ones) or a larger
+    // grid waits for the next round
+    if (static_cast<std::int32_t>(s->it.slices) != s->launch_slices || 2 * m > s->round_cap)
+      s->it.run = 0;
   }
   if (!s->user_regions) s->cum_iters++;
   s->iter++;

@@ -286,6 +286,10 @@ int grow_keep(pagani_ws* ws, DevBuf<T>& b, std::size_t need, std::size_t keep, c
   return PAGANI_OK;
 }
 
+// Iterations queued per round at most: a round that stops early leaves the
+// rest as no-op launches, so the length trades round trips for those.
+constexpr int kDevQueueMax = 16;
+
 // What a device-resident segment ended with (see run_device_segment).
 enum DevOutcome { kSegHost, kSegDone, kSegEmpty, kSegLimit };
 
@@ -298,7 +302,7 @@ struct LoopVars {
   Classifier* clf;
 };
 
-// Run iterations on the device, in rounds of 2, 4, ..., 64 queued
+// Run iterations on the device, in rounds of 2, 4, 8, 16 queued
 // iterations with one host round trip each, until the device loop stops.
 // Each iteration: evaluation, refinement, the reductions, k_decide, filter
 // + split, all with fixed launch shapes for kDevLoopMax regions, reading
@@ -356,7 +360,6 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   h.iters_collected = v.clf->iters_collected;
 
   pagani_eval_args args{};
-  args.n = cap;
   args.L = ws->L[0].p;
   args.H = ws->H[0].p;
   args.L2 = ws->L[1].p;
@@ -368,26 +371,43 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   args.dev = &ws->d_loop->it;
   make_consts(ws, lows, highs, cap, args.consts);
 
-  for (int queued = 2;; queued = std::min(queued * 2, kDevTrace)) {
+  for (int queued = 2;; queued = std::min(queued * 2, kDevQueueMax)) {
     // A round's launches have as many waves per workgroup as its first
-    // iteration's slice count; k_decide pauses the round (it.run = 0 with
-    // status kDevRun) before an iteration that would need more.
+    // iteration's slice count, and grids for the largest list with that
+    // slice count (workgroups past the actual list return at once; queued
+    // iterations after the loop stops return at once too, so the grids are
+    // kept no larger than they can be used).  k_decide pauses the round
+    // (it.run = 0, status kDevRun) before an iteration with another slice
+    // count.
     h.it.run = 1;
     h.launch_slices = static_cast<std::int32_t>(h.it.slices);
+    {
+      std::uint64_t lo = h.it.n, hi = cap;
+      while (lo < hi) {  // largest n <= cap with the same slice count
+        const std::uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (choose_slices(ws, mid) == h.launch_slices)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      h.round_cap = lo;
+    }
     set_slices(args.consts, h.launch_slices);
+    args.n = h.round_cap;
+    const std::size_t rcap = h.round_cap;
     HIP_TRY(ws, hipMemcpyAsync(ws->d_loop, ws->h_loop, sizeof(DevLoop), hipMemcpyHostToDevice,
                                ws->stream));
     if (ws->timing) record(ws, EV_DEV0);
     for (int q = 0; q < queued; ++q) {
       rc = eval(ctx, &args);
       if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
-      HIP_TRY(ws, ph::refine_dev(ws->stream, cap, &ws->d_loop->it, ws->est.p, ws->err0.p,
+      HIP_TRY(ws, ph::refine_dev(ws->stream, rcap, &ws->d_loop->it, ws->est.p, ws->err0.p,
                                  ws->err1.p, ws->pest.p, epsrel, relerr, ws->active.p,
                                  ws->part.p, ws->cnt.p));
-      HIP_TRY(ws, ph::finalize_dev(ws->stream, cap, &ws->d_loop->it, ws->part.p, ws->cnt.p,
+      HIP_TRY(ws, ph::finalize_dev(ws->stream, rcap, &ws->d_loop->it, ws->part.p, ws->cnt.p,
                                    ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p, ws->d_scal));
       HIP_TRY(ws, launch_decide(ws->stream, ws->d_loop, ws->d_scal));
-      HIP_TRY(ws, ph::compact_dev(ws->stream, cap, &ws->d_loop->cp, d, ws->L[0].p, ws->H[0].p,
+      HIP_TRY(ws, ph::compact_dev(ws->stream, rcap, &ws->d_loop->cp, d, ws->L[0].p, ws->H[0].p,
                                   ws->L[1].p, ws->H[1].p, ws->active.p, ws->est.p, ws->err0.p,
                                   ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,
                                   ws->perr.p));

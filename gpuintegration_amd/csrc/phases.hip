@@ -621,7 +621,9 @@ __global__ void __launch_bounds__(kFinThreads)
 }
 
 // Chunk-local exclusive position of the active element t (ballot within the
-// wave, wave totals through LDS).  Returns the block total.
+// wave, wave totals through LDS) for a workgroup of T threads.  Returns the
+// workgroup total.
+template <int T = kThreads>
 __device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
                                                     std::uint32_t* wsum)
 {
@@ -633,7 +635,7 @@ __device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
   __syncthreads();
   std::uint32_t woff = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) {
+  for (int w = 0; w < T / 64; ++w) {
     if (w < wave) woff += wsum[w];
     tot += wsum[w];
   }
@@ -641,6 +643,11 @@ __device__ __forceinline__ std::uint32_t chunk_rank(bool a, std::uint32_t& rank,
   rank = woff + pre;
   return tot;
 }
+
+// The compaction moves data only, so it can use wider workgroups than the
+// reductions (whose per-thread order fixes their rounding): a chunk of
+// kChunk regions is covered in kChunk / kCompactThreads steps.
+constexpr int kCompactThreads = 1024;
 
 // alignRegions (Sub_region_filter.cuh:8-47) and, with split, the bisection
 // of divideIntervalsGPU (Sub_region_splitter.cuh:8-46) fused: the active
@@ -653,14 +660,14 @@ __device__ __forceinline__ void compact_chunk(
     const std::uint64_t* __restrict__ foff, double* __restrict__ L2, double* __restrict__ H2,
     double* __restrict__ pest, double* __restrict__ perr, std::uint8_t* __restrict__ dim2)
 {
-  __shared__ std::uint32_t wsum[kThreads / 64];
+  __shared__ std::uint32_t wsum[kCompactThreads / 64];
   const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
   std::uint64_t run = off[blockIdx.x] + foff[blockIdx.x / kFold];
-  for (int k = 0; k < kPerThread; ++k) {
-    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+  for (int k = 0; k < static_cast<int>(kChunk / kCompactThreads); ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
     const bool a = t < n && active[t] == 1;
     std::uint32_t rank;
-    const std::uint32_t tot = chunk_rank(a, rank, wsum);
+    const std::uint32_t tot = chunk_rank<kCompactThreads>(a, rank, wsum);
     if (a) {
       const std::size_t idx = run + rank;
       if (idx < m) {
@@ -699,7 +706,7 @@ __device__ __forceinline__ void compact_chunk(
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kCompactThreads)
     k_compact(int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
               const double* __restrict__ H, const std::uint8_t* __restrict__ active,
               const double* __restrict__ est, const double* __restrict__ err,
@@ -714,7 +721,7 @@ __global__ void __launch_bounds__(kThreads)
 // The device-resident loop's filter + split: the descriptor says whether it
 // runs, the counts, which pair holds the parents (the children go to the
 // other) and which error array the refinement wrote.
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kCompactThreads)
     k_compact_dev(const DevCompact* __restrict__ cp, int d, double* L0, double* H0, double* L1,
                   double* H1, const std::uint8_t* __restrict__ active,
                   const double* __restrict__ est, const double* err0, const double* err1,
@@ -895,7 +902,7 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
                    double* pest, double* perr, std::uint8_t* dim2)
 {
   if (n == 0 || m == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
+  hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kCompactThreads), 0, s,
                      split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr,
                      dim2);
   return hipGetLastError();
@@ -933,7 +940,7 @@ hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, i
                        const std::uint64_t* foff, double* pest, double* perr)
 {
   hipLaunchKernelGGL(k_compact_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))),
-                     dim3(kThreads), 0, s, cp, d, L0, H0, L1, H1, active, est, err0, err1, dim,
+                     dim3(kCompactThreads), 0, s, cp, d, L0, H0, L1, H1, active, est, err0, err1, dim,
                      off, foff, pest, perr);
   return hipGetLastError();
 }

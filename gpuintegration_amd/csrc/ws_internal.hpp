@@ -60,8 +60,10 @@ struct DevLoop {
   std::int32_t cum_status, status;
   double hist[3];
   std::uint64_t iters_collected;
-  // the evaluation's slice choice (choose_slices) and this round's launch
+  // the evaluation's slice choice (choose_slices) and this round's launch:
+  // its waves per workgroup and the largest list its grids cover
   std::int32_t num_cus, slices_override, generic, launch_slices;
+  std::uint64_t round_cap;
   // per round
   std::uint64_t evaluated, iterations, max_n;
   std::uint32_t ntrace, pad;

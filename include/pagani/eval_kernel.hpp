@@ -109,7 +109,7 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
 // One wave's share of the evaluation of up to 64 regions, one lane per
 // region t of a SoA pool with row stride `stride`.  `wave` indexes the
 // `slices` waves that split the batch's points (0 when slices == 1); with
-// more than one, every wave of the workgroup passes two barriers per call.
+// more than one, every wave of the workgroup passes one barrier.
 // `lows`, `range`, `jac` describe the integration box.
 template <int D, class F, class Real>
 __device__ __forceinline__ void eval_regions_wave(
@@ -208,8 +208,6 @@ __device__ __forceinline__ void eval_regions_wave(
           rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
       }
     }
-    // the partials may be overwritten by the workgroup's next batch
-    __syncthreads();
     if (wave > 0) return;
   }
 
@@ -238,15 +236,15 @@ __device__ __forceinline__ void eval_regions_wave(
 
 }  // namespace detail
 
-// Kernel: workgroups of 64 * S threads step over the batches of 64
-// regions of an n-region SoA pool (a launch with ceil(n/64) workgroups does
-// one batch each).  For a host-driven launch `it` is NULL and n, the pool
-// (L0, H0) and S = C.slices come from the arguments.  In the device-
-// resident loop they come from *it (n, pair, slices; nothing if !it->run)
-// and the launch is fixed: its workgroups may hold more waves than the
-// iteration's slice count, and the extra waves only keep the barriers
-// company.  Both forms run this one kernel, so a given n and slice count
-// give the same bits either way.  Dynamic LDS: (waves - 1)*5*64 doubles.
+// Kernel: workgroup b evaluates regions [64 b, 64 b + 64) of an n-region
+// SoA pool with S = slices waves sharing the batch's points.  For a
+// host-driven launch `it` is NULL and n, the pool (L0, H0) and S = C.slices
+// come from the arguments.  In the device-resident loop they come from *it
+// (nothing at all if !it->run), and the launch is sized for the round's
+// largest list: workgroups past n return at once, and waves past the
+// iteration's slice count only keep the barrier company.  Both forms run
+// this one kernel, so a given n and slice count give the same bits either
+// way.  Dynamic LDS: (waves per workgroup - 1)*5*64 doubles.
 template <int D, class F, class Real>
 __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     eval_kernel(F f, const pagani_dev_iter* __restrict__ it, std::size_t n_arg,
@@ -269,22 +267,20 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     }
     slices = static_cast<int>(it->slices);
   }
+  if (static_cast<std::size_t>(blockIdx.x) * 64 >= n) return;
   const int lane = threadIdx.x & 63;
   // wave index is uniform across the wave: keep it (and everything derived
   // from it, the slice bounds and the point loop) in scalar registers
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
-  for (std::size_t b = blockIdx.x; b * 64 < n; b += gridDim.x) {
-    if (wave < slices) {
-      const std::size_t t0 = b * 64 + lane;
-      const bool live = t0 < n;
-      const std::size_t t = live ? t0 : n - 1;
-      detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C,
-                                            slices, wave, lane, part, est, err, dim);
-    } else if (slices > 1) {
-      __syncthreads();
-      __syncthreads();
-    }
+  if (wave >= slices) {
+    if (slices > 1) __syncthreads();
+    return;
   }
+  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
+  const bool live = t0 < n;
+  const std::size_t t = live ? t0 : n - 1;
+  detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C, slices,
+                                        wave, lane, part, est, err, dim);
 }
 
 // Launch the evaluation of args->n regions for integrand f.  Returns 0 or
@@ -299,13 +295,7 @@ int eval_regions(const F& f, const pagani_eval_args* args)
   // loop's counts for its launches)
   const int waves = args->consts.slices;
   if (waves < 1 || waves > PAGANI_MAX_SLICES) return PAGANI_EBADARG;
-  const std::size_t batches = (args->n + 63) / 64;
-  std::size_t blocks = batches;
-  if (args->dev) {
-    // enough workgroups to fill the device, each stepping over batches
-    const std::size_t cap = 4096 / static_cast<std::size_t>(waves);
-    blocks = batches < cap ? batches : cap;
-  }
+  const std::size_t blocks = (args->n + 63) / 64;
   const std::size_t lds =
       waves > 1 ? static_cast<std::size_t>(waves - 1) * PAGANI_NRULES * 64 * sizeof(double) : 0;
   hipLaunchKernelGGL((eval_kernel<D, F, Real>), dim3(static_cast<unsigned>(blocks)),
