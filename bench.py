@@ -32,8 +32,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per region of the 8D Gaussian evaluation kernel, from two
 # rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over `bench.py
 # --eval-only`, each scaled by what the counter reads for a 1 GiB device
-# copy: 129.09 B read + 17.00 B written (algorithmic: 128 + 17).
-EVAL_HBM_BYTES_PER_REGION_8D = 146.09
+# copy: 129.13 B read + 17.00 B written (algorithmic: 128 + 17).
+EVAL_HBM_BYTES_PER_REGION_8D = 146.13
 
 
 def _sync():
