@@ -18,6 +18,9 @@
 
 namespace {
 
+// whether the drivers below let the workspace run its device-resident loop
+int g_device_loop = 1;
+
 class PTest {
  public:
   __device__ __host__ double operator()(double, double) { return 15.37; }
@@ -116,6 +119,7 @@ template <int D, class F>
 int run(double epsrel, double epsabs, pagani_result* out)
 {
   Workspace<double, D> pagani;
+  pagani_ws_set_device_loop(pagani.handle(), g_device_loop);
   quad::Volume<double, D> vol;
   const numint::integration_result r = pagani.integrate(F{}, epsrel, epsabs, vol);
   out->estimate = r.estimate;
@@ -135,6 +139,7 @@ int run_split(int k, double epsrel, double epsabs, std::size_t mem_total, pagani
               std::size_t* final_n)
 {
   Workspace<double, D> pagani;
+  pagani_ws_set_device_loop(pagani.handle(), g_device_loop);
   if (mem_total) pagani_ws_set_device_mem(pagani.handle(), mem_total);
   quad::Volume<double, D> vol;
   Sub_regions<double, D> regions(static_cast<std::size_t>(k));
@@ -216,6 +221,8 @@ int sample(int k, double* est_host, double* err_host, std::size_t cap)
 }
 
 }  // namespace
+
+extern "C" void reftest_set_device_loop(int on) { g_device_loop = on ? 1 : 0; }
 
 extern "C" int reftest_integrate(const char* name, double epsrel, double epsabs,
                                  pagani_result* out)

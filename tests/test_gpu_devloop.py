@@ -78,3 +78,48 @@ def test_device_loop_round_trips(pg, gpu):
     r, tr, st = _run(pg, "gaussian", a, u, 1e-6, True)
     assert st["device_iterations"] >= 4
     assert st["device_rounds"] < st["device_iterations"]
+
+
+@pytest.mark.parametrize("case", [("gaussian", 5, 1e-4), ("product_peak", 6, 1e-5),
+                                  ("corner_peak", 4, 1e-6)])
+def test_device_loop_fp32_integrand(pg, gpu, case):
+    fam, d, eps = case
+    a, u = _params(fam, d, seed=d + 101)
+    out = []
+    for device in (False, True):
+        with pg.Workspace(d, 0) as ws:
+            ws.set_device_loop(device)
+            r = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20, precision=pg.FP32)
+            out.append((r.estimate, r.errorest, r.nregions, r.iters, r.status,
+                        ws.stats()["device_iterations"]))
+    assert out[0][:5] == out[1][:5]
+    assert out[1][5] > 0
+
+
+def test_device_loop_user_functors(pg, gpu):
+    """The reference tests' integrands through the C++ header, both loops."""
+    import ctypes
+    import os
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cxx",
+                                   "libreftests.so"))
+
+    class _Res(ctypes.Structure):
+        _fields_ = [("estimate", ctypes.c_double), ("errorest", ctypes.c_double),
+                    ("neval", ctypes.c_size_t), ("nregions", ctypes.c_size_t),
+                    ("nFinishedRegions", ctypes.c_size_t), ("status", ctypes.c_int),
+                    ("lastPhase", ctypes.c_int), ("chi_sq", ctypes.c_double),
+                    ("iters", ctypes.c_size_t)]
+    lib.reftest_integrate.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(_Res)]
+    try:
+        for name, eps in (("F_4_5D", 1e-6), ("F_2_6D", 1e-4), ("F_6_6D", 1e-4), ("XY", 1e-8),
+                          ("Fun6", 1e-4), ("BoxIntegral8_15", 1e-5)):
+            got = []
+            for on in (0, 1):
+                lib.reftest_set_device_loop(on)
+                r = _Res()
+                assert lib.reftest_integrate(name.encode(), eps, 1e-20, ctypes.byref(r)) == 0
+                got.append((r.estimate, r.errorest, r.nregions, r.iters, r.status))
+            assert got[0] == got[1], name
+    finally:
+        lib.reftest_set_device_loop(1)
