@@ -1,11 +1,13 @@
-// devloop.hip -- the per-iteration decisions of the device-resident loop.
+// devloop_decide.hpp -- the per-iteration decisions of the device-resident
+// loop.
 //
 // Workspace::integrate (Workspace.cuh:362-494) decides, after each
 // iteration's reductions, whether the estimate is accurate enough, which
 // regions finished, whether the classifier must run and what the next list
 // holds.  While the region list is small and the memory model far from its
 // budget, those decisions need nothing but the iteration's sums, so they run
-// here, in one thread, between the reductions and the filter + split of the
+// on the device, in one thread of the workgroup that finishes the
+// iteration's reductions last (k_step_dev), before the filter + split of the
 // same iteration; the host queues whole rounds of iterations and reads the
 // state back once per round.  Everything the device cannot decide alone --
 // the error-budget fix (Workspace.cuh:121-146), a memory model past a tenth
@@ -14,8 +16,9 @@
 // before any state changes, and the host finishes that iteration exactly as
 // it finishes its own.
 //
-// Compiled with -ffp-contract=off: the same IEEE operations in the same
+// Plain IEEE double without contraction: the same operations in the same
 // order as the host loop, so both make identical decisions.
+#pragma once
 
 #include <hip/hip_runtime.h>
 
@@ -24,27 +27,20 @@
 namespace pagani {
 namespace internal {
 
-namespace {
-
-__device__ void push_trace(DevLoop* s, std::uint64_t n, std::uint64_t m, double est, double err)
+__device__ __forceinline__ void push_trace(DevLoop* s, std::uint64_t n, std::uint64_t m,
+                                           double est, double err)
 {
   if (s->ntrace < kDevTrace) s->trace[s->ntrace++] = TraceRec{n, m, est, err};
 }
 
-__global__ void __launch_bounds__(64) k_decide(DevLoop* __restrict__ s,
-                                               const pagani::phases::Scalars* __restrict__ sc)
+// One iteration's decisions from its sums (estimates and errors of all
+// regions and of the active ones) and its count m of active regions.  Runs
+// in one thread while it.run is set.
+__device__ __forceinline__ void decide_step(DevLoop* s, double iter_est, double iter_err,
+                                            double act_est, double act_err, std::uint64_t m)
 {
-  if (threadIdx.x != 0) return;
-  if (!s->it.run) {
-    s->cp.run = 0;
-    return;
-  }
+#pragma clang fp contract(off)
   const std::uint64_t n = s->it.n;
-  const double iter_est = sc->sum[0];
-  const double iter_err = sc->sum[1];
-  const double act_est = sc->sum[2];
-  const double act_err = sc->sum[3];
-  const std::uint64_t m = sc->count;
   s->evaluated += n;
   s->iterations += 1;
   if (n > s->max_n) s->max_n = n;
@@ -118,14 +114,6 @@ __global__ void __launch_bounds__(64) k_decide(DevLoop* __restrict__ s,
     s->status = kDevLimit;
     s->it.run = 0;
   }
-}
-
-}  // namespace
-
-hipError_t launch_decide(hipStream_t s, DevLoop* loop, const pagani::phases::Scalars* sc)
-{
-  hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, loop, sc);
-  return hipGetLastError();
 }
 
 }  // namespace internal

@@ -304,9 +304,10 @@ struct LoopVars {
 
 // Run iterations on the device, in rounds of 2, 4, 8, 16 queued
 // iterations with one host round trip each, until the device loop stops.
-// Each iteration: evaluation, refinement, the reductions, k_decide, filter
-// + split, all with fixed launch shapes for kDevLoopMax regions, reading
-// the counts from the DevLoop descriptors.  On return the loop variables
+// Each iteration is three launches: the evaluation, k_step_dev (refinement,
+// the reductions and the decisions) and the filter + split, with fixed
+// launch shapes for the round, reading the counts from the DevLoop
+// descriptors.  On return the loop variables
 // describe where the host continues; with kSegHost the iteration `v.it`
 // has been evaluated and reduced (its sums are in d_scal) and the host
 // finishes it.
@@ -376,7 +377,7 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
     // iteration's slice count, and grids for the largest list with that
     // slice count (workgroups past the actual list return at once; queued
     // iterations after the loop stops return at once too, so the grids are
-    // kept no larger than they can be used).  k_decide pauses the round
+    // kept no larger than they can be used).  decide_step pauses the round
     // (it.run = 0, status kDevRun) before an iteration with another slice
     // count.
     h.it.run = 1;
@@ -401,12 +402,10 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
     for (int q = 0; q < queued; ++q) {
       rc = eval(ctx, &args);
       if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
-      HIP_TRY(ws, ph::refine_dev(ws->stream, rcap, &ws->d_loop->it, ws->est.p, ws->err0.p,
-                                 ws->err1.p, ws->pest.p, epsrel, relerr, ws->active.p,
-                                 ws->part.p, ws->cnt.p));
-      HIP_TRY(ws, ph::finalize_dev(ws->stream, rcap, &ws->d_loop->it, ws->part.p, ws->cnt.p,
-                                   ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p, ws->d_scal));
-      HIP_TRY(ws, launch_decide(ws->stream, ws->d_loop, ws->d_scal));
+      HIP_TRY(ws, ph::step_dev(ws->stream, rcap, ws->d_loop, ws->est.p, ws->err0.p, ws->err1.p,
+                               ws->pest.p, epsrel, relerr, ws->active.p, ws->part.p, ws->cnt.p,
+                               ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p, ws->d_scal,
+                               &ws->d_loop->ticket));
       HIP_TRY(ws, ph::compact_dev(ws->stream, rcap, &ws->d_loop->cp, d, ws->L[0].p, ws->H[0].p,
                                   ws->L[1].p, ws->H[1].p, ws->active.p, ws->est.p, ws->err0.p,
                                   ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,

@@ -7,6 +7,8 @@
 
 #include <cmath>
 
+#include "devloop_decide.hpp"
+
 namespace pagani {
 namespace phases {
 
@@ -170,23 +172,6 @@ __global__ void __launch_bounds__(kThreads)
   refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt);
 }
 
-// The same for the device-resident loop: n and whether there are parents
-// come from the iteration descriptor (no parents: the raw errors are
-// refined in place, as in the host-driven loop's first iteration).
-__global__ void __launch_bounds__(kThreads)
-    k_refine_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ est,
-                 double* err0, double* err1, const double* __restrict__ pest, double epsrel,
-                 int relerr, std::uint8_t* __restrict__ active, double* __restrict__ part,
-                 std::uint32_t* __restrict__ cnt)
-{
-  if (!it->run) return;
-  const std::size_t n = it->n;
-  if (blockIdx.x >= num_chunks(n)) return;
-  const bool parents = it->has_parents != 0;
-  refine_chunk(n, est, err0, parents ? err1 : err0, parents ? pest : nullptr, epsrel, relerr,
-               active, part, cnt);
-}
-
 __global__ void __launch_bounds__(kThreads)
     k_count_flags(std::size_t n, const std::uint8_t* __restrict__ flags,
                   const double* __restrict__ est, const double* __restrict__ err,
@@ -342,17 +327,20 @@ __global__ void __launch_bounds__(kThreads)
 // Stage A of finalize: one workgroup folds kFold chunk records (fixed
 // tree), writes the chunks' offsets relative to the fold and one fold
 // record.
-__device__ __forceinline__ void fold_block(std::size_t nchunks, const double* __restrict__ part,
+__device__ __forceinline__ void fold_block(std::size_t fold, std::size_t nchunks,
+                                           const double* __restrict__ part,
                                            const std::uint32_t* __restrict__ cnt,
                                            std::uint64_t* __restrict__ off,
                                            double* __restrict__ fpart,
-                                           std::uint32_t* __restrict__ fcnt, int with_minmax)
+                                           std::uint32_t* __restrict__ fcnt, int with_minmax,
+                                           DD* sums_out = nullptr,
+                                           std::uint32_t* total_out = nullptr)
 {
   __shared__ DD sh[kThreads / 64];
   __shared__ std::uint32_t wtot[kThreads / 64];
   __shared__ double smn[kThreads / 64], smx[kThreads / 64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const std::size_t c = static_cast<std::size_t>(blockIdx.x) * kFold + threadIdx.x;
+  const std::size_t c = fold * kFold + threadIdx.x;
   const bool live = c < nchunks;
   DD s[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   double mn = HUGE_VAL, mx = -HUGE_VAL;
@@ -385,7 +373,8 @@ __device__ __forceinline__ void fold_block(std::size_t nchunks, const double* __
       total += wtot[w];
     }
     if (live && off) off[c] = wbase + inc - v;
-    if (threadIdx.x == 0) fcnt[blockIdx.x] = total;
+    if (threadIdx.x == 0) fcnt[fold] = total;
+    if (total_out) *total_out = total;
   }
   if (with_minmax) {
 #pragma unroll
@@ -405,7 +394,7 @@ __device__ __forceinline__ void fold_block(std::size_t nchunks, const double* __
       }
   }
   if (threadIdx.x == 0) {
-    double* q = fpart + static_cast<std::size_t>(blockIdx.x) * kPartDoubles;
+    double* q = fpart + fold * kPartDoubles;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       q[2 * i] = s[i].hi;
@@ -414,6 +403,10 @@ __device__ __forceinline__ void fold_block(std::size_t nchunks, const double* __
     q[8] = mn;
     q[9] = mx;
   }
+  if (sums_out) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sums_out[i] = s[i];
+  }
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -421,18 +414,7 @@ __global__ void __launch_bounds__(kThreads)
            const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
            double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt, int with_minmax)
 {
-  fold_block(nchunks, part, cnt, off, fpart, fcnt, with_minmax);
-}
-
-__global__ void __launch_bounds__(kThreads)
-    k_fold_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ part,
-               const std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
-               double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt)
-{
-  if (!it->run) return;
-  const std::size_t nchunks = num_chunks(it->n);
-  if (blockIdx.x >= num_folds(nchunks)) return;
-  fold_block(nchunks, part, cnt, off, fpart, fcnt, 0);
+  fold_block(blockIdx.x, nchunks, part, cnt, off, fpart, fcnt, with_minmax);
 }
 
 // device_set_true_for_larger_than + the reductions of apply_threshold and
@@ -611,13 +593,64 @@ __global__ void __launch_bounds__(kFinThreads)
   finalize_block(nchunks, part, cnt, off, out, with_minmax);
 }
 
-__global__ void __launch_bounds__(kFinThreads)
-    k_finalize_dev(const pagani_dev_iter* __restrict__ it, const double* __restrict__ fpart,
-                   const std::uint32_t* __restrict__ fcnt, std::uint64_t* __restrict__ foff,
-                   Scalars* __restrict__ out)
+// One step of the device-resident loop after the evaluation: the
+// refinement of every chunk (k_refine's work), then -- in whichever
+// workgroup finishes last -- the fold of the chunk records (k_fold's work
+// for the single fold a list of at most kDevLoopMax regions has), the final
+// sums exactly as k_finalize forms them from one fold record, and the
+// iteration's decisions (decide_step).  All workgroups of the grid take a
+// ticket, those past the list too, so the descriptor the decision rewrites
+// has been read by every workgroup before it changes.
+__global__ void __launch_bounds__(kThreads)
+    k_step_dev(internal::DevLoop* __restrict__ loop, const double* __restrict__ est,
+               double* err0, double* err1, const double* __restrict__ pest, double epsrel,
+               int relerr, std::uint8_t* __restrict__ active, double* __restrict__ part,
+               std::uint32_t* __restrict__ cnt, std::uint64_t* __restrict__ off,
+               double* __restrict__ fpart, std::uint32_t* __restrict__ fcnt,
+               std::uint64_t* __restrict__ foff, Scalars* __restrict__ sc,
+               std::uint32_t* __restrict__ ticket)
 {
-  if (!it->run) return;
-  finalize_block(num_folds(num_chunks(it->n)), fpart, fcnt, foff, out, 0);
+  __shared__ int last;
+  const pagani_dev_iter it = loop->it;
+  if (!it.run) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) loop->cp.run = 0;
+    return;
+  }
+  const std::size_t n = it.n;
+  const std::size_t nch = num_chunks(n);
+  if (blockIdx.x < nch) {
+    const bool parents = it.has_parents != 0;
+    refine_chunk(n, est, err0, parents ? err1 : err0, parents ? pest : nullptr, epsrel, relerr,
+                 active, part, cnt);
+  }
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  DD fs[4];
+  std::uint32_t total = 0;
+  fold_block(0, nch, part, cnt, off, fpart, fcnt, 0, fs, &total);
+  if (threadIdx.x == 0) {
+    double sums[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      // k_finalize over one fold record: its thread 0 adds the record to
+      // zero, then meets only zeros on the way up its tree
+      DD x = dd_add(DD{0.0, 0.0}, DD{fs[i].hi, fs[i].lo});
+      for (int stride = kFinThreads / 2; stride > 0; stride >>= 1) x = dd_add(x, DD{0.0, 0.0});
+      sums[i] = x.hi + x.lo;
+      sc->sum[i] = sums[i];
+    }
+    foff[0] = 0;
+    sc->count = total;
+    sc->mn = 0.0;
+    sc->mx = 0.0;
+    sc->pad = 0;
+    internal::decide_step(loop, sums[0], sums[1], sums[2], sums[3], total);
+    *ticket = 0;
+  }
 }
 
 // Chunk-local exclusive position of the active element t (ballot within the
@@ -908,28 +941,15 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
   return hipGetLastError();
 }
 
-hipError_t refine_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
-                      const double* est, double* err0, double* err1, const double* pest,
-                      double epsrel, int relerr, std::uint8_t* active, double* part,
-                      std::uint32_t* cnt)
+hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, const double* est,
+                    double* err0, double* err1, const double* pest, double epsrel, int relerr,
+                    std::uint8_t* active, double* part, std::uint32_t* cnt, std::uint64_t* off,
+                    double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* sc,
+                    std::uint32_t* ticket)
 {
-  hipLaunchKernelGGL(k_refine_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))),
-                     dim3(kThreads), 0, s, it, est, err0, err1, pest, epsrel, relerr, active,
-                     part, cnt);
-  return hipGetLastError();
-}
-
-hipError_t finalize_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
-                        const double* part, const std::uint32_t* cnt, std::uint64_t* off,
-                        double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out)
-{
-  const std::size_t nfold = num_folds(num_chunks(n_cap));
-  hipLaunchKernelGGL(k_fold_dev, dim3(static_cast<unsigned>(nfold)), dim3(kThreads), 0, s, it,
-                     part, cnt, off, fpart, fcnt);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_finalize_dev, dim3(1), dim3(kFinThreads), 0, s, it, fpart, fcnt, foff,
-                     out);
+  hipLaunchKernelGGL(k_step_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))), dim3(kThreads),
+                     0, s, loop, est, err0, err1, pest, epsrel, relerr, active, part, cnt, off,
+                     fpart, fcnt, foff, sc, ticket);
   return hipGetLastError();
 }
 

@@ -16,6 +16,9 @@
 #include "pagani.h"
 
 namespace pagani {
+namespace internal {
+struct DevLoop;
+}
 namespace phases {
 
 constexpr int kThreads = 256;
@@ -73,13 +76,11 @@ hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, dou
 // Device-resident loop variants (lists of at most kDevLoopMax regions):
 // fixed launch shapes for n_cap regions, the actual count and selections
 // read from the descriptors; a descriptor with run == 0 makes them no-ops.
-hipError_t refine_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
-                      const double* est, double* err0, double* err1, const double* pest,
-                      double epsrel, int relerr, std::uint8_t* active, double* part,
-                      std::uint32_t* cnt);
-hipError_t finalize_dev(hipStream_t s, std::size_t n_cap, const pagani_dev_iter* it,
-                        const double* part, const std::uint32_t* cnt, std::uint64_t* off,
-                        double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* out);
+hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, const double* est,
+                    double* err0, double* err1, const double* pest, double epsrel, int relerr,
+                    std::uint8_t* active, double* part, std::uint32_t* cnt, std::uint64_t* off,
+                    double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* sc,
+                    std::uint32_t* ticket);
 hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, int d,
                        double* L0, double* H0, double* L1, double* H1,
                        const std::uint8_t* active, const double* est, const double* err0,

@@ -34,9 +34,9 @@ struct TraceRec {
 };
 
 // State of the device-resident loop (device memory, mirrored in pinned host
-// memory between rounds).  k_decide (devloop.hip) runs the host loop's
-// per-iteration decisions on it; see integrate_impl for when the loop runs
-// on the device.
+// memory between rounds).  decide_step (devloop_decide.hpp, run by the last
+// workgroup of k_step_dev) makes the host loop's per-iteration decisions on
+// it; see integrate_impl for when the loop runs on the device.
 constexpr int kDevTrace = 64;  // trace records per round (a round is at most 64 iterations)
 enum : std::int32_t {
   kDevRun = 0,        // every queued iteration committed; more may follow
@@ -66,7 +66,9 @@ struct DevLoop {
   std::uint64_t round_cap;
   // per round
   std::uint64_t evaluated, iterations, max_n;
-  std::uint32_t ntrace, pad;
+  std::uint32_t ntrace;
+  // k_step_dev's workgroup counter (zero between launches)
+  std::uint32_t ticket;
   TraceRec trace[kDevTrace];
 };
 
@@ -392,9 +394,6 @@ struct Classifier {
 
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
              double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr);
-
-// k_decide launcher (devloop.hip): one iteration's decisions on the device.
-hipError_t launch_decide(hipStream_t s, DevLoop* loop, const pagani::phases::Scalars* sc);
 
 }  // namespace internal
 }  // namespace pagani
