@@ -38,7 +38,6 @@ HIP_SOURCES = {
     "interop.hip": [],
     "batch.hip": [],
     "checkpoint.hip": [],
-    "devloop.hip": ["-ffp-contract=off"],
 }
 CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"]}
 
