@@ -141,6 +141,30 @@ _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
 _sz = ctypes.c_size_t
 
+MC_BINS = 500
+MC_MAX_DIM = 16
+MC_RNG = {"default": 0, "lcg": 1}
+
+
+class McParams(ctypes.Structure):
+    _fields_ = [("ncall", ctypes.c_double), ("total_iters", ctypes.c_int),
+                ("adjust_iters", ctypes.c_int), ("skip_iters", ctypes.c_int),
+                ("generator", ctypes.c_int), ("one_axis", ctypes.c_int),
+                ("adjust_params", ctypes.c_int), ("seed", ctypes.c_uint64)]
+
+
+class McResult(ctypes.Structure):
+    _fields_ = [("estimate", ctypes.c_double), ("errorest", ctypes.c_double),
+                ("chi_sq", ctypes.c_double), ("status", ctypes.c_int), ("pad", ctypes.c_int),
+                ("iters", ctypes.c_uint64), ("runs", ctypes.c_uint64),
+                ("neval", ctypes.c_uint64), ("ncall", ctypes.c_double)]
+
+
+class McStats(ctypes.Structure):
+    _fields_ = [("sample_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("samples", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("integrals", ctypes.c_uint64)]
+
 # name -> (restype, argtypes)
 _SIGS = {
     "pagani_rule_tables": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp,
@@ -211,6 +235,24 @@ _SIGS = {
     "pagani_integrate_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_BatchItem), _sz,
                                               ctypes.POINTER(_Result),
                                               ctypes.POINTER(ctypes.c_int)]),
+    "pagani_mc_create": (_vp, [ctypes.c_int, ctypes.c_int]),
+    "pagani_mc_destroy": (None, [_vp]),
+    "pagani_mc_last_error": (ctypes.c_char_p, [_vp]),
+    "pagani_mc_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_mc_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(McStats)]),
+    "pagani_mc_reset_stats": (ctypes.c_int, [_vp]),
+    "pagani_mc_integrate_cb": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                              _dp, _dp, ctypes.POINTER(McParams),
+                                              ctypes.POINTER(McResult)]),
+    "pagani_mc_integrate_genz": (ctypes.c_int, [_vp, ctypes.POINTER(_Genz), ctypes.c_double,
+                                                ctypes.c_double, _dp, _dp,
+                                                ctypes.POINTER(McParams),
+                                                ctypes.POINTER(McResult)]),
+    "pagani_mc_trace": (ctypes.c_int, [_vp, _dp, _dp, _sz, ctypes.POINTER(_sz)]),
+    "pagani_mc_grid": (ctypes.c_int, [_vp, _dp]),
+    "pagani_mc_sample_genz": (ctypes.c_int, [_vp, ctypes.POINTER(_Genz), ctypes.c_double, _dp,
+                                             _dp, _dp, ctypes.c_uint64, ctypes.c_int,
+                                             ctypes.c_int, _dp, _dp, _dp]),
 }
 
 _LIB = None
@@ -616,6 +658,108 @@ class Batch:
         if rc < 0 and rc != ENOMEM:
             self._check(rc)
         return [IntegrationResult._from(res[i]) for i in range(n)], list(codes)[:n]
+
+
+class MCubes:
+    """Owns a pagani_mc handle: the m-Cubes VEGAS integrator for one
+    dimension (cuda/mcubes/vegasT.cuh of the reference)."""
+
+    def __init__(self, ndim, device=-1):
+        self.lib = load()
+        self.ndim = int(ndim)
+        self._h = self.lib.pagani_mc_create(self.ndim, int(device))
+        if not self._h:
+            why = self.lib.pagani_mc_last_error(None)
+            raise PaganiError(EHIP, f"pagani_mc_create failed: "
+                                    f"{why.decode() if why else 'unknown'}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.pagani_mc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = self.lib.pagani_mc_last_error(self._h)
+            raise PaganiError(rc, msg.decode() if msg else "")
+        return rc
+
+    def set_timing(self, on=True):
+        self._check(self.lib.pagani_mc_set_timing(self._h, 1 if on else 0))
+
+    def stats(self):
+        st = McStats()
+        self._check(self.lib.pagani_mc_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in McStats._fields_}
+
+    def reset_stats(self):
+        self._check(self.lib.pagani_mc_reset_stats(self._h))
+
+    @staticmethod
+    def params(ncall, total_iters=15, adjust_iters=15, skip_iters=5, generator="default",
+               one_axis=False, adjust_params=False, seed=0):
+        return McParams(float(ncall), int(total_iters), int(adjust_iters), int(skip_iters),
+                        MC_RNG.get(generator, generator), int(one_axis), int(adjust_params),
+                        int(seed))
+
+    def integrate_genz(self, family, a, u=None, epsrel=1e-3, epsabs=1e-20, ncall=1e6,
+                       total_iters=15, adjust_iters=15, skip_iters=5, generator="default",
+                       one_axis=False, adjust_params=False, seed=0, lows=None, highs=None):
+        """Returns a dict with estimate, errorest, chi_sq, status, iters, runs,
+        neval, ncall."""
+        g = genz(family, a, u)
+        p = self.params(ncall, total_iters, adjust_iters, skip_iters, generator, one_axis,
+                        adjust_params, seed)
+        r = McResult()
+        self._check(self.lib.pagani_mc_integrate_genz(self._h, ctypes.byref(g), float(epsrel),
+                                                      float(epsabs), _arr(lows, self.ndim),
+                                                      _arr(highs, self.ndim), ctypes.byref(p),
+                                                      ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in McResult._fields_ if k != "pad"}
+
+    def trace(self, cap=1000):
+        import numpy as np
+        ti = np.zeros(cap)
+        tsi = np.zeros(cap)
+        n = _sz(0)
+        self._check(self.lib.pagani_mc_trace(self._h, ti.ctypes.data_as(_dp),
+                                             tsi.ctypes.data_as(_dp), cap, ctypes.byref(n)))
+        k = min(n.value, cap)
+        return ti[:k], tsi[:k]
+
+    def grid(self):
+        import numpy as np
+        xi = np.zeros(self.ndim * (MC_BINS + 1))
+        self._check(self.lib.pagani_mc_grid(self._h, xi.ctypes.data_as(_dp)))
+        return xi
+
+    def sample_genz(self, family, a, u, ncall, xi, key, generator="lcg", hist_dims=0,
+                    lows=None, highs=None):
+        """One sampling pass on grid xi: (ti, tsi unscaled, hist)."""
+        import numpy as np
+        g = genz(family, a, u)
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        ti = ctypes.c_double()
+        tsi = ctypes.c_double()
+        hist = np.zeros(max(hist_dims, 1) * MC_BINS)
+        self._check(self.lib.pagani_mc_sample_genz(
+            self._h, ctypes.byref(g), float(ncall), _arr(lows, self.ndim),
+            _arr(highs, self.ndim), xi.ctypes.data_as(_dp), int(key),
+            MC_RNG.get(generator, generator), int(hist_dims), ctypes.byref(ti),
+            ctypes.byref(tsi), hist.ctypes.data_as(_dp)))
+        return ti.value, tsi.value, hist[: hist_dims * MC_BINS]
 
 
 def do_integration_from_c():

@@ -38,8 +38,10 @@ HIP_SOURCES = {
     "interop.hip": [],
     "batch.hip": [],
     "checkpoint.hip": [],
+    "mcubes.hip": [],
 }
-CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"]}
+CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"],
+               "mcubes_grid.cpp": ["-O2", "-ffp-contract=off"]}
 
 
 def _run(cmd):

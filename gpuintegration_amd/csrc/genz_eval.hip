@@ -6,6 +6,7 @@
 
 #include "pagani/eval_kernel.hpp"
 #include "pagani/genz.hpp"
+#include "genz_visit.hpp"
 #include "pagani_ext.h"
 
 namespace {
@@ -13,53 +14,9 @@ namespace {
 template <class Real, int D>
 int launch(const pagani_genz* g, const pagani_eval_args* a)
 {
-  switch (g->family) {
-  case PAGANI_GENZ_OSCILLATORY: {
-    pagani::Oscillatory<Real, D> f;
-    for (int i = 0; i < D; ++i) f.a[i] = static_cast<Real>(g->a[i]);
-    f.phase = static_cast<Real>(2.0 * M_PI * g->u[0]);
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  case PAGANI_GENZ_PRODUCT_PEAK: {
-    pagani::ProductPeak<Real, D> f;
-    for (int i = 0; i < D; ++i) {
-      f.ia2[i] = static_cast<Real>(1.0 / (g->a[i] * g->a[i]));
-      f.u[i] = static_cast<Real>(g->u[i]);
-    }
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  case PAGANI_GENZ_CORNER_PEAK: {
-    pagani::CornerPeak<Real, D> f;
-    for (int i = 0; i < D; ++i) f.a[i] = static_cast<Real>(g->a[i]);
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  case PAGANI_GENZ_GAUSSIAN: {
-    pagani::Gaussian<Real, D> f;
-    for (int i = 0; i < D; ++i) {
-      f.a2[i] = static_cast<Real>(g->a[i] * g->a[i]);
-      f.u[i] = static_cast<Real>(g->u[i]);
-    }
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  case PAGANI_GENZ_C0: {
-    pagani::C0<Real, D> f;
-    for (int i = 0; i < D; ++i) {
-      f.a[i] = static_cast<Real>(g->a[i]);
-      f.u[i] = static_cast<Real>(g->u[i]);
-    }
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  case PAGANI_GENZ_DISCONTINUOUS: {
-    pagani::Discontinuous<Real, D> f;
-    for (int i = 0; i < D; ++i) {
-      f.a[i] = static_cast<Real>(g->a[i]);
-      f.u[i] = static_cast<Real>(g->u[i]);
-    }
-    return pagani::eval_regions<D, decltype(f), Real>(f, a);
-  }
-  default:
-    return PAGANI_EBADARG;
-  }
+  return pagani::internal::visit_genz<Real, D>(g, [&](const auto& f) {
+    return pagani::eval_regions<D, std::decay_t<decltype(f)>, Real>(f, a);
+  });
 }
 
 template <class Real>

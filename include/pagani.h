@@ -405,6 +405,125 @@ int pagani_classify_dev(pagani_ws* ws, const double* err, size_t n,
                         double total_finished_errorest, uint8_t* flags,
                         pagani_class_res* out);
 
+/* ---- m-Cubes: VEGAS with stratified sampling ------------------------------- */
+/* The reference's second integrator: cuda/mcubes/vegasT.cuh (vegas(),
+ * vegas_kernel, vegas_kernelF, integrate, simple_integrate, class mcubes),
+ * cuda/mcubes/vegasT1D.cuh (one grid shared by all axes) and
+ * cuda/mcubes/mcubes.cuh (quad::mcubes).  The C++ surface is
+ * include/pagani/mcubes.hpp; these are the entry points under it.
+ *
+ * Importance grid (host and device): axis j has PAGANI_MC_BINS bins with
+ * right edges xi[j*(PAGANI_MC_BINS+1) + i], i = 1..PAGANI_MC_BINS, in
+ * [0, 1], and xi[j*(PAGANI_MC_BINS+1)] = 0.  Bin histograms:
+ * hist[j*PAGANI_MC_BINS + i - 1].  Cubes (ng strata per axis) are numbered
+ * with the last axis fastest (get_indx, vegasT.cuh:137-156). */
+#define PAGANI_MC_BINS 500 /* Internal_Vegas_Params ndmx, vegas_utils.cuh:157 */
+#define PAGANI_MC_MAX_DIM 16
+/* random streams, one per (iteration, cube) */
+#define PAGANI_MC_RNG_DEFAULT 0 /* xoroshiro128+ seeded by SplitMix64 of (seed, iteration, cube);
+                                   stands in for the reference's clock-seeded curand */
+#define PAGANI_MC_RNG_LCG 1     /* Custom_generator, vegas_utils.cuh:24-48: the C LCG
+                                   reseeded with the cube index (same numbers every iteration) */
+
+typedef struct pagani_mc_params {
+  double ncall;      /* calls per iteration (ncall) */
+  int total_iters;   /* titer */
+  int adjust_iters;  /* itmax: the leading iterations that refine the grid */
+  int skip_iters;    /* skip: leading iterations left out of the estimate */
+  int generator;     /* PAGANI_MC_RNG_* */
+  int one_axis;      /* 1: vegasT1D.cuh -- axis 0's grid is adapted and copied to all axes */
+  int adjust_params; /* 1: simple_integrate -- rerun with AdjustParams while status is 1 */
+  uint64_t seed;     /* PAGANI_MC_RNG_DEFAULT streams */
+} pagani_mc_params;
+
+/* numint::integration_result as m-Cubes fills it (estimate, errorest,
+ * chi_sq, status, iters), plus the run's size. */
+typedef struct pagani_mc_result {
+  double estimate;
+  double errorest;
+  double chi_sq;
+  int status;     /* 0: |errorest/estimate| <= epsrel or errorest <= epsabs after >= 5
+                     iterations (GetStatus, vegas_utils.cuh:233-254); 1 otherwise */
+  int pad;
+  uint64_t iters; /* iterations over all runs */
+  uint64_t runs;  /* vegas() runs (simple_integrate retries + 1) */
+  uint64_t neval; /* integrand evaluations over all runs */
+  double ncall;   /* ncall of the last run */
+} pagani_mc_result;
+
+/* One sampling launch: every cube of the stratification, npg points each
+ * (vegas_kernel when hist_dims > 0, vegas_kernelF when 0).  Workgroup g
+ * handles cubes [g*kThreads*cubes_per_thread, ...) and writes its sums of
+ * fb and f2b to part[2g], part[2g+1] and its bin histograms to
+ * hist[g*hist_dims*PAGANI_MC_BINS ...]. */
+typedef struct pagani_mc_args {
+  const double* xi; /* device grid, ndim x (PAGANI_MC_BINS+1) */
+  double lo[PAGANI_MC_MAX_DIM];
+  double dx[PAGANI_MC_MAX_DIM];
+  double xjac, dxg, xnd;
+  uint64_t key;     /* the iteration's stream key (PAGANI_MC_RNG_DEFAULT) */
+  uint32_t ncubes;
+  uint32_t ng, npg;
+  uint32_t cubes_per_thread;
+  uint32_t hist_dims; /* 0, 1 (one_axis) or ndim */
+  uint32_t grid;      /* workgroups */
+  int32_t generator;  /* PAGANI_MC_RNG_* */
+  int32_t pad;
+  double* part;
+  double* hist;
+  void* stream;       /* hipStream_t */
+} pagani_mc_args;
+
+/* A sampling callback launches the m-Cubes kernel for one integrand on
+ * args->stream (pagani::mc::sample_callback<ndim, IntegT, Generator> from
+ * include/pagani/mcubes_kernel.hpp); 0 on success. */
+typedef int (*pagani_mc_sample_fn)(void* ctx, const pagani_mc_args* args);
+
+typedef struct pagani_mc pagani_mc;
+
+typedef struct pagani_mc_stats {
+  double sample_ms; /* sampling kernels (with timing enabled) */
+  double wall_ms;   /* integrate calls */
+  uint64_t samples; /* integrand evaluations */
+  uint64_t launches;
+  uint64_t integrals;
+} pagani_mc_stats;
+
+/* NULL on failure (reason from pagani_mc_last_error(NULL)); ndim in
+ * [1, PAGANI_MC_MAX_DIM]; device < 0 is the current device. */
+pagani_mc* pagani_mc_create(int ndim, int device);
+void pagani_mc_destroy(pagani_mc* mc);
+const char* pagani_mc_last_error(const pagani_mc* mc);
+int pagani_mc_set_timing(pagani_mc* mc, int enable);
+int pagani_mc_get_stats(const pagani_mc* mc, pagani_mc_stats* out);
+int pagani_mc_reset_stats(pagani_mc* mc);
+
+/* vegas() / integrate / simple_integrate (vegasT.cuh:640-1112) for any
+ * integrand behind a sampling callback; lows/highs may be NULL (unit
+ * cube).  Returns PAGANI_OK, or a negative code (then *out holds status 1). */
+int pagani_mc_integrate_cb(pagani_mc* mc, pagani_mc_sample_fn sample, void* ctx,
+                           double epsrel, double epsabs, const double* lows,
+                           const double* highs, const pagani_mc_params* params,
+                           pagani_mc_result* out);
+/* The same for a precompiled Genz integrand (ndim 2..8, FP64). */
+int pagani_mc_integrate_genz(pagani_mc* mc, const pagani_genz* f, double epsrel,
+                             double epsabs, const double* lows, const double* highs,
+                             const pagani_mc_params* params, pagani_mc_result* out);
+/* Per-iteration sums of the last integrate call (ti, and tsi scaled by
+ * dv2g, vegasT.cuh:842-845), up to cap entries; *len = iterations run. */
+int pagani_mc_trace(const pagani_mc* mc, double* ti, double* tsi, size_t cap, size_t* len);
+/* The grid at the end of the last integrate call (host array of
+ * ndim*(PAGANI_MC_BINS+1)). */
+int pagani_mc_grid(const pagani_mc* mc, double* xi);
+/* One sampling pass of a Genz integrand on a given host grid: the phase
+ * under vegas_kernel / vegas_kernelF.  ti = sum fb, tsi = sum f2b (not
+ * scaled by dv2g); hist (hist_dims*PAGANI_MC_BINS, may be NULL when
+ * hist_dims == 0) receives the bin sums of f^2. */
+int pagani_mc_sample_genz(pagani_mc* mc, const pagani_genz* f, double ncall,
+                          const double* lows, const double* highs, const double* xi,
+                          uint64_t key, int generator, int hist_dims, double* ti,
+                          double* tsi, double* hist);
+
 /* ---- the reference's C interop shim ---------------------------------------- */
 /* test/cuda/pagani/do_integration_from_c.cu:20-38: integrates x*y over the
  * unit square to epsrel 1e-6, epsabs 1e-12; writes *res only when the

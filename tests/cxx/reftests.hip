@@ -548,3 +548,162 @@ extern "C" int reftest_exp_neg(const double* x, std::size_t n, double* mine, dou
     return -1;
   }
 }
+
+// ---------------------------------------------------------------------------
+// m-Cubes through the reference's C++ surface (include/pagani/mcubes.hpp).
+// The integrands are the ones of cuda/mcubes/demos (Genz2_6D.cu, Genz3_3D.cu,
+// Genz4_5D.cu, Genz5_8D.cu, Genz6_6D.cu, Genz1_8D.cu, SinSum6D.cu, Gauss9D.cu).
+// ---------------------------------------------------------------------------
+#include "pagani/mcubes.hpp"
+
+namespace {
+
+class MC_F_3_3D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z)
+  {
+    return pow(1 + 3 * x + 2 * y + z, -4);
+  }
+};
+
+class MC_F_6_6D {
+ public:
+  __device__ __host__ double operator()(double u, double v, double w, double x, double y, double z)
+  {
+    if (z > .9 || y > .8 || x > .7 || w > .6 || v > .5 || u > .4) return 0.;
+    return exp(10 * z + 9 * y + 8 * x + 7 * w + 6 * v + 5 * u);
+  }
+};
+
+class MC_F_1_8D {
+ public:
+  __device__ __host__ double operator()(double s, double t, double u, double v, double w,
+                                        double x, double y, double z)
+  {
+    return cos(s + 2. * t + 3. * u + 4. * v + 5. * w + 6. * x + 7. * y + 8. * z);
+  }
+};
+
+class SinSum6D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double k, double l,
+                                        double m)
+  {
+    return sin(x + y + z + k + l + m);
+  }
+};
+
+class Gauss9D {
+ public:
+  __device__ __host__ double operator()(double x, double y, double z, double k, double l,
+                                        double m, double n, double o, double p)
+  {
+    const double sum = x * x + y * y + z * z + k * k + l * l + m * m + n * n + o * o + p * p;
+    const double s = 0.01;
+    return exp(-1 * sum / (2 * s * s)) * (1 / pow(sqrt(2 * M_PI) * s, 9));
+  }
+};
+
+template <int D, class F, class G>
+int mc_run(int mode, double epsrel, double ncall, int titer, int itmax, int skip,
+           const double* lows, const double* highs, pagani_result* out)
+{
+  F f;
+  quad::Volume<double, D> vol;
+  if (lows && highs) vol = quad::Volume<double, D>(lows, highs);
+  numint::integration_result r;
+  const double epsabs = 1e-20;
+  switch (mode) {
+    case 0:
+      r = cuda_mcubes::integrate<F, D, false, G>(f, epsrel, epsabs, ncall, &vol, titer, itmax,
+                                                 skip);
+      break;
+    case 1:
+      r = cuda_mcubes::simple_integrate<F, D, false, G>(f, epsrel, epsabs, ncall, &vol, titer,
+                                                        itmax, skip);
+      break;
+    case 2:
+      r = cuda_mcubes::integrate1D<F, D, false, G>(f, epsrel, epsabs, ncall, &vol, titer, itmax,
+                                                   skip);
+      break;
+    case 3: {
+      // quad::mcubes (mcubes.cuh): arity deduced from F, default generator
+      quad::mcubes m;
+      m.maxcalls = static_cast<long long>(ncall);
+      m.total_iters = titer;
+      m.adjust_iters = itmax;
+      m.skip_iters = skip;
+      r = m.integrate(f, epsabs, epsrel, &vol);
+      break;
+    }
+    case 4: {
+      cuda_mcubes::mcubes m(epsrel, epsabs, ncall, titer, itmax, skip);
+      r = m.integrate<F, D>(f, &vol);
+      break;
+    }
+    default:
+      return -1;
+  }
+  std::memset(out, 0, sizeof(*out));
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->chi_sq = r.chi_sq;
+  out->status = r.status;
+  out->iters = r.iters;
+  out->neval = r.neval;
+  return 0;
+}
+
+template <class G>
+int mc_dispatch(const char* name, int mode, double epsrel, double ncall, int titer, int itmax,
+                int skip, pagani_result* out)
+{
+  if (!std::strcmp(name, "XY")) return mc_run<2, XY, G>(mode, epsrel, ncall, titer, itmax, skip,
+                                                         nullptr, nullptr, out);
+  if (!std::strcmp(name, "F_2_6D"))
+    return mc_run<6, F_2_6D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr, out);
+  if (!std::strcmp(name, "F_3_3D"))
+    return mc_run<3, MC_F_3_3D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr,
+                                   out);
+  if (!std::strcmp(name, "F_4_5D"))
+    return mc_run<5, F_4_5D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr, out);
+  if (!std::strcmp(name, "F_5_8D"))
+    return mc_run<8, F_5_8D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr, out);
+  if (!std::strcmp(name, "F_6_6D"))
+    return mc_run<6, MC_F_6_6D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr,
+                                   out);
+  if (!std::strcmp(name, "F_1_8D"))
+    return mc_run<8, MC_F_1_8D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr,
+                                   out);
+  if (!std::strcmp(name, "SinSum6D")) {
+    const double lo[6] = {0, 0, 0, 0, 0, 0}, hi[6] = {10, 10, 10, 10, 10, 10};
+    return mc_run<6, SinSum6D, G>(mode, epsrel, ncall, titer, itmax, skip, lo, hi, out);
+  }
+  if (!std::strcmp(name, "Gauss9D")) {
+    double lo[9], hi[9];
+    for (int i = 0; i < 9; ++i) {
+      lo[i] = -1;
+      hi[i] = 1;
+    }
+    return mc_run<9, Gauss9D, G>(mode, epsrel, ncall, titer, itmax, skip, lo, hi, out);
+  }
+  return -1;
+}
+
+}  // namespace
+
+// generator: 0 = Curand_generator (the library's default streams), 1 =
+// Custom_generator; mode: 0 integrate, 1 simple_integrate, 2 integrate1D,
+// 3 quad::mcubes, 4 cuda_mcubes::mcubes.
+extern "C" int reftest_mcubes(const char* name, int generator, int mode, double epsrel,
+                              double ncall, int titer, int itmax, int skip, pagani_result* out)
+{
+  if (!name || !out) return -1;
+  try {
+    if (generator == 1)
+      return mc_dispatch<Custom_generator>(name, mode, epsrel, ncall, titer, itmax, skip, out);
+    return mc_dispatch<Curand_generator>(name, mode, epsrel, ncall, titer, itmax, skip, out);
+  } catch (const std::exception&) {
+    return -2;
+  }
+}

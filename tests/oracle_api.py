@@ -95,6 +95,40 @@ class OrcClassRes(ctypes.Structure):
     ]
 
 
+MC_BINS = 500
+MC_MAXDIM = 16
+MC_RNG = {"default": 0, "lcg": 1}
+
+
+class McRng(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("lcg", ctypes.c_uint32), ("s0", ctypes.c_uint64),
+                ("s1", ctypes.c_uint64)]
+
+
+class McParams(ctypes.Structure):
+    _fields_ = [("ncall", ctypes.c_double), ("total_iters", ctypes.c_int),
+                ("adjust_iters", ctypes.c_int), ("skip_iters", ctypes.c_int),
+                ("generator", ctypes.c_int), ("one_axis", ctypes.c_int),
+                ("adjust_params", ctypes.c_int), ("seed", ctypes.c_uint64)]
+
+
+class McSetup(ctypes.Structure):
+    _fields_ = [("ng", ctypes.c_uint32), ("npg", ctypes.c_uint32), ("ncubes", ctypes.c_double),
+                ("calls", ctypes.c_double), ("dv2g", ctypes.c_double), ("dxg", ctypes.c_double),
+                ("xnd", ctypes.c_double), ("xjac", ctypes.c_double),
+                ("lo", ctypes.c_double * 16), ("dx", ctypes.c_double * 16)]
+
+
+class McResult(ctypes.Structure):
+    _fields_ = [("estimate", ctypes.c_double), ("errorest", ctypes.c_double),
+                ("chi_sq", ctypes.c_double), ("status", ctypes.c_int),
+                ("iters", ctypes.c_size_t), ("runs", ctypes.c_size_t), ("ncall", ctypes.c_double)]
+
+
+class McTrace(ctypes.Structure):
+    _fields_ = [("cap", ctypes.c_size_t), ("len", ctypes.c_size_t), ("ti", _dp), ("tsi", _dp)]
+
+
 def _load():
     if not os.path.exists(ORACLE_SO):
         raise RuntimeError(f"oracle not built: {ORACLE_SO} (run __graft_entry__.build())")
@@ -143,6 +177,34 @@ def _load():
     lib.orc_cuhre_seq.argtypes = [
         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
         ctypes.c_size_t, ctypes.POINTER(OrcResult)]
+    # m-Cubes (mcubes_oracle.h)
+    lib.orc_mc_lcg_step.argtypes = [ctypes.c_uint32]
+    lib.orc_mc_lcg_step.restype = ctypes.c_uint32
+    lib.orc_mc_iter_key.argtypes = [ctypes.c_uint64, ctypes.c_int]
+    lib.orc_mc_iter_key.restype = ctypes.c_uint64
+    lib.orc_mc_rng_init.argtypes = [ctypes.POINTER(McRng), ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.c_uint64]
+    lib.orc_mc_rng_init.restype = None
+    lib.orc_mc_rng_next.argtypes = [ctypes.POINTER(McRng)]
+    lib.orc_mc_rng_next.restype = ctypes.c_double
+    lib.orc_mc_get_indx.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
+                                    ctypes.c_int, ctypes.c_uint32]
+    lib.orc_mc_get_indx.restype = None
+    lib.orc_mc_rebin.argtypes = [ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+    lib.orc_mc_rebin.restype = None
+    lib.orc_mc_grid_init.argtypes = [ctypes.c_int, _dp]
+    lib.orc_mc_grid_init.restype = None
+    lib.orc_mc_refine.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+    lib.orc_mc_refine.restype = None
+    lib.orc_mc_setup_init.argtypes = [ctypes.POINTER(McSetup), ctypes.c_int, ctypes.c_double,
+                                      _dp, _dp]
+    lib.orc_mc_setup_init.restype = None
+    lib.orc_mc_sample.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.POINTER(McSetup), _dp,
+                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    lib.orc_mc_sample.restype = None
+    lib.orc_mc_integrate.argtypes = [
+        ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp,
+        ctypes.POINTER(McParams), ctypes.POINTER(McResult), ctypes.POINTER(McTrace), _dp]
     lib.orc_num_threads.restype = ctypes.c_int
     lib.orc_set_num_threads.argtypes = [ctypes.c_int]
     lib.orc_set_num_threads.restype = None
@@ -346,3 +408,94 @@ def ref_closed_form_lib():
     lib.ref_genz_phi.argtypes = [ctypes.c_double]
     lib.ref_genz_phi.restype = ctypes.c_double
     return lib
+
+
+# ---------------------------------------------------------------------------
+# m-Cubes oracle (oracle/mcubes_oracle.c)
+# ---------------------------------------------------------------------------
+
+def mc_lcg_step(s):
+    return LIB.orc_mc_lcg_step(s)
+
+
+def mc_iter_key(seed, it):
+    return LIB.orc_mc_iter_key(seed, it)
+
+
+def mc_randoms(kind, key, cube, count):
+    g = McRng()
+    LIB.orc_mc_rng_init(ctypes.byref(g), MC_RNG.get(kind, kind), key, cube)
+    return np.array([LIB.orc_mc_rng_next(ctypes.byref(g)) for _ in range(count)])
+
+
+def mc_get_indx(m, d, ng):
+    kg = (ctypes.c_uint32 * d)()
+    LIB.orc_mc_get_indx(m, kg, d, ng)
+    return list(kg)
+
+
+def mc_rebin(rc, r, xi):
+    """r[1..nd] (r[0] ignored), xi[0..nd]; returns the new xi."""
+    nd = len(xi) - 1
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    xi = np.array(xi, dtype=np.float64)
+    xin = np.zeros(nd + 1)
+    LIB.orc_mc_rebin(rc, nd, dptr(r), dptr(xin), dptr(xi))
+    return xi
+
+
+def mc_grid_init(d):
+    xi = np.zeros(d * (MC_BINS + 1))
+    LIB.orc_mc_grid_init(d, dptr(xi))
+    return xi
+
+
+def mc_refine(d, hd, hist, xi):
+    hist = np.array(hist, dtype=np.float64)
+    xi = np.array(xi, dtype=np.float64)
+    LIB.orc_mc_refine(d, hd, dptr(hist), dptr(xi))
+    return hist, xi
+
+
+def mc_setup(d, ncall, lows=None, highs=None):
+    s = McSetup()
+    lw = None if lows is None else np.ascontiguousarray(lows, dtype=np.float64)
+    hg = None if highs is None else np.ascontiguousarray(highs, dtype=np.float64)
+    LIB.orc_mc_setup_init(ctypes.byref(s), d, ncall, dptr(lw), dptr(hg))
+    return s
+
+
+def mc_sample(fam, d, prm, setup, xi, key, generator, hd):
+    ti = ctypes.c_double()
+    tsi = ctypes.c_double()
+    hist = np.zeros(max(hd, 1) * MC_BINS)
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    LIB.orc_mc_sample(FAM.get(fam, fam), d, dptr(prm), ctypes.byref(setup), dptr(xi), key,
+                      MC_RNG.get(generator, generator), hd, ctypes.byref(ti), ctypes.byref(tsi),
+                      dptr(hist))
+    return ti.value, tsi.value, hist[: hd * MC_BINS]
+
+
+def mc_integrate(fam, d, prm, epsrel, epsabs, ncall, total_iters=15, adjust_iters=15,
+                 skip_iters=5, generator="lcg", one_axis=False, adjust_params=False, seed=0,
+                 lows=None, highs=None, trace_cap=0):
+    p = McParams(ncall, total_iters, adjust_iters, skip_iters, MC_RNG.get(generator, generator),
+                 int(one_axis), int(adjust_params), seed)
+    res = McResult()
+    tr = None
+    arrs = None
+    if trace_cap:
+        arrs = (np.zeros(trace_cap), np.zeros(trace_cap))
+        tr = McTrace(trace_cap, 0, dptr(arrs[0]), dptr(arrs[1]))
+    lw = None if lows is None else np.ascontiguousarray(lows, dtype=np.float64)
+    hg = None if highs is None else np.ascontiguousarray(highs, dtype=np.float64)
+    xi = np.zeros(d * (MC_BINS + 1))
+    assert LIB.orc_mc_integrate(FAM.get(fam, fam), d, dptr(prm), epsrel, epsabs, dptr(lw),
+                                dptr(hg), ctypes.byref(p), ctypes.byref(res),
+                                ctypes.byref(tr) if tr is not None else None, dptr(xi)) == 0
+    out = dict(estimate=res.estimate, errorest=res.errorest, chi_sq=res.chi_sq,
+               status=res.status, iters=res.iters, runs=res.runs, ncall=res.ncall, xi=xi)
+    if trace_cap:
+        out["ti"] = arrs[0][: tr.len].copy()
+        out["tsi"] = arrs[1][: tr.len].copy()
+    return out
