@@ -180,6 +180,8 @@ def _load():
     # m-Cubes (mcubes_oracle.h)
     lib.orc_mc_lcg_step.argtypes = [ctypes.c_uint32]
     lib.orc_mc_lcg_step.restype = ctypes.c_uint32
+    lib.orc_mc_splitmix64.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+    lib.orc_mc_splitmix64.restype = ctypes.c_uint64
     lib.orc_mc_iter_key.argtypes = [ctypes.c_uint64, ctypes.c_int]
     lib.orc_mc_iter_key.restype = ctypes.c_uint64
     lib.orc_mc_rng_init.argtypes = [ctypes.POINTER(McRng), ctypes.c_int, ctypes.c_uint64,
@@ -416,6 +418,11 @@ def ref_closed_form_lib():
 
 def mc_lcg_step(s):
     return LIB.orc_mc_lcg_step(s)
+
+
+def mc_splitmix64(seed, count):
+    z = ctypes.c_uint64(seed)
+    return [LIB.orc_mc_splitmix64(ctypes.byref(z)) for _ in range(count)]
 
 
 def mc_iter_key(seed, it):
