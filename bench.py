@@ -16,6 +16,11 @@ Other configurations (each prints its own JSON line):
               1e-3..1e-7, FP32 integrand vs FP64, error vs analytic per run
   --config 5  memory-bound: 8D cos(sum (i+1) x_i) and 8D discontinuous at
               epsrel 1e-8 with the library's own memory model
+
+The headline line also carries "mcubes": the m-Cubes VEGAS integrator on the
+same 8D Gaussian (1e9 calls per iteration), samples/s and its kernel's
+algorithmic FP64 rate (--no-mcubes leaves it out, --mcubes-only runs it
+alone).
 """
 import argparse
 import json
@@ -217,6 +222,8 @@ def config2(args, pg, W):
                          abs_err_vs_analytic=abs(r64.estimate - exact), errorest=r64.errorest,
                          nregions=r64.nregions),
         }
+    if not args.no_mcubes:
+        out["mcubes"] = mcubes_leg(args, pg, W)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fam, a, u, args.cpu_seconds)
     return out
@@ -350,6 +357,61 @@ def config4(args, pg, W):
     }
 
 
+def mcubes_leg(args, pg, W, runs=3):
+    """m-Cubes on the same seeded 8D Gaussian: 1e9 calls per iteration,
+    10 iterations (5 adjusting the grid, 2 skipped), default streams; epsrel
+    1e-12 so every run does the same work.  Timed over `runs` runs after
+    one warm-up run."""
+    fam, a, u = W.gaussian_8d(args.seed)
+    d = len(a)
+    exact = W.gaussian_exact(a, u)
+    kw = dict(epsrel=1e-12, epsabs=1e-30, ncall=args.mc_ncall, total_iters=10, adjust_iters=5,
+              skip_iters=2, generator="default", seed=args.seed)
+    with pg.MCubes(d) as mc:
+        mc.integrate_genz(fam, a, u, **kw)
+        mc.set_timing(True)
+        mc.reset_stats()
+        _sync()
+        t0 = time.perf_counter()
+        for _ in range(runs):
+            r = mc.integrate_genz(fam, a, u, **kw)
+        dt = time.perf_counter() - t0
+        st = mc.stats()
+    fl = W.mc_flops_per_sample(fam, d)
+    kern_ms = st["sample_ms"] / max(st["launches"], 1)
+    per_launch = st["samples"] / max(st["launches"], 1)
+    out = {
+        "workload": "m-Cubes VEGAS, same 8D Gaussian, %.0e calls/iteration, 10 iterations "
+                    "(5 grid-adjusting, 2 skipped), default streams" % args.mc_ncall,
+        "samples_per_s": st["samples"] / dt,
+        "ms_per_run": dt / runs * 1e3,
+        "ms_per_iteration": dt / runs / 10 * 1e3,
+        "sample_kernel_avg_ms": kern_ms,
+        "samples_per_launch": per_launch,
+        "flops_per_sample": fl,
+        "fp64_tflops_algorithmic": fl * per_launch / (kern_ms * 1e-3) / 1e12 if kern_ms else None,
+        "fp64_frac_algorithmic": (fl * per_launch / (kern_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS
+                                  if kern_ms else None),
+        "estimate": r["estimate"], "errorest": r["errorest"], "chi_sq": r["chi_sq"],
+        "analytic": exact, "deviation_in_errorest": (r["estimate"] - exact) / r["errorest"],
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_api as O
+        prm = O.params(a, u)
+        n = 2e6
+        t0 = time.perf_counter()
+        O.mc_integrate(fam, d, prm, 1e-12, 1e-30, n, total_iters=2, adjust_iters=1,
+                       skip_iters=0, generator="default", seed=args.seed)
+        cdt = time.perf_counter() - t0
+        s = O.mc_setup(d, n)
+        out["cpu_baseline"] = {"value": 2 * s.calls / cdt, "unit": "samples/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle m-Cubes (one core), 2 iterations of %d calls "
+                                         "in %.1f s" % (s.calls, cdt)}
+    return out
+
+
 def eval_only(args, pg, W):
     """The evaluation kernel in isolation, for hardware-counter runs: one
     device-to-device copy of known size (the reference point that scales
@@ -410,6 +472,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-mcubes", action="store_true")
+    ap.add_argument("--mcubes-only", action="store_true",
+                    help="the m-Cubes leg alone (for profiles)")
+    ap.add_argument("--mc-ncall", type=float, default=1e9)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -420,6 +486,8 @@ def main():
     fn = {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}[args.config]
     if args.eval_only:
         fn = eval_only
+    if args.mcubes_only:
+        fn = mcubes_leg
     print(json.dumps(fn(args, pg, W)), flush=True)
 
 

@@ -163,6 +163,14 @@ def eval_flops_per_point(family, d):
     return 2 * d + cf + 10
 
 
+def mc_flops_per_sample(family, d):
+    """Algorithmic FLOPs per m-Cubes sample: the point map (11 per axis:
+    stratum offset, bin position, bin width and offset, two FMAs, the
+    weight's two products), the integrand, the weight product, the square
+    and the two running sums.  Random-number integer work is not counted."""
+    return 11 * d + (eval_flops_per_point(family, d) - 2 * d - 10) + 4
+
+
 def eval_flops_per_region(family, d):
     F = 1 + 8 * d + 2 * d * (d - 1) + 4 * d * (d - 1) + 4 * d * (d - 1) * (d - 2) // 3 + (1 << d)
     return F * eval_flops_per_point(family, d) + 160

@@ -420,7 +420,7 @@ int pagani_classify_dev(pagani_ws* ws, const double* err, size_t n,
 #define PAGANI_MC_BINS 500 /* Internal_Vegas_Params ndmx, vegas_utils.cuh:157 */
 #define PAGANI_MC_MAX_DIM 16
 /* random streams, one per (iteration, cube) */
-#define PAGANI_MC_RNG_DEFAULT 0 /* xoroshiro128+ seeded by SplitMix64 of (seed, iteration, cube);
+#define PAGANI_MC_RNG_DEFAULT 0 /* xoroshiro128++ seeded by SplitMix64 of (seed, iteration, cube);
                                    stands in for the reference's clock-seeded curand */
 #define PAGANI_MC_RNG_LCG 1     /* Custom_generator, vegas_utils.cuh:24-48: the C LCG
                                    reseeded with the cube index (same numbers every iteration) */

@@ -14,7 +14,7 @@
 //
 // GeneratorType: Custom_generator is the reference's per-cube LCG (same
 // numbers in the same order); Curand_generator, the reference's default,
-// selects this library's default streams (xoroshiro128+ per iteration and
+// selects this library's default streams (xoroshiro128++ per iteration and
 // cube, seeded from a fixed seed) in place of curand's clock-seeded XORWOW,
 // so runs are reproducible.  DEBUG_MCUBES is accepted and ignored (the
 // reference writes sample dumps to files with it).

@@ -3,9 +3,10 @@
 //
 // One launch draws npg points in every cube of the ng^ndim stratification
 // (vegas_kernel / vegas_kernelF, cuda/mcubes/vegasT.cuh:400-612).  Each
-// thread takes cubes_per_thread consecutive cubes, stepping their stratum
-// coordinates like an odometer (vegasT.cuh:386-391); each cube draws from
-// its own random stream, so results do not depend on the launch shape.
+// thread takes a chunk of cubes_per_thread consecutive cubes, stepping their
+// stratum coordinates like an odometer (vegasT.cuh:386-391); each cube draws
+// from its own random stream, so the points do not depend on the launch
+// shape (only the order of the sums does).
 // The importance grid is staged in LDS (one table of right edges per axis)
 // and the adjusting iterations' bin histograms of f^2 are accumulated in
 // LDS (fp64 LDS atomics) and written once per workgroup; the reference adds
@@ -31,16 +32,25 @@ constexpr int kBins = PAGANI_MC_BINS;
 constexpr int kThreads = 256;
 constexpr double kTiny = 1.0e-30;  // TINY, vegas_utils.cuh / seqCodesDefs.hh
 
-// Random stream of the reference's Custom_generator (vegas_utils.cuh:24-48):
+// Random streams.  A stream yields one sample's D numbers as integers x
+// whose value is x * kScale exactly; the kernel forms kg - ran as one fused
+// multiply-add, which rounds exactly as the subtraction of the value would.
+//
+// LcgStream: the reference's Custom_generator (vegas_utils.cuh:24-48),
 // s' = (1103515245 s + 12345) mod 2^31 in 32-bit unsigned arithmetic, the
 // value s' / 2^31, reseeded with the cube index (vegasT.cuh:351-353).
 struct LcgStream {
+  static constexpr double kScale = 0x1.0p-31;
   std::uint32_t s;
   __device__ LcgStream(std::uint64_t, std::uint32_t cube) : s(cube) {}
-  __device__ double operator()()
+  template <int D>
+  __device__ void fill(std::uint32_t (&x)[D])
   {
-    s = (1103515245u * s + 12345u) & 0x7fffffffu;
-    return static_cast<double>(s) * 0x1.0p-31;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      s = (1103515245u * s + 12345u) & 0x7fffffffu;
+      x[j] = s;
+    }
   }
 };
 
@@ -60,10 +70,14 @@ __host__ __device__ inline std::uint64_t iter_key(std::uint64_t seed, int it)
   return splitmix64(z);
 }
 
-// Default stream: xoroshiro128+ (Blackman and Vigna 2018), state from two
-// SplitMix64 outputs of key + cube * 2^64/phi; doubles from the top 53 bits
-// in (0, 1], the range of curand_uniform_double.
+// Default stream: xoroshiro128++ (Blackman and Vigna 2018), state from two
+// SplitMix64 outputs of key + cube * 2^64/phi.  Each 64-bit output gives
+// two numbers, from its low then its high 32 bits x: (x | 1) 2^-32, the
+// midpoints of 2^31 equal cells of (0, 1) (32 bits place a point within its
+// stratum to 2^-31 of the stratum width).  A sample draws ceil(D/2)
+// outputs; with D odd the last high half is unused.
 struct XoroStream {
+  static constexpr double kScale = 0x1.0p-32;
   std::uint64_t s0, s1;
   __device__ XoroStream(std::uint64_t key, std::uint32_t cube)
   {
@@ -71,15 +85,25 @@ struct XoroStream {
     s0 = splitmix64(z);
     s1 = splitmix64(z);
   }
-  __device__ double operator()()
+  __device__ std::uint64_t next()
   {
     const std::uint64_t a = s0;
-    std::uint64_t b = s1;
-    const std::uint64_t r = a + b;
-    b ^= a;
-    s0 = ((a << 24) | (a >> 40)) ^ b ^ (b << 16);
-    s1 = (b << 37) | (b >> 27);
-    return static_cast<double>((r >> 11) + 1) * 0x1.0p-53;
+    const std::uint64_t t = s1 ^ a;
+    const std::uint64_t sum = a + s1;
+    const std::uint64_t r = ((sum << 17) | (sum >> 47)) + a;
+    s0 = ((a << 49) | (a >> 15)) ^ t ^ (t << 21);
+    s1 = (t << 28) | (t >> 36);
+    return r;
+  }
+  template <int D>
+  __device__ void fill(std::uint32_t (&x)[D])
+  {
+#pragma unroll
+    for (int j = 0; j < D; j += 2) {
+      const std::uint64_t v = next();
+      x[j] = static_cast<std::uint32_t>(v) | 1u;
+      if (j + 1 < D) x[j + 1] = static_cast<std::uint32_t>(v >> 32) | 1u;
+    }
   }
 };
 
@@ -123,8 +147,14 @@ __global__ void __launch_bounds__(kThreads) sample_kernel(F f, pagani_mc_args a)
   for (int i = tid; i < hd * kBins; i += kThreads) shist[i] = 0.0;
   __syncthreads();
 
-  const std::uint64_t first =
-      (static_cast<std::uint64_t>(blockIdx.x) * kThreads + tid) * a.cubes_per_thread;
+  // Thread g of the grid (wave W = g / 64, lane l) takes chunk
+  // l * (T / 64) + W of the T chunks: the lanes of a wave work in cubes far
+  // apart, in different strata of every axis, so their bins (LDS reads and
+  // histogram additions) rarely collide.
+  const std::uint32_t T = gridDim.x * kThreads;
+  const std::uint32_t g = blockIdx.x * kThreads + tid;
+  const std::uint32_t chunk = (g & 63u) * (T >> 6) + (g >> 6);
+  const std::uint64_t first = static_cast<std::uint64_t>(chunk) * a.cubes_per_thread;
   double fbg = 0.0, f2bg = 0.0;
   if (first < a.ncubes) {
     const std::uint64_t last64 = first + a.cubes_per_thread;
@@ -133,38 +163,49 @@ __global__ void __launch_bounds__(kThreads) sample_kernel(F f, pagani_mc_args a)
     // get_indx (vegasT.cuh:137-156): digits of the first cube, most
     // significant first, plus one
     std::uint32_t kg[D];
+    double kgd[D];  // kg as doubles, kept in step
     {
       std::uint32_t m = static_cast<std::uint32_t>(first);
 #pragma unroll
       for (int j = D - 1; j >= 0; --j) {
         const std::uint32_t q = m / ng;
         kg[j] = m - q * ng + 1;
+        kgd[j] = static_cast<double>(kg[j]);
         m = q;
       }
     }
     const double dxg = a.dxg, xnd = a.xnd, xjac = a.xjac;
     const double dnpg = static_cast<double>(a.npg);
     for (std::uint32_t c = static_cast<std::uint32_t>(first); c < end; ++c) {
-      Stream g(a.key, c);
+      Stream rng(a.key, c);
       double fb = 0.0, f2b = 0.0;
       for (std::uint32_t k = 0; k < a.npg; ++k) {
         double wgt = xjac;
         double x[D];
+        std::uint32_t ran[D];
         int ia[D];
+        rng.fill(ran);
+        // Setup_Integrand_Eval, vegasT.cuh:213-241, in two passes so that
+        // every axis' pair of bin edges is requested from LDS before the
+        // first is used.  kg - ran00 is one FMA on the stream's integer; bin
+        // 1 takes its left edge 0 from the table instead of a branch.
+        double xn[D], lo[D], hi[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
-          // Setup_Integrand_Eval, vegasT.cuh:213-241; bin 1 takes its left
-          // edge 0 from the table instead of a branch
-          const double ran = g();
-          const double xn = __builtin_fma(static_cast<double>(kg[j]) - ran, dxg, 1.0);
-          int b = static_cast<int>(xn);
+          const double kr = __builtin_fma(-static_cast<double>(ran[j]), Stream::kScale, kgd[j]);
+          xn[j] = __builtin_fma(kr, dxg, 1.0);
+          int b = static_cast<int>(xn[j]);
           b = b > kBins ? kBins : b;
           b = b < 1 ? 1 : b;
           ia[j] = b;
           const double* t = sxi + j * (kBins + 1) + b;
-          const double hi = t[0], lo = t[-1];
-          const double xo = hi - lo;
-          const double rc = __builtin_fma(xn - static_cast<double>(b), xo, lo);
+          hi[j] = t[0];
+          lo[j] = t[-1];
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          const double xo = hi[j] - lo[j];
+          const double rc = __builtin_fma(xn[j] - static_cast<double>(ia[j]), xo, lo[j]);
           x[j] = __builtin_fma(rc, a.dx[j], a.lo[j]);
           wgt *= xo * xnd;
         }
@@ -172,9 +213,11 @@ __global__ void __launch_bounds__(kThreads) sample_kernel(F f, pagani_mc_args a)
         const double f2 = fx * fx;
         fb += fx;
         f2b += f2;
-        for (int j = 0; j < hd; ++j)
-          __hip_atomic_fetch_add(&shist[j * kBins + ia[j] - 1], f2, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+          if (j < hd)  // uniform: a scalar branch, ia[j] stays a register
+            __hip_atomic_fetch_add(&shist[j * kBins + ia[j] - 1], f2, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       // vegasT.cuh:373-384
       f2b = __builtin_sqrt(f2b * dnpg);
@@ -186,9 +229,11 @@ __global__ void __launch_bounds__(kThreads) sample_kernel(F f, pagani_mc_args a)
       for (int j = D - 1; j >= 0; --j) {
         if (kg[j] != ng) {
           ++kg[j];
+          kgd[j] = static_cast<double>(kg[j]);
           break;
         }
         kg[j] = 1;
+        kgd[j] = 1.0;
       }
     }
   }

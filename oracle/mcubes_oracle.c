@@ -60,36 +60,51 @@ uint64_t orc_mc_iter_key(uint64_t seed, int it)
 
 /* Stream of one cube.  LCG: the reference's Custom_generator, reseeded with
  * the cube index before each cube (vegasT.cuh:351-353) -- the same numbers
- * every iteration.  Default: xoroshiro128+ (Blackman and Vigna 2018) seeded
- * by two SplitMix64 outputs of key + cube * gamma; the reference's default,
- * curand's XORWOW seeded from the clock, is not reproducible and is replaced
- * by this counter-seeded stream (results depend on the seed and the cube
- * numbering only, not on the launch shape). */
+ * every iteration.  Default: xoroshiro128++ (Blackman and Vigna 2018)
+ * seeded by two SplitMix64 outputs of key + cube * gamma; the reference's
+ * default, curand's XORWOW seeded from the clock, is not reproducible and
+ * is replaced by this counter-seeded stream (results depend on the seed and
+ * the cube numbering only, not on the launch shape). */
 void orc_mc_rng_init(orc_mc_rng* g, int kind, uint64_t key, uint64_t cube)
 {
   g->kind = kind;
   g->lcg = (uint32_t)cube;
-  uint64_t z = key + cube * 0x9e3779b97f4a7c15ull;
+  uint64_t z = key + (uint64_t)(uint32_t)cube * 0x9e3779b97f4a7c15ull;
   g->s0 = orc_mc_splitmix64(&z);
   g->s1 = orc_mc_splitmix64(&z);
 }
 
 static uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
-double orc_mc_rng_next(orc_mc_rng* g)
+static uint64_t xoro_next(orc_mc_rng* g)
 {
-  if (g->kind == ORC_MC_RNG_LCG) {
-    g->lcg = orc_mc_lcg_step(g->lcg);
-    return (double)g->lcg / 2147483648.0;
-  }
   const uint64_t s0 = g->s0;
   uint64_t s1 = g->s1;
-  const uint64_t r = s0 + s1;
+  const uint64_t r = rotl64(s0 + s1, 17) + s0;
   s1 ^= s0;
-  g->s0 = rotl64(s0, 24) ^ s1 ^ (s1 << 16);
-  g->s1 = rotl64(s1, 37);
-  /* top 53 bits, mapped to (0, 1] like curand_uniform_double */
-  return (double)((r >> 11) + 1) * 0x1.0p-53;
+  g->s0 = rotl64(s0, 49) ^ s1 ^ (s1 << 21);
+  g->s1 = rotl64(s1, 28);
+  return r;
+}
+
+/* One sample's n numbers in axis order.  Default stream: each 64-bit output
+ * gives two, from its low then its high 32 bits x: (x | 1) 2^-32, the
+ * midpoints of 2^31 equal cells of (0, 1); with n odd the last high half is
+ * unused. */
+void orc_mc_rng_fill(orc_mc_rng* g, int n, double* r)
+{
+  if (g->kind == ORC_MC_RNG_LCG) {
+    for (int j = 0; j < n; ++j) {
+      g->lcg = orc_mc_lcg_step(g->lcg);
+      r[j] = (double)g->lcg / 2147483648.0;
+    }
+    return;
+  }
+  for (int j = 0; j < n; j += 2) {
+    const uint64_t v = xoro_next(g);
+    r[j] = (double)((uint32_t)v | 1u) * 0x1.0p-32;
+    if (j + 1 < n) r[j + 1] = (double)((uint32_t)(v >> 32) | 1u) * 0x1.0p-32;
+  }
 }
 
 /* ======================================================================== */
@@ -229,9 +244,11 @@ void orc_mc_sample(int fam, int d, const double* prm, const orc_mc_setup* s,
     double fb = 0.0, f2b = 0.0;
     for (uint32_t k = 1; k <= s->npg; ++k) {
       double wgt = s->xjac;
+      double ran[ORC_MC_MAXDIM];
+      orc_mc_rng_fill(&g, d, ran);
       for (int j = 0; j < d; ++j) {
         const double* xj = xi + (size_t)j * (NB + 1);
-        const double ran00 = orc_mc_rng_next(&g);
+        const double ran00 = ran[j];
         const double xn = fma((double)kg[j] - ran00, s->dxg, 1.0);
         int a = (int)xn;
         if (a > NB) a = NB;

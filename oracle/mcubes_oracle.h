@@ -47,7 +47,8 @@ uint32_t orc_mc_lcg_step(uint32_t s);
 uint64_t orc_mc_splitmix64(uint64_t* z);
 uint64_t orc_mc_iter_key(uint64_t seed, int it);
 void orc_mc_rng_init(orc_mc_rng* g, int kind, uint64_t key, uint64_t cube);
-double orc_mc_rng_next(orc_mc_rng* g);
+/* the n numbers of one sample, in axis order */
+void orc_mc_rng_fill(orc_mc_rng* g, int n, double* r);
 
 /* kg[0..d-1] = 1-based stratum coordinates of cube m (last axis fastest) */
 void orc_mc_get_indx(uint64_t m, uint32_t* kg, int d, uint32_t ng);

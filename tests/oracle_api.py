@@ -187,8 +187,8 @@ def _load():
     lib.orc_mc_rng_init.argtypes = [ctypes.POINTER(McRng), ctypes.c_int, ctypes.c_uint64,
                                     ctypes.c_uint64]
     lib.orc_mc_rng_init.restype = None
-    lib.orc_mc_rng_next.argtypes = [ctypes.POINTER(McRng)]
-    lib.orc_mc_rng_next.restype = ctypes.c_double
+    lib.orc_mc_rng_fill.argtypes = [ctypes.POINTER(McRng), ctypes.c_int, _dp]
+    lib.orc_mc_rng_fill.restype = None
     lib.orc_mc_get_indx.argtypes = [ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint32),
                                     ctypes.c_int, ctypes.c_uint32]
     lib.orc_mc_get_indx.restype = None
@@ -429,10 +429,19 @@ def mc_iter_key(seed, it):
     return LIB.orc_mc_iter_key(seed, it)
 
 
-def mc_randoms(kind, key, cube, count):
+def mc_randoms(kind, key, cube, count, per_sample=None):
+    """count numbers of one cube's stream, drawn per_sample at a time (one
+    sample's axes; default: all in one draw)."""
     g = McRng()
     LIB.orc_mc_rng_init(ctypes.byref(g), MC_RNG.get(kind, kind), key, cube)
-    return np.array([LIB.orc_mc_rng_next(ctypes.byref(g)) for _ in range(count)])
+    per = per_sample or count
+    out = np.zeros(count)
+    buf = np.zeros(per)
+    for i in range(0, count, per):
+        LIB.orc_mc_rng_fill(ctypes.byref(g), per, dptr(buf))
+        k = min(per, count - i)
+        out[i:i + k] = buf[:k]
+    return out
 
 
 def mc_get_indx(m, d, ng):

@@ -45,10 +45,13 @@ def test_splitmix64_published_vectors():
                                            9817491932198370423]
 
 
-def test_default_stream_is_uniform_on_open_closed_unit_interval():
+def test_default_stream_is_uniform_on_the_open_unit_interval():
     key = O.mc_iter_key(0, 1)
     x = np.concatenate([O.mc_randoms("default", key, c, 200) for c in range(200)])
-    assert np.all(x > 0) and np.all(x <= 1)
+    assert np.all(x > 0) and np.all(x < 1)
+    # cell midpoints: odd multiples of 2^-32
+    k = x * 2.0**32
+    assert np.all(k == np.round(k)) and np.all(k.astype(np.int64) % 2 == 1)
     n = x.size
     assert abs(x.mean() - 0.5) < 4 * np.sqrt(1 / 12 / n)
     counts, _ = np.histogram(x, bins=50, range=(0, 1))
