@@ -33,22 +33,37 @@ constexpr int kOutDoubles = 2 + PAGANI_MC_MAX_DIM * kBins;
 
 thread_local std::string g_mc_create_error;
 
-// Sums over the workgroups in index order: out[0] = sum fb, out[1] =
-// sum f2b, out[2 + i] = histogram entry i.
-__global__ void __launch_bounds__(256)
+// Sums over the workgroups: out[0] = sum fb, out[1] = sum f2b, out[2 + i] =
+// histogram entry i.  Workgroup b folds 64 entries: wave w of its 16 adds
+// the partials of workgroups w, w + 16, ... in that order, then the wave
+// sums are added in wave order -- a fixed order for any run.
+constexpr int kRedWaves = 16;
+
+__global__ void __launch_bounds__(64 * kRedWaves)
     k_mc_reduce(const double* __restrict__ part, const double* __restrict__ hist,
                 std::uint32_t grid, std::uint32_t nhist, double* __restrict__ out)
 {
-  const std::uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= nhist + 2) return;
+  __shared__ double ws[kRedWaves][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const std::uint32_t t = blockIdx.x * 64 + lane;  // output entry
   double s = 0.0;
-  if (t < 2) {
-    for (std::uint32_t g = 0; g < grid; ++g) s += part[2 * g + t];
-  } else {
-    const std::uint32_t i = t - 2;
-    for (std::uint32_t g = 0; g < grid; ++g) s += hist[static_cast<std::size_t>(g) * nhist + i];
+  if (t < nhist + 2) {
+    if (t < 2) {
+      for (std::uint32_t g = wave; g < grid; g += kRedWaves) s += part[2 * g + t];
+    } else {
+      const std::uint32_t i = t - 2;
+      for (std::uint32_t g = wave; g < grid; g += kRedWaves)
+        s += hist[static_cast<std::size_t>(g) * nhist + i];
+    }
   }
-  out[t] = s;
+  ws[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && t < nhist + 2) {
+    double r = ws[0][lane];
+#pragma unroll
+    for (int w = 1; w < kRedWaves; ++w) r += ws[w][lane];
+    out[t] = r;
+  }
 }
 
 }  // namespace
@@ -147,8 +162,8 @@ int sample_pass(pagani_mc* mc, pagani_mc_sample_fn sample, void* ctx, pagani_mc_
   const int rc = sample(ctx, &a);
   if (rc) return fail(mc, rc < 0 ? rc : PAGANI_EHIP, "sampling callback failed");
   if (mc->timing) MC_TRY(mc, hipEventRecord(mc->ev1, mc->stream));
-  hipLaunchKernelGGL(k_mc_reduce, dim3((nhist + 2 + 255) / 256), dim3(256), 0, mc->stream,
-                     mc->d_part, mc->d_hist, a.grid, nhist, mc->d_out);
+  hipLaunchKernelGGL(k_mc_reduce, dim3((nhist + 2 + 63) / 64), dim3(64 * kRedWaves), 0,
+                     mc->stream, mc->d_part, mc->d_hist, a.grid, nhist, mc->d_out);
   MC_TRY(mc, hipGetLastError());
   MC_TRY(mc, hipMemcpyAsync(mc->h_out, mc->d_out, (nhist + 2) * sizeof(double),
                             hipMemcpyDeviceToHost, mc->stream));
