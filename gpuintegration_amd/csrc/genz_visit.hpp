@@ -40,6 +40,7 @@ int visit_genz(const pagani_genz* g, Fn&& fn)
     case PAGANI_GENZ_GAUSSIAN: {
       pagani::Gaussian<Real, D> f;
       for (int i = 0; i < D; ++i) {
+        f.a[i] = static_cast<Real>(g->a[i]);
         f.a2[i] = static_cast<Real>(g->a[i] * g->a[i]);
         f.u[i] = static_cast<Real>(g->u[i]);
       }

@@ -72,6 +72,15 @@ template <class F>
 struct has_origin<F, std::void_t<decltype(std::declval<const F&>().origin(0))>>
     : std::true_type {};
 
+// A centred integrand may also give `double axis_scale(int i) const` (a_i):
+// its `centered` then takes v = a_i (x_i - u_i), and the kernel scales the
+// region's centre offset and half-widths by a_i once per region.
+template <class F, class = void>
+struct has_axis_scale : std::false_type {};
+template <class F>
+struct has_axis_scale<F, std::void_t<decltype(std::declval<const F&>().axis_scale(0))>>
+    : std::true_type {};
+
 template <class F, class Real, int D, std::size_t... I>
 __device__ __forceinline__ double call_at(F& f, const Real (&x)[D],
                                           std::index_sequence<I...>)
@@ -136,6 +145,10 @@ __device__ __forceinline__ void eval_regions_wave(
     cR[j] = lows[j] + (0.5 * (lo + up)) * range[j];
     hR[j] = r * range[j];
     if constexpr (has_origin<F>::value) cR[j] -= f.origin(j);
+    if constexpr (has_origin<F>::value && has_axis_scale<F>::value) {
+      cR[j] *= f.axis_scale(j);
+      hR[j] *= f.axis_scale(j);
+    }
   }
 
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};

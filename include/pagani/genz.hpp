@@ -128,6 +128,7 @@ struct CornerPeak {
 // exp(-sum a_i^2 (x_i - u_i)^2)
 template <class Real, int D>
 struct Gaussian {
+  Real a[D];   // a_i
   Real a2[D];  // a_i^2
   Real u[D];
   template <class... X>
@@ -143,14 +144,18 @@ struct Gaussian {
     }
     return math::exp_neg(-s);
   }
+  // f(u + v / a): the evaluation kernel hands over v = a (x - u), the
+  // region's centre and half-widths scaled by a once per region, so a point
+  // costs one FMA per axis for v and one for the sum of squares
   __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  __device__ double axis_scale(int i) const { return static_cast<double>(a[i]); }
   template <class... T>
-  __device__ Real centered(T... ts) const
+  __device__ Real centered(T... vs) const
   {
-    const Real t[D] = {static_cast<Real>(ts)...};
-    Real s = a2[0] * (t[0] * t[0]);
+    const Real v[D] = {static_cast<Real>(vs)...};
+    Real s = v[0] * v[0];
 #pragma unroll
-    for (int i = 1; i < D; ++i) s += a2[i] * (t[i] * t[i]);
+    for (int i = 1; i < D; ++i) s = __builtin_fma(v[i], v[i], s);
     return math::exp_neg(-s);
   }
 };

@@ -79,13 +79,17 @@ EPS = 2.0 ** -53
 
 
 def centred_cond(family, a):
-    """Relative difference the centred integrand forms may add: the device
+    """Relative difference the centred integrand forms may add.  The device
     forms t = x - u in one FMA from (centre - u), the oracle as (centre - g h)
     - u, so t differs by about 2 eps (|t| <= 1); f inherits
-    sum_j |d log f / d t_j| |dt_j|: 2 a_j^2 |t_j| for the Gaussian, a_j for
-    C0, at most a_j for the product peak."""
+    sum_j |d log f / d t_j| |dt_j|: a_j for C0, at most a_j for the product
+    peak.  The Gaussian takes v = a (x - u) from the centre offset and
+    half-width each scaled by a (three roundings: |dv| <= eps a (|c - u| +
+    |g h| + |t|) <= 2.5 eps a), so v^2 moves by at most 5 eps a^2; with the
+    oracle's own t error (2 eps, 4 eps a^2 on a^2 t^2) and the roundings of
+    the squares and the sum, 12 eps a_j^2 per axis."""
     if family == "gaussian":
-        return 4 * EPS * sum(2 * v * v for v in a)
+        return 12 * EPS * sum(v * v for v in a)
     if family in ("c0", "product_peak"):
         return 4 * EPS * sum(abs(v) for v in a)
     return 0.0
