@@ -155,7 +155,7 @@ struct Gaussian {
     const Real v[D] = {static_cast<Real>(vs)...};
     Real s = v[0] * v[0];
 #pragma unroll
-    for (int i = 1; i < D; ++i) s = __builtin_fma(v[i], v[i], s);
+    for (int i = 1; i < D; ++i) s += v[i] * v[i];  // one FMA per axis (contracted)
     return math::exp_neg(-s);
   }
 };
