@@ -604,6 +604,34 @@ class Gauss9D {
   }
 };
 
+class Gauss1D {
+ public:
+  __device__ double operator()(double x) const { return exp(-50.0 * (x - 0.3) * (x - 0.3)); }
+};
+
+// 12D and 16D products of cos: exact prod_j sin(1)/1 = sin(1)^d on [0, 1]^d
+template <int D>
+struct CosProd;
+template <>
+struct CosProd<12> {
+  __device__ double operator()(double a, double b, double c, double d, double e, double f,
+                               double g, double h, double i, double j, double k, double l) const
+  {
+    return cos(a) * cos(b) * cos(c) * cos(d) * cos(e) * cos(f) * cos(g) * cos(h) * cos(i) *
+           cos(j) * cos(k) * cos(l);
+  }
+};
+template <>
+struct CosProd<16> {
+  __device__ double operator()(double a, double b, double c, double d, double e, double f,
+                               double g, double h, double i, double j, double k, double l,
+                               double m, double n, double o, double p) const
+  {
+    return cos(a) * cos(b) * cos(c) * cos(d) * cos(e) * cos(f) * cos(g) * cos(h) * cos(i) *
+           cos(j) * cos(k) * cos(l) * cos(m) * cos(n) * cos(o) * cos(p);
+  }
+};
+
 template <int D, class F, class G>
 int mc_run(int mode, double epsrel, double ncall, int titer, int itmax, int skip,
            const double* lows, const double* highs, pagani_result* out)
@@ -679,6 +707,14 @@ int mc_dispatch(const char* name, int mode, double epsrel, double ncall, int tit
     const double lo[6] = {0, 0, 0, 0, 0, 0}, hi[6] = {10, 10, 10, 10, 10, 10};
     return mc_run<6, SinSum6D, G>(mode, epsrel, ncall, titer, itmax, skip, lo, hi, out);
   }
+  if (!std::strcmp(name, "Gauss1D"))
+    return mc_run<1, Gauss1D, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr, out);
+  if (!std::strcmp(name, "CosProd12"))
+    return mc_run<12, CosProd<12>, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr,
+                                      out);
+  if (!std::strcmp(name, "CosProd16"))
+    return mc_run<16, CosProd<16>, G>(mode, epsrel, ncall, titer, itmax, skip, nullptr, nullptr,
+                                      out);
   if (!std::strcmp(name, "Gauss9D")) {
     double lo[9], hi[9];
     for (int i = 0; i < 9; ++i) {

@@ -232,6 +232,11 @@ DEMOS = [
     ("XY", 0, 3, 1e-3, 1e6, 70, 20, 0, 0.25),
     ("XY", 1, 4, 1e-3, 1e6, 15, 10, 5, 0.25),
     ("F_4_5D", 0, 2, 1e-3, 1e6, 100, 20, 5, 1.79132603674879e-06),
+    # the dimension range beyond the demos: 1D, and 12D / 16D, whose LDS
+    # (grid + histograms, 96 KB / 128 KB) exceeds the default 64 KB
+    ("Gauss1D", 0, 0, 1e-4, 1e5, 20, 10, 2, 0.25032445820538396),
+    ("CosProd12", 0, 0, 1e-3, 1e7, 20, 10, 3, 0.8414709848078965 ** 12),
+    ("CosProd16", 1, 0, 1e-3, 1e7, 20, 10, 3, 0.8414709848078965 ** 16),
 ]
 
 
