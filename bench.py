@@ -42,13 +42,15 @@ EVAL_HBM_BYTES_PER_REGION_8D = 146.13
 # FP64 work the 8D Gaussian evaluation kernel executes per region, from one
 # rocprofv3 counter pass over `bench.py --eval-only` (SQ_INSTS_VALU_FMA_F64,
 # _MUL_F64, _ADD_F64 and SQ_INSTS_VALU, one lane per region, every lane
-# active): 30723 FMA + 2281 MUL + 1146 ADD = 64873 FLOP and 39073 VALU
-# instructions.  The algorithmic count (90770 FLOP) charges the Gaussian
-# 4 FLOP per axis and point; the kernel forms a_j (x_j - u_j) from a centre
-# and half-width pre-scaled per region and pays 2, so the algorithmic rate
-# would exceed the FP64 peak: the roofline uses the executed FLOPs.
-EVAL_EXEC_FLOP_PER_REGION_8D = 64873
-EVAL_VALU_PER_REGION_8D = 39073
+# active): 18439 FMA + 1468 MUL + 2122 ADD = 40468 FLOP and 27497 VALU
+# instructions.  The algorithmic count (90770 FLOP) charges every one of the
+# 1105 points a full d-axis evaluation; the kernel evaluates the points that
+# move at most three coordinates off the centre from per-region term
+# differences (the Gaussian's exponent is a sum over axes), so the
+# algorithmic rate would exceed the FP64 peak: the roofline uses the
+# executed FLOPs.
+EVAL_EXEC_FLOP_PER_REGION_8D = 40468
+EVAL_VALU_PER_REGION_8D = 27497
 VALU_ISSUE_PEAK = 256 * 2.4e9  # wave-instructions per second (one per CU per clock)
 
 

@@ -92,6 +92,8 @@ class EvalConsts(ctypes.Structure):
         ("set_begin", ctypes.c_int * (NSETS + 1)),
         ("slices", ctypes.c_int),
         ("slice_begin", ctypes.c_int * (MAX_SLICES + 1)),
+        ("lambda3", ctypes.c_double),
+        ("lambda4", ctypes.c_double),
     ]
 
 

@@ -69,6 +69,8 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
   C.jac = jac;
   C.lambda1 = R.lambda[0];
   C.lambda2 = R.lambda[1];
+  C.lambda3 = R.lambda[2];
+  C.lambda4 = R.lambda[3];
   const double q = R.lambda[1] / R.lambda[0];
   C.ratio = q * q;
   C.gtab = ws->d_gtab;

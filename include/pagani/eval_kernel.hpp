@@ -81,6 +81,19 @@ template <class F>
 struct has_axis_scale<F, std::void_t<decltype(std::declval<const F&>().axis_scale(0))>>
     : std::true_type {};
 
+// A centred, scaled integrand that is additively separable in v --
+// f = outer(sum_j term(j, v_j)) -- may say so with `term` and `outer`.  The
+// kernel then evaluates the points of sets 1-7, which move at most three
+// coordinates off the centre, as outer(S0 + the changed axes' term
+// differences): the centre's terms and the differences at +-lambda1 and
+// +-lambda2 are computed once per region.
+template <class F, class = void>
+struct is_separable : std::false_type {};
+template <class F>
+struct is_separable<F, std::void_t<decltype(std::declval<const F&>().term(0, 0.0)),
+                                   decltype(std::declval<const F&>().outer(0.0))>>
+    : std::bool_constant<has_origin<F>::value && has_axis_scale<F>::value> {};
+
 template <class F, class Real, int D, std::size_t... I>
 __device__ __forceinline__ double call_at(F& f, const Real (&x)[D],
                                           std::index_sequence<I...>)
@@ -113,6 +126,133 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
     acc += call(f, x);
   }
   return acc;
+}
+
+// Rule sums and split axis of one region for a separable integrand
+// (slices == 1: this wave evaluates all points).  Points are visited in the
+// generic order of every set, and each set's values are summed in that
+// order, as set_sum does; a point x = c - g h changes v_j by -g h_j.
+template <int D, class F>
+__device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+                                               double jac, const pagani_eval_consts& C,
+                                               double (&rs)[PAGANI_NRULES], int& bisect)
+{
+  double T0[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
+  double S0 = T0[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) S0 += T0[j];
+  // term differences at g = +lambda (x = c - lambda h) and g = -lambda
+  double d1p[D], d1n[D], d2p[D], d2n[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    d1p[j] = f.term(j, __builtin_fma(-C.lambda1, hR[j], cR[j])) - T0[j];
+    d1n[j] = f.term(j, __builtin_fma(C.lambda1, hR[j], cR[j])) - T0[j];
+    d2p[j] = f.term(j, __builtin_fma(-C.lambda2, hR[j], cR[j])) - T0[j];
+    d2n[j] = f.term(j, __builtin_fma(C.lambda2, hR[j], cR[j])) - T0[j];
+  }
+  // sets 0-2 and the fourth difference, Sample.cuh:176-218
+  const double f0 = f.outer(S0);
+  const double base = (f0 * jac) * 2 * (1 - C.ratio);
+  double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const double fa = f.outer(S0 + d1p[j]);
+    const double fb = f.outer(S0 + d1n[j]);
+    const double fc = f.outer(S0 + d2p[j]);
+    const double fd = f.outer(S0 + d2n[j]);
+    s1 += fa + fb;
+    s2 += fc + fd;
+    const double diff =
+        __builtin_fabs(base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
+    if (diff > maxdiff) {
+      maxdiff = diff;
+      bisect = j;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r)
+    rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
+  // sets 3-4: +-lambda3, +-lambda4 on one axis
+  const double lam34[2] = {C.lambda3, C.lambda4};
+#pragma unroll
+  for (int s = 3; s <= 4; ++s) {
+    const double lam = lam34[s - 3];
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      acc += f.outer(S0 + (f.term(j, __builtin_fma(-lam, hR[j], cR[j])) - T0[j]));
+      acc += f.outer(S0 + (f.term(j, __builtin_fma(lam, hR[j], cR[j])) - T0[j]));
+    }
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
+  }
+  // set 5: (+-lambda1, +-lambda1) on axes j < k
+  {
+    double acc = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < D; ++j) {
+      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+#pragma unroll 1
+      for (int k = j + 1; k < D; ++k) {
+        acc += f.outer(aj[0] + d1p[k]);
+        acc += f.outer(aj[1] + d1p[k]);
+        acc += f.outer(aj[0] + d1n[k]);
+        acc += f.outer(aj[1] + d1n[k]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[5][r], acc, rs[r]);
+  }
+  // set 6: (+-lambda1 on axis j, +-lambda2 on axis k != j)
+  {
+    double acc = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < D; ++j) {
+      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+#pragma unroll 1
+      for (int k = 0; k < D; ++k) {
+        if (k == j) continue;
+        acc += f.outer(aj[0] + d2p[k]);
+        acc += f.outer(aj[1] + d2p[k]);
+        acc += f.outer(aj[0] + d2n[k]);
+        acc += f.outer(aj[1] + d2n[k]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[6][r], acc, rs[r]);
+  }
+  // set 7: (+-lambda1, +-lambda1, +-lambda1) on axes j < k < l
+  {
+    double acc = 0.0;
+#pragma unroll 1
+    for (int j = 0; j < D; ++j) {
+      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+#pragma unroll 1
+      for (int k = j + 1; k < D; ++k) {
+        const double ajk[4] = {aj[0] + d1p[k], aj[1] + d1p[k], aj[0] + d1n[k], aj[1] + d1n[k]};
+#pragma unroll 1
+        for (int l = k + 1; l < D; ++l) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc += f.outer(ajk[q] + d1p[l]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc += f.outer(ajk[q] + d1n[l]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
+  }
+  // set 8: all coordinates +-lambda5, from the table
+  {
+    const int p0 = C.set_begin[3];
+    const double acc = set_sum<D, F, double>(
+        f, C.gtab + static_cast<std::size_t>(C.set_begin[8] - p0) * D, C.set_begin[8],
+        C.set_begin[9], cR, hR);
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+  }
 }
 
 // One wave's share of the evaluation of up to 64 regions, one lane per
@@ -154,74 +294,83 @@ __device__ __forceinline__ void eval_regions_wave(
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int bisect = maxdim;
 
-  if (wave == 0) {
-    // Centre and the +-lambda1, +-lambda2 axis points (sets 0-2), whose
-    // values give the fourth difference per axis, Sample.cuh:176-218.
-    Real xc[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) xc[j] = static_cast<Real>(cR[j]);
-    const double f0 = detail::call(f, xc);
-    const double base = (f0 * jac) * 2 * (1 - C.ratio);
-    double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      Real x[D];
-#pragma unroll
-      for (int k = 0; k < D; ++k) x[k] = xc[k];
-      x[j] = static_cast<Real>(__builtin_fma(-C.lambda1, hR[j], cR[j]));
-      const double fa = detail::call(f, x);
-      x[j] = static_cast<Real>(__builtin_fma(C.lambda1, hR[j], cR[j]));
-      const double fb = detail::call(f, x);
-      x[j] = static_cast<Real>(__builtin_fma(-C.lambda2, hR[j], cR[j]));
-      const double fc = detail::call(f, x);
-      x[j] = static_cast<Real>(__builtin_fma(C.lambda2, hR[j], cR[j]));
-      const double fd = detail::call(f, x);
-      s1 += fa + fb;
-      s2 += fc + fd;
-      const double diff = __builtin_fabs(
-          base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
-      if (diff > maxdiff) {
-        maxdiff = diff;
-        bisect = j;
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r)
-      rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
-  }
-
-  // Sets 3..8 from the generator table, this wave's slice of the points.
-  const int p0 = C.set_begin[3];
-  int pa, pb;
-  slice_range(slices, wave, p0, C.feval, pa, pb);
-#pragma unroll
-  for (int s = 3; s < PAGANI_NSETS; ++s) {
-    const int a = pa > C.set_begin[s] ? pa : C.set_begin[s];
-    const int b = pb < C.set_begin[s + 1] ? pb : C.set_begin[s + 1];
-    if (a < b) {
-      const double acc = detail::set_sum<D, F, Real>(
-          f, C.gtab + static_cast<std::size_t>(a - p0) * D, a, b, cR, hR);
-#pragma unroll
-      for (int r = 0; r < PAGANI_NRULES; ++r)
-        rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
+  bool done = false;
+  if constexpr (is_separable<F>::value && std::is_same<Real, double>::value) {
+    if (slices == 1) {
+      separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
+      done = true;
     }
   }
-
-  if (slices > 1) {
-    if (wave > 0) {
-#pragma unroll
-      for (int r = 0; r < PAGANI_NRULES; ++r)
-        part[((wave - 1) * PAGANI_NRULES + r) * 64 + lane] = rs[r];
-    }
-    __syncthreads();
+  if (!done) {
     if (wave == 0) {
-      for (int w = 1; w < slices; ++w) {
+      // Centre and the +-lambda1, +-lambda2 axis points (sets 0-2), whose
+      // values give the fourth difference per axis, Sample.cuh:176-218.
+      Real xc[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) xc[j] = static_cast<Real>(cR[j]);
+      const double f0 = detail::call(f, xc);
+      const double base = (f0 * jac) * 2 * (1 - C.ratio);
+      double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        Real x[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) x[k] = xc[k];
+        x[j] = static_cast<Real>(__builtin_fma(-C.lambda1, hR[j], cR[j]));
+        const double fa = detail::call(f, x);
+        x[j] = static_cast<Real>(__builtin_fma(C.lambda1, hR[j], cR[j]));
+        const double fb = detail::call(f, x);
+        x[j] = static_cast<Real>(__builtin_fma(-C.lambda2, hR[j], cR[j]));
+        const double fc = detail::call(f, x);
+        x[j] = static_cast<Real>(__builtin_fma(C.lambda2, hR[j], cR[j]));
+        const double fd = detail::call(f, x);
+        s1 += fa + fb;
+        s2 += fc + fd;
+        const double diff = __builtin_fabs(
+            base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
+        if (diff > maxdiff) {
+          maxdiff = diff;
+          bisect = j;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r)
+        rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
+    }
+
+    // Sets 3..8 from the generator table, this wave's slice of the points.
+    const int p0 = C.set_begin[3];
+    int pa, pb;
+    slice_range(slices, wave, p0, C.feval, pa, pb);
+#pragma unroll
+    for (int s = 3; s < PAGANI_NSETS; ++s) {
+      const int a = pa > C.set_begin[s] ? pa : C.set_begin[s];
+      const int b = pb < C.set_begin[s + 1] ? pb : C.set_begin[s + 1];
+      if (a < b) {
+        const double acc = detail::set_sum<D, F, Real>(
+            f, C.gtab + static_cast<std::size_t>(a - p0) * D, a, b, cR, hR);
 #pragma unroll
         for (int r = 0; r < PAGANI_NRULES; ++r)
-          rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
+          rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
       }
     }
-    if (wave > 0) return;
+
+    if (slices > 1) {
+      if (wave > 0) {
+#pragma unroll
+        for (int r = 0; r < PAGANI_NRULES; ++r)
+          part[((wave - 1) * PAGANI_NRULES + r) * 64 + lane] = rs[r];
+      }
+      __syncthreads();
+      if (wave == 0) {
+        for (int w = 1; w < slices; ++w) {
+#pragma unroll
+          for (int r = 0; r < PAGANI_NRULES; ++r)
+            rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
+        }
+      }
+      if (wave > 0) return;
+    }
   }
 
   // Null-rule error estimate, Sample.cuh:261-289.

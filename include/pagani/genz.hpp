@@ -158,6 +158,9 @@ struct Gaussian {
     for (int i = 1; i < D; ++i) s += v[i] * v[i];  // one FMA per axis (contracted)
     return math::exp_neg(-s);
   }
+  // additively separable in v: f = outer(sum_j term(j, v_j))
+  __device__ double term(int, double v) const { return v * v; }
+  __device__ double outer(double s) const { return math::exp_neg(-s); }
 };
 
 // exp(-sum a_i |x_i - u_i|)
@@ -174,16 +177,20 @@ struct C0 {
     for (int i = 0; i < D; ++i) s += a[i] * math::fabs_(x[i] - u[i]);
     return math::exp_neg(-s);
   }
+  // f(u + v / a): the kernel hands over v = a (x - u) (see Gaussian)
   __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  __device__ double axis_scale(int i) const { return static_cast<double>(a[i]); }
   template <class... T>
-  __device__ Real centered(T... ts) const
+  __device__ Real centered(T... vs) const
   {
-    const Real t[D] = {static_cast<Real>(ts)...};
-    Real s = a[0] * math::fabs_(t[0]);
+    const Real v[D] = {static_cast<Real>(vs)...};
+    Real s = math::fabs_(v[0]);
 #pragma unroll
-    for (int i = 1; i < D; ++i) s += a[i] * math::fabs_(t[i]);
+    for (int i = 1; i < D; ++i) s += math::fabs_(v[i]);
     return math::exp_neg(-s);
   }
+  __device__ double term(int, double v) const { return __builtin_fabs(v); }
+  __device__ double outer(double s) const { return math::exp_neg(-s); }
 };
 
 // 0 if any x_i > u_i, else exp(sum a_i x_i)
