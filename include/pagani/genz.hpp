@@ -75,6 +75,20 @@ struct Oscillatory {
     for (int i = 0; i < D; ++i) s += a[i] * x[i];
     return math::cos_(s);
   }
+  // v = a x (origin 0): f = cos(phase + sum v), additively separable
+  __device__ double origin(int) const { return 0.0; }
+  __device__ double axis_scale(int i) const { return static_cast<double>(a[i]); }
+  template <class... T>
+  __device__ Real centered(T... vs) const
+  {
+    const Real v[D] = {static_cast<Real>(vs)...};
+    Real s = phase;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += v[i];
+    return math::cos_(s);
+  }
+  __device__ double term(int, double v) const { return v; }
+  __device__ double outer(double s) const { return math::cos_(static_cast<double>(phase) + s); }
 };
 
 // prod 1 / (a_i^-2 + (x_i - u_i)^2), evaluated as one division
@@ -118,11 +132,29 @@ struct CornerPeak {
     Real s = 1;
 #pragma unroll
     for (int i = 0; i < D; ++i) s += a[i] * x[i];
+    return power(s);
+  }
+  static __device__ Real power(Real s)
+  {
     Real p = s;
 #pragma unroll
     for (int i = 0; i < D; ++i) p *= s;
     return Real(1) / p;
   }
+  // v = a x (origin 0): f = (1 + sum v)^-(d+1), additively separable
+  __device__ double origin(int) const { return 0.0; }
+  __device__ double axis_scale(int i) const { return static_cast<double>(a[i]); }
+  template <class... T>
+  __device__ Real centered(T... vs) const
+  {
+    const Real v[D] = {static_cast<Real>(vs)...};
+    Real s = 1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += v[i];
+    return power(s);
+  }
+  __device__ double term(int, double v) const { return v; }
+  __device__ double outer(double s) const { return CornerPeak<double, D>::power(1.0 + s); }
 };
 
 // exp(-sum a_i^2 (x_i - u_i)^2)
