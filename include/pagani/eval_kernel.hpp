@@ -295,7 +295,7 @@ __device__ __forceinline__ void eval_regions_wave(
   int bisect = maxdim;
 
   bool done = false;
-  if constexpr (is_separable<F>::value && std::is_same<Real, double>::value) {
+  if constexpr (is_separable<F>::value) {
     if (slices == 1) {
       separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
       done = true;

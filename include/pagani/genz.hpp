@@ -88,7 +88,10 @@ struct Oscillatory {
     return math::cos_(s);
   }
   __device__ double term(int, double v) const { return v; }
-  __device__ double outer(double s) const { return math::cos_(static_cast<double>(phase) + s); }
+  __device__ double outer(double s) const
+  {
+    return static_cast<double>(math::cos_(static_cast<Real>(static_cast<double>(phase) + s)));
+  }
 };
 
 // prod 1 / (a_i^-2 + (x_i - u_i)^2), evaluated as one division
@@ -154,7 +157,10 @@ struct CornerPeak {
     return power(s);
   }
   __device__ double term(int, double v) const { return v; }
-  __device__ double outer(double s) const { return CornerPeak<double, D>::power(1.0 + s); }
+  __device__ double outer(double s) const
+  {
+    return static_cast<double>(power(static_cast<Real>(1.0 + s)));
+  }
 };
 
 // exp(-sum a_i^2 (x_i - u_i)^2)
@@ -190,9 +196,13 @@ struct Gaussian {
     for (int i = 1; i < D; ++i) s += v[i] * v[i];  // one FMA per axis (contracted)
     return math::exp_neg(-s);
   }
-  // additively separable in v: f = outer(sum_j term(j, v_j))
+  // additively separable in v: f = outer(sum_j term(j, v_j)); the terms
+  // and their sums in FP64, outer in the integrand's precision
   __device__ double term(int, double v) const { return v * v; }
-  __device__ double outer(double s) const { return math::exp_neg(-s); }
+  __device__ double outer(double s) const
+  {
+    return static_cast<double>(math::exp_neg(static_cast<Real>(-s)));
+  }
 };
 
 // exp(-sum a_i |x_i - u_i|)
@@ -222,7 +232,10 @@ struct C0 {
     return math::exp_neg(-s);
   }
   __device__ double term(int, double v) const { return __builtin_fabs(v); }
-  __device__ double outer(double s) const { return math::exp_neg(-s); }
+  __device__ double outer(double s) const
+  {
+    return static_cast<double>(math::exp_neg(static_cast<Real>(-s)));
+  }
 };
 
 // 0 if any x_i > u_i, else exp(sum a_i x_i)
