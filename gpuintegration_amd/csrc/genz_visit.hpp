@@ -26,9 +26,12 @@ int visit_genz(const pagani_genz* g, Fn&& fn)
     }
     case PAGANI_GENZ_PRODUCT_PEAK: {
       pagani::ProductPeak<Real, D> f;
+      f.k2 = 1.0;
       for (int i = 0; i < D; ++i) {
         f.ia2[i] = static_cast<Real>(1.0 / (g->a[i] * g->a[i]));
         f.u[i] = static_cast<Real>(g->u[i]);
+        f.a[i] = static_cast<Real>(g->a[i]);
+        f.k2 *= g->a[i] * g->a[i];
       }
       return fn(f);
     }

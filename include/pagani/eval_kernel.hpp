@@ -81,18 +81,38 @@ template <class F>
 struct has_axis_scale<F, std::void_t<decltype(std::declval<const F&>().axis_scale(0))>>
     : std::true_type {};
 
-// A centred, scaled integrand that is additively separable in v --
-// f = outer(sum_j term(j, v_j)) -- may say so with `term` and `outer`.  The
-// kernel then evaluates the points of sets 1-7, which move at most three
-// coordinates off the centre, as outer(S0 + the changed axes' term
-// differences): the centre's terms and the differences at +-lambda1 and
-// +-lambda2 are computed once per region.
+// A centred, scaled integrand that is separable in v -- f = outer(sum_j
+// term(j, v_j)), or outer(prod_j term(j, v_j)) (is_multiplicative) -- may
+// say so with `term` and `outer`.  The kernel then evaluates the points of
+// sets 1-7, which move at most three coordinates off the centre, as outer
+// of the centre's combined terms joined with the changed axes' term
+// differences (ratios): the centre's terms and the changes at +-lambda1
+// and +-lambda2 are computed once per region.
 template <class F, class = void>
 struct is_separable : std::false_type {};
 template <class F>
 struct is_separable<F, std::void_t<decltype(std::declval<const F&>().term(0, 0.0)),
                                    decltype(std::declval<const F&>().outer(0.0))>>
     : std::bool_constant<has_origin<F>::value && has_axis_scale<F>::value> {};
+
+// Multiplicatively separable integrands (f = outer(prod_j term(j, v_j)))
+// say so with `static constexpr bool multiplicative = true`; the kernel then
+// combines the centre's terms and the changed axes' term ratios by products.
+template <class F, class = void>
+struct is_multiplicative : std::false_type {};
+template <class F>
+struct is_multiplicative<F, std::void_t<decltype(F::multiplicative)>>
+    : std::bool_constant<F::multiplicative> {};
+
+template <bool Mul>
+struct SepOp {
+  // combine the centre's value with a change; the change of term t0 -> t
+  static __device__ __forceinline__ double join(double a, double b) { return Mul ? a * b : a + b; }
+  static __device__ __forceinline__ double change(double t, double t0)
+  {
+    return Mul ? t / t0 : t - t0;
+  }
+};
 
 template <class F, class Real, int D, std::size_t... I>
 __device__ __forceinline__ double call_at(F& f, const Real (&x)[D],
@@ -137,20 +157,22 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
                                                double jac, const pagani_eval_consts& C,
                                                double (&rs)[PAGANI_NRULES], int& bisect)
 {
+  using Op = SepOp<is_multiplicative<F>::value>;
   double T0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
   double S0 = T0[0];
 #pragma unroll
-  for (int j = 1; j < D; ++j) S0 += T0[j];
-  // term differences at g = +lambda (x = c - lambda h) and g = -lambda
+  for (int j = 1; j < D; ++j) S0 = Op::join(S0, T0[j]);
+  // term changes (differences or ratios) at g = +lambda (x = c - lambda h)
+  // and g = -lambda
   double d1p[D], d1n[D], d2p[D], d2n[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    d1p[j] = f.term(j, __builtin_fma(-C.lambda1, hR[j], cR[j])) - T0[j];
-    d1n[j] = f.term(j, __builtin_fma(C.lambda1, hR[j], cR[j])) - T0[j];
-    d2p[j] = f.term(j, __builtin_fma(-C.lambda2, hR[j], cR[j])) - T0[j];
-    d2n[j] = f.term(j, __builtin_fma(C.lambda2, hR[j], cR[j])) - T0[j];
+    d1p[j] = Op::change(f.term(j, __builtin_fma(-C.lambda1, hR[j], cR[j])), T0[j]);
+    d1n[j] = Op::change(f.term(j, __builtin_fma(C.lambda1, hR[j], cR[j])), T0[j]);
+    d2p[j] = Op::change(f.term(j, __builtin_fma(-C.lambda2, hR[j], cR[j])), T0[j]);
+    d2n[j] = Op::change(f.term(j, __builtin_fma(C.lambda2, hR[j], cR[j])), T0[j]);
   }
   // sets 0-2 and the fourth difference, Sample.cuh:176-218
   const double f0 = f.outer(S0);
@@ -158,10 +180,10 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
   double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    const double fa = f.outer(S0 + d1p[j]);
-    const double fb = f.outer(S0 + d1n[j]);
-    const double fc = f.outer(S0 + d2p[j]);
-    const double fd = f.outer(S0 + d2n[j]);
+    const double fa = f.outer(Op::join(S0, d1p[j]));
+    const double fb = f.outer(Op::join(S0, d1n[j]));
+    const double fc = f.outer(Op::join(S0, d2p[j]));
+    const double fd = f.outer(Op::join(S0, d2n[j]));
     s1 += fa + fb;
     s2 += fc + fd;
     const double diff =
@@ -182,8 +204,8 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      acc += f.outer(S0 + (f.term(j, __builtin_fma(-lam, hR[j], cR[j])) - T0[j]));
-      acc += f.outer(S0 + (f.term(j, __builtin_fma(lam, hR[j], cR[j])) - T0[j]));
+      acc += f.outer(Op::join(S0, Op::change(f.term(j, __builtin_fma(-lam, hR[j], cR[j])), T0[j])));
+      acc += f.outer(Op::join(S0, Op::change(f.term(j, __builtin_fma(lam, hR[j], cR[j])), T0[j])));
     }
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
@@ -193,13 +215,13 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double acc = 0.0;
 #pragma unroll 1
     for (int j = 0; j < D; ++j) {
-      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
 #pragma unroll 1
       for (int k = j + 1; k < D; ++k) {
-        acc += f.outer(aj[0] + d1p[k]);
-        acc += f.outer(aj[1] + d1p[k]);
-        acc += f.outer(aj[0] + d1n[k]);
-        acc += f.outer(aj[1] + d1n[k]);
+        acc += f.outer(Op::join(aj[0], d1p[k]));
+        acc += f.outer(Op::join(aj[1], d1p[k]));
+        acc += f.outer(Op::join(aj[0], d1n[k]));
+        acc += f.outer(Op::join(aj[1], d1n[k]));
       }
     }
 #pragma unroll
@@ -210,14 +232,14 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double acc = 0.0;
 #pragma unroll 1
     for (int j = 0; j < D; ++j) {
-      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
 #pragma unroll 1
       for (int k = 0; k < D; ++k) {
         if (k == j) continue;
-        acc += f.outer(aj[0] + d2p[k]);
-        acc += f.outer(aj[1] + d2p[k]);
-        acc += f.outer(aj[0] + d2n[k]);
-        acc += f.outer(aj[1] + d2n[k]);
+        acc += f.outer(Op::join(aj[0], d2p[k]));
+        acc += f.outer(Op::join(aj[1], d2p[k]));
+        acc += f.outer(Op::join(aj[0], d2n[k]));
+        acc += f.outer(Op::join(aj[1], d2n[k]));
       }
     }
 #pragma unroll
@@ -228,16 +250,17 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double acc = 0.0;
 #pragma unroll 1
     for (int j = 0; j < D; ++j) {
-      const double aj[2] = {S0 + d1p[j], S0 + d1n[j]};
+      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
 #pragma unroll 1
       for (int k = j + 1; k < D; ++k) {
-        const double ajk[4] = {aj[0] + d1p[k], aj[1] + d1p[k], aj[0] + d1n[k], aj[1] + d1n[k]};
+        const double ajk[4] = {Op::join(aj[0], d1p[k]), Op::join(aj[1], d1p[k]),
+                               Op::join(aj[0], d1n[k]), Op::join(aj[1], d1n[k])};
 #pragma unroll 1
         for (int l = k + 1; l < D; ++l) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc += f.outer(ajk[q] + d1p[l]);
+          for (int q = 0; q < 4; ++q) acc += f.outer(Op::join(ajk[q], d1p[l]));
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc += f.outer(ajk[q] + d1n[l]);
+          for (int q = 0; q < 4; ++q) acc += f.outer(Op::join(ajk[q], d1n[l]));
         }
       }
     }

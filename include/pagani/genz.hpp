@@ -99,6 +99,8 @@ template <class Real, int D>
 struct ProductPeak {
   Real ia2[D];  // 1 / a_i^2
   Real u[D];
+  Real a[D];
+  double k2;    // prod a_i^2
   template <class... X>
   __device__ Real operator()(X... xs) const
   {
@@ -111,17 +113,22 @@ struct ProductPeak {
     }
     return Real(1) / den;
   }
-  // f(u + t): the evaluation kernel hands over t = x - u directly
+  // f(u + v / a) = prod a_i^2 / (1 + v_i^2): the kernel hands over
+  // v = a (x - u); multiplicatively separable, term 1 / (1 + v^2)
   __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  __device__ double axis_scale(int i) const { return static_cast<double>(a[i]); }
   template <class... T>
-  __device__ Real centered(T... ts) const
+  __device__ Real centered(T... vs) const
   {
-    const Real t[D] = {static_cast<Real>(ts)...};
+    const Real v[D] = {static_cast<Real>(vs)...};
     Real den = 1;
 #pragma unroll
-    for (int i = 0; i < D; ++i) den *= ia2[i] + t[i] * t[i];
-    return Real(1) / den;
+    for (int i = 0; i < D; ++i) den *= Real(1) + v[i] * v[i];
+    return static_cast<Real>(k2) / den;
   }
+  static constexpr bool multiplicative = true;
+  __device__ double term(int, double v) const { return 1.0 / (1.0 + v * v); }
+  __device__ double outer(double p) const { return static_cast<double>(static_cast<Real>(k2 * p)); }
 };
 
 // (1 + sum a_i x_i)^-(d+1), the power by repeated multiplication
