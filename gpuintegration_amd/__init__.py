@@ -94,6 +94,7 @@ class EvalConsts(ctypes.Structure):
         ("slice_begin", ctypes.c_int * (MAX_SLICES + 1)),
         ("lambda3", ctypes.c_double),
         ("lambda4", ctypes.c_double),
+        ("lambda5", ctypes.c_double),
     ]
 
 

@@ -71,6 +71,7 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
   C.lambda2 = R.lambda[1];
   C.lambda3 = R.lambda[2];
   C.lambda4 = R.lambda[3];
+  C.lambda5 = R.lambda[4];
   const double q = R.lambda[1] / R.lambda[0];
   C.ratio = q * q;
   C.gtab = ws->d_gtab;

@@ -98,7 +98,7 @@ typedef struct pagani_eval_consts {
   int set_begin[PAGANI_NSETS + 1]; /* first point index of each set       */
   int slices;                      /* waves sharing one batch of 64 regions */
   int slice_begin[PAGANI_MAX_SLICES + 1]; /* generic-point slice bounds     */
-  double lambda3, lambda4;         /* generator magnitudes of sets 3 and 4 */
+  double lambda3, lambda4, lambda5; /* generator magnitudes of sets 3, 4, 8 */
 } pagani_eval_consts;
 
 /* Iteration descriptor of the device-resident loop, in device memory: the

@@ -104,6 +104,11 @@ template <class F>
 struct is_multiplicative<F, std::void_t<decltype(F::multiplicative)>>
     : std::bool_constant<F::multiplicative> {};
 
+template <class F, class = void>
+struct heavy_outer : std::false_type {};
+template <class F>
+struct heavy_outer<F, std::void_t<decltype(F::heavy_outer)>> : std::bool_constant<F::heavy_outer> {};
+
 template <bool Mul>
 struct SepOp {
   // combine the centre's value with a change; the change of term t0 -> t
@@ -267,12 +272,52 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
   }
-  // set 8: all coordinates +-lambda5, from the table
-  {
+  // set 8: the 2^D corners x = c - g h, g_j = +lambda5 where bit j of the
+  // corner index is 0 and -lambda5 where it is 1 (the table's order).  The
+  // low LB axes' combined terms are tabulated once; each group of 2^LB
+  // corners adds the high axes' combination.  An integrand whose outer
+  // function is long (heavy_outer) takes the table loop instead: unrolled
+  // 2^LB times, its outer would not fit the registers.
+  if constexpr (heavy_outer<F>::value) {
     const int p0 = C.set_begin[3];
     const double acc = set_sum<D, F, double>(
         f, C.gtab + static_cast<std::size_t>(C.set_begin[8] - p0) * D, C.set_begin[8],
         C.set_begin[9], cR, hR);
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+  } else {
+    constexpr int LB = D < 4 ? D : 4;
+    constexpr int HB = D - LB;
+    double tp[D], tn[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      tp[j] = f.term(j, __builtin_fma(-C.lambda5, hR[j], cR[j]));
+      tn[j] = f.term(j, __builtin_fma(C.lambda5, hR[j], cR[j]));
+    }
+    double lo[1 << LB];
+    lo[0] = tp[0];
+    lo[1] = tn[0];
+#pragma unroll
+    for (int j = 1; j < LB; ++j) {
+#pragma unroll
+      for (int q = 0; q < (1 << j); ++q) {
+        lo[q + (1 << j)] = Op::join(lo[q], tn[j]);
+        lo[q] = Op::join(lo[q], tp[j]);
+      }
+    }
+    double acc = 0.0;
+#pragma unroll 1
+    for (int hq = 0; hq < (1 << HB); ++hq) {
+      double hs = 0.0;
+      if constexpr (HB > 0) {
+        hs = (hq & 1) ? tn[LB] : tp[LB];
+#pragma unroll
+        for (int j = LB + 1; j < D; ++j) hs = Op::join(hs, ((hq >> (j - LB)) & 1) ? tn[j] : tp[j]);
+      }
+#pragma unroll
+      for (int q = 0; q < (1 << LB); ++q)
+        acc += f.outer(HB > 0 ? Op::join(lo[q], hs) : lo[q]);
+    }
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
   }

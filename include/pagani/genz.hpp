@@ -87,6 +87,7 @@ struct Oscillatory {
     for (int i = 0; i < D; ++i) s += v[i];
     return math::cos_(s);
   }
+  static constexpr bool heavy_outer = true;  // cos: argument reduction and two polynomials
   __device__ double term(int, double v) const { return v; }
   __device__ double outer(double s) const
   {

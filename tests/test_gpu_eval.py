@@ -117,11 +117,17 @@ def check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, cond=
 
 @pytest.mark.parametrize("family", FAMILIES)
 @pytest.mark.parametrize("d", [2, 3, 5, 8])
-def test_eval_random_regions_fp64(pg, gpu, family, d):
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_random_regions_fp64(pg, gpu, family, d, slices):
+    """slices 0: the library's choice (several waves per batch at these
+    sizes: the table-driven point loop); 1: one wave per batch, where the
+    separable families take their term-difference evaluation."""
     n = 3000 if d < 8 else 1000
     a, u = genz_params(family, d, seed=100 + d)
     L, H = random_regions(d, n, seed=7 * d)
     with pg.Workspace(d, 0) as ws:
+        if slices:
+            ws.set_slices(slices)
         ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n)
     prm = O.params(a, u)
     oe, oerr, odim, fd, rs = O.eval_regions(d, family, prm, L, H, n, want_extra=True)
@@ -185,19 +191,25 @@ def test_eval_general_volume(pg, gpu):
     L, H = random_regions(d, n, seed=19)
     with pg.Workspace(d, 0) as ws:
         ge, gerr, gdim = gpu_eval(ws, gpu, "c0", a, u, L, H, n, lows=lows, highs=highs)
+        ws.set_slices(1)
+        ge1, gerr1, gdim1 = gpu_eval(ws, gpu, "c0", a, u, L, H, n, lows=lows, highs=highs)
     oe, oerr, odim, fd, rs = O.eval_regions(d, "c0", O.params(a, u), L, H, n,
                                             lows=np.array(lows), highs=np.array(highs),
                                             want_extra=True)
     check_region_parity(d, "c0", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+    check_region_parity(d, "c0", ge1, gerr1, gdim1, oe, oerr, odim, fd, rs)
 
 
 @pytest.mark.parametrize("family", FAMILIES)
-def test_eval_fp32_integrand(pg, gpu, family):
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_fp32_integrand(pg, gpu, family, slices):
     """Mixed precision: FP32 integrand, FP64 sums, vs the all-FP64 oracle."""
     d, n = 6, 2000
     a, u = genz_params(family, d, seed=31)
     L, H = random_regions(d, n, seed=37)
     with pg.Workspace(d, 0) as ws:
+        if slices:
+            ws.set_slices(slices)
         ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n, precision=32)
     oe, oerr, odim, fd, rs = O.eval_regions(d, family, O.params(a, u), L, H, n,
                                             want_extra=True)
