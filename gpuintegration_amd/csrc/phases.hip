@@ -699,6 +699,26 @@ __device__ __forceinline__ void compact_chunk(
   for (int k = 0; k < static_cast<int>(kChunk / kCompactThreads); ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
     const bool a = t < n && active[t] == 1;
+    // the split's reads do not wait for the rank: a region's bounds are
+    // requested before the workgroup's scan (whole cache lines are read for
+    // a partly active wave in any case)
+    double lv[PAGANI_MAX_DIM], hv[PAGANI_MAX_DIM];
+    double ev = 0.0, rv = 0.0;
+    int b = 0;
+    if (a) {
+      if (pest) ev = est[t];
+      if (perr) rv = err[t];
+    }
+    if (split && a) {
+      b = dim[t];
+#pragma unroll
+      for (int j = 0; j < PAGANI_MAX_DIM; ++j) {
+        if (j < d) {
+          lv[j] = L[static_cast<std::size_t>(j) * n + t];
+          hv[j] = H[static_cast<std::size_t>(j) * n + t];
+        }
+      }
+    }
     std::uint32_t rank;
     const std::uint32_t tot = chunk_rank<kCompactThreads>(a, rank, wsum);
     if (a) {
@@ -706,10 +726,11 @@ __device__ __forceinline__ void compact_chunk(
       if (idx < m) {
         if (split) {
           const std::size_t n2 = 2 * m;
-          const int b = dim[t];
-          for (int j = 0; j < d; ++j) {
-            const double l = L[static_cast<std::size_t>(j) * n + t];
-            const double h = H[static_cast<std::size_t>(j) * n + t];
+#pragma unroll
+          for (int j = 0; j < PAGANI_MAX_DIM; ++j) {
+            if (j >= d) break;
+            const double l = lv[j];
+            const double h = hv[j];
             const std::size_t o = static_cast<std::size_t>(j) * n2 + idx;
             if (j == b) {
               const double half = h / 2;
@@ -731,8 +752,8 @@ __device__ __forceinline__ void compact_chunk(
           }
           if (dim2) dim2[idx] = dim[t];
         }
-        if (pest) pest[idx] = est[t];
-        if (perr) perr[idx] = err[t];
+        if (pest) pest[idx] = ev;
+        if (perr) perr[idx] = rv;
       }
     }
     run += tot;
