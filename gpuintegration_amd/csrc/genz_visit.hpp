@@ -59,9 +59,11 @@ int visit_genz(const pagani_genz* g, Fn&& fn)
     }
     case PAGANI_GENZ_DISCONTINUOUS: {
       pagani::Discontinuous<Real, D> f;
+      f.c = 0.0;
       for (int i = 0; i < D; ++i) {
         f.a[i] = static_cast<Real>(g->a[i]);
         f.u[i] = static_cast<Real>(g->u[i]);
+        f.c += g->a[i] * g->u[i];
       }
       return fn(f);
     }

@@ -88,12 +88,22 @@ struct has_axis_scale<F, std::void_t<decltype(std::declval<const F&>().axis_scal
 // of the centre's combined terms joined with the changed axes' term
 // differences (ratios): the centre's terms and the changes at +-lambda1
 // and +-lambda2 are computed once per region.
+// `static constexpr bool separable = false` turns the path off where the
+// generic loop is faster (an instantiation whose separable form would not
+// fit the registers).
+template <class F, class = void>
+struct separable_enabled : std::true_type {};
+template <class F>
+struct separable_enabled<F, std::void_t<decltype(F::separable)>>
+    : std::bool_constant<F::separable> {};
+
 template <class F, class = void>
 struct is_separable : std::false_type {};
 template <class F>
 struct is_separable<F, std::void_t<decltype(std::declval<const F&>().term(0, 0.0)),
                                    decltype(std::declval<const F&>().outer(0.0))>>
-    : std::bool_constant<has_origin<F>::value && has_axis_scale<F>::value> {};
+    : std::bool_constant<has_origin<F>::value && has_axis_scale<F>::value &&
+                         separable_enabled<F>::value> {};
 
 // Multiplicatively separable integrands (f = outer(prod_j term(j, v_j)))
 // say so with `static constexpr bool multiplicative = true`; the kernel then
@@ -108,6 +118,15 @@ template <class F, class = void>
 struct heavy_outer : std::false_type {};
 template <class F>
 struct heavy_outer<F, std::void_t<decltype(F::heavy_outer)>> : std::bool_constant<F::heavy_outer> {};
+
+// Masked integrands (`static constexpr bool masked = true`, `bool valid(j,
+// v)`): f is zero unless every axis' coordinate is valid; `term` ignores
+// validity.  The kernel tracks the axes invalid at the centre and gives a
+// changed coordinate that is invalid the term change -infinity.
+template <class F, class = void>
+struct is_masked : std::false_type {};
+template <class F>
+struct is_masked<F, std::void_t<decltype(F::masked)>> : std::bool_constant<F::masked> {};
 
 template <bool Mul>
 struct SepOp {
@@ -163,9 +182,29 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
                                                double (&rs)[PAGANI_NRULES], int& bisect)
 {
   using Op = SepOp<is_multiplicative<F>::value>;
+  constexpr bool kMasked = is_masked<F>::value;
   double T0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
+  // axes invalid at the centre: a point is zero unless it changes them all
+  std::uint32_t vmask = 0;
+  if constexpr (kMasked) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (!f.valid(j, cR[j])) vmask |= 1u << j;
+  }
+  // the change of axis j's term when its coordinate moves to v
+  auto change = [&](int j, double v) {
+    if constexpr (kMasked)
+      if (!f.valid(j, v)) return -__builtin_inf();
+    return Op::change(f.term(j, v), T0[j]);
+  };
+  // outer, zero when an unchanged axis is invalid at the centre
+  auto gated = [&](std::uint32_t changed, double s) {
+    if constexpr (kMasked)
+      if (vmask & ~changed) return 0.0;
+    return f.outer(s);
+  };
   double S0 = T0[0];
 #pragma unroll
   for (int j = 1; j < D; ++j) S0 = Op::join(S0, T0[j]);
@@ -174,21 +213,21 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
   double d1p[D], d1n[D], d2p[D], d2n[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    d1p[j] = Op::change(f.term(j, __builtin_fma(-C.lambda1, hR[j], cR[j])), T0[j]);
-    d1n[j] = Op::change(f.term(j, __builtin_fma(C.lambda1, hR[j], cR[j])), T0[j]);
-    d2p[j] = Op::change(f.term(j, __builtin_fma(-C.lambda2, hR[j], cR[j])), T0[j]);
-    d2n[j] = Op::change(f.term(j, __builtin_fma(C.lambda2, hR[j], cR[j])), T0[j]);
+    d1p[j] = change(j, __builtin_fma(-C.lambda1, hR[j], cR[j]));
+    d1n[j] = change(j, __builtin_fma(C.lambda1, hR[j], cR[j]));
+    d2p[j] = change(j, __builtin_fma(-C.lambda2, hR[j], cR[j]));
+    d2n[j] = change(j, __builtin_fma(C.lambda2, hR[j], cR[j]));
   }
   // sets 0-2 and the fourth difference, Sample.cuh:176-218
-  const double f0 = f.outer(S0);
+  const double f0 = gated(0u, S0);
   const double base = (f0 * jac) * 2 * (1 - C.ratio);
   double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    const double fa = f.outer(Op::join(S0, d1p[j]));
-    const double fb = f.outer(Op::join(S0, d1n[j]));
-    const double fc = f.outer(Op::join(S0, d2p[j]));
-    const double fd = f.outer(Op::join(S0, d2n[j]));
+    const double fa = gated(1u << j, Op::join(S0, d1p[j]));
+    const double fb = gated(1u << j, Op::join(S0, d1n[j]));
+    const double fc = gated(1u << j, Op::join(S0, d2p[j]));
+    const double fd = gated(1u << j, Op::join(S0, d2n[j]));
     s1 += fa + fb;
     s2 += fc + fd;
     const double diff =
@@ -209,8 +248,8 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double acc = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      acc += f.outer(Op::join(S0, Op::change(f.term(j, __builtin_fma(-lam, hR[j], cR[j])), T0[j])));
-      acc += f.outer(Op::join(S0, Op::change(f.term(j, __builtin_fma(lam, hR[j], cR[j])), T0[j])));
+      acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(-lam, hR[j], cR[j]))));
+      acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(lam, hR[j], cR[j]))));
     }
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
@@ -223,10 +262,11 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
       const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
 #pragma unroll 1
       for (int k = j + 1; k < D; ++k) {
-        acc += f.outer(Op::join(aj[0], d1p[k]));
-        acc += f.outer(Op::join(aj[1], d1p[k]));
-        acc += f.outer(Op::join(aj[0], d1n[k]));
-        acc += f.outer(Op::join(aj[1], d1n[k]));
+        const std::uint32_t ch = (1u << j) | (1u << k);
+        acc += gated(ch, Op::join(aj[0], d1p[k]));
+        acc += gated(ch, Op::join(aj[1], d1p[k]));
+        acc += gated(ch, Op::join(aj[0], d1n[k]));
+        acc += gated(ch, Op::join(aj[1], d1n[k]));
       }
     }
 #pragma unroll
@@ -241,10 +281,11 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
 #pragma unroll 1
       for (int k = 0; k < D; ++k) {
         if (k == j) continue;
-        acc += f.outer(Op::join(aj[0], d2p[k]));
-        acc += f.outer(Op::join(aj[1], d2p[k]));
-        acc += f.outer(Op::join(aj[0], d2n[k]));
-        acc += f.outer(Op::join(aj[1], d2n[k]));
+        const std::uint32_t ch = (1u << j) | (1u << k);
+        acc += gated(ch, Op::join(aj[0], d2p[k]));
+        acc += gated(ch, Op::join(aj[1], d2p[k]));
+        acc += gated(ch, Op::join(aj[0], d2n[k]));
+        acc += gated(ch, Op::join(aj[1], d2n[k]));
       }
     }
 #pragma unroll
@@ -262,10 +303,11 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
                                Op::join(aj[0], d1n[k]), Op::join(aj[1], d1n[k])};
 #pragma unroll 1
         for (int l = k + 1; l < D; ++l) {
+          const std::uint32_t ch = (1u << j) | (1u << k) | (1u << l);
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc += f.outer(Op::join(ajk[q], d1p[l]));
+          for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1p[l]));
 #pragma unroll
-          for (int q = 0; q < 4; ++q) acc += f.outer(Op::join(ajk[q], d1n[l]));
+          for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1n[l]));
         }
       }
     }
@@ -291,8 +333,15 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
     double tp[D], tn[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
-      tp[j] = f.term(j, __builtin_fma(-C.lambda5, hR[j], cR[j]));
-      tn[j] = f.term(j, __builtin_fma(C.lambda5, hR[j], cR[j]));
+      // the corners change every axis: plain terms, -infinity where invalid
+      const double vp = __builtin_fma(-C.lambda5, hR[j], cR[j]);
+      const double vn = __builtin_fma(C.lambda5, hR[j], cR[j]);
+      tp[j] = f.term(j, vp);
+      tn[j] = f.term(j, vn);
+      if constexpr (kMasked) {
+        if (!f.valid(j, vp)) tp[j] = -__builtin_inf();
+        if (!f.valid(j, vn)) tn[j] = -__builtin_inf();
+      }
     }
     double lo[1 << LB];
     lo[0] = tp[0];

@@ -21,17 +21,18 @@
 namespace pagani {
 namespace math {
 
-// e^x for x <= 0, the argument range of the Gaussian and C0 families (their
-// exponent is minus a sum of non-negative terms).  x = n ln2 + r with
-// n = rint(x / ln2) and |r| <= ln2/2, r from a two-part ln2 (Cody-Waite);
-// e^r by a degree-11 polynomial; 2^n by ldexp, which also rounds
-// subnormal results and gives 0 below about -745.1.  The coefficients
-// interpolate e^r at the 12 Chebyshev nodes of [-ln2/2, ln2/2] (computed in
-// extended precision, then rounded to double; the check that recomputes
-// them is in the tests): max polynomial error 0.14 ulp.  With no overflow
-// to guard against, the range handling is one clamp, which also maps -inf
-// to 0 (and NaN to 0: the families never produce NaN from finite input).
-__device__ __forceinline__ double exp_neg(double x)
+// e^x for x <= 709 (the exponents of the Gaussian and C0 families are minus
+// sums of non-negative terms, the discontinuous family's are bounded by
+// sum a_i).  x = n ln2 + r with n = rint(x / ln2) and |r| <= ln2/2, r from a
+// two-part ln2 (Cody-Waite); e^r by a degree-11 polynomial; 2^n by ldexp,
+// which also rounds subnormal results, gives 0 below about -745.1 and
+// infinity above about 709.78.  The coefficients interpolate e^r at the 12
+// Chebyshev nodes of [-ln2/2, ln2/2] (computed in extended precision, then
+// rounded to double; the check that recomputes them is in the tests): max
+// polynomial error 0.14 ulp.  The range handling is one clamp, which also
+// maps -inf to 0 (and NaN to 0: the families never produce NaN from finite
+// input).
+__device__ __forceinline__ double exp_bounded(double x)
 {
   constexpr double kLog2e = 0x1.71547652b82fep+0;
   constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
@@ -49,7 +50,9 @@ __device__ __forceinline__ double exp_neg(double x)
   for (int k = 10; k >= 0; --k) p = __builtin_fma(p, r, c[k]);
   return __builtin_ldexp(p, static_cast<int>(n));
 }
+__device__ __forceinline__ double exp_neg(double x) { return exp_bounded(x); }
 __device__ __forceinline__ float exp_neg(float x) { return ::expf(x); }
+__device__ __forceinline__ float exp_bounded(float x) { return ::expf(x); }
 
 __device__ __forceinline__ double exp_(double x) { return ::exp(x); }
 __device__ __forceinline__ float exp_(float x) { return ::expf(x); }
@@ -251,6 +254,7 @@ template <class Real, int D>
 struct Discontinuous {
   Real a[D];
   Real u[D];
+  double c;  // sum a_i u_i
   template <class... X>
   __device__ Real operator()(X... xs) const
   {
@@ -266,6 +270,33 @@ struct Discontinuous {
     }
     const Real v = math::exp_(s);
     return m > Real(0) ? Real(0) : v;
+  }
+  // t = x - u (origin u, unit scale): f = [all t_i <= 0] exp(c + sum a_i t_i),
+  // additively separable with a validity mask per axis
+  __device__ double origin(int i) const { return static_cast<double>(u[i]); }
+  __device__ double axis_scale(int) const { return 1.0; }
+  template <class... T>
+  __device__ Real centered(T... ts) const
+  {
+    const Real t[D] = {static_cast<Real>(ts)...};
+    Real m = t[0];
+    Real s = a[0] * t[0];
+#pragma unroll
+    for (int i = 1; i < D; ++i) {
+      m = math::fmax_(m, t[i]);
+      s += a[i] * t[i];
+    }
+    const Real v = math::exp_bounded(static_cast<Real>(c) + s);
+    return m > Real(0) ? Real(0) : v;
+  }
+  static constexpr bool masked = true;
+  // at 8D in double the masked separable form spills; the generic loop wins
+  static constexpr bool separable = D < 8 || sizeof(Real) == 4;
+  __device__ bool valid(int, double t) const { return t <= 0.0; }
+  __device__ double term(int i, double t) const { return static_cast<double>(a[i]) * t; }
+  __device__ double outer(double s) const
+  {
+    return static_cast<double>(math::exp_bounded(static_cast<Real>(c + s)));
   }
 };
 

@@ -1,5 +1,6 @@
-"""The exponential used by the Gaussian and C0 integrands for their
-non-positive arguments (pagani::math::exp_neg in include/pagani/genz.hpp).
+"""The exponential used by the Gaussian, C0 and discontinuous integrands
+(pagani::math::exp_bounded in include/pagani/genz.hpp, valid for arguments
+up to 709; exp_neg is the same function for non-positive arguments).
 
 CPU: its constants are recomputed from their definition (ln2 split into two
 doubles; e^r interpolated at the 12 Chebyshev nodes of [-ln2/2, ln2/2] in
@@ -26,7 +27,7 @@ LD = np.longdouble
 
 def header_constants():
     text = open(HEADER).read()
-    body = text[text.index("double exp_neg(double x)"):]
+    body = text[text.index("double exp_bounded(double x)"):]
     body = body[:body.index("__builtin_ldexp")]
     named = dict(re.findall(r"k(\w+) = (0x[0-9a-fp.+-]+);", body))
     coeffs = re.findall(r"(0x[0-9a-f.]+p[+-]\d+)", body[body.index("c[12]"):])
@@ -107,6 +108,10 @@ def test_device_exp_neg_accuracy(pg, gpu):
         -(np.arange(1, 1100) * math.log(2) / 2),            # reduction boundaries
         np.array([0.0, -0.0, -5e-324, -1e-300, -708.39, -708.4, -745.13, -745.14, -800.0,
                   -1e4, -1e10]),
+        # the positive side the discontinuous family uses (exponents up to sum a_i)
+        rng.uniform(0, 709.7, 50000),
+        np.arange(1, 1024) * math.log(2) / 2,
+        np.array([1e-300, 5e-324, 1.0, 100.0, 709.0, 709.78]),
     ]).astype(np.float64)
     n = len(x)
     mine = np.zeros(n)

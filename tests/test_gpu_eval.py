@@ -92,6 +92,9 @@ def centred_cond(family, a):
         return 12 * EPS * sum(v * v for v in a)
     if family in ("c0", "product_peak"):
         return 4 * EPS * sum(abs(v) for v in a)
+    if family == "discontinuous":
+        # exponent formed as sum a u + sum a (x - u): 2 eps sum a_j (|u_j| + |x_j|)
+        return 8 * EPS * sum(abs(v) for v in a)
     return 0.0
 
 
