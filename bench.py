@@ -42,15 +42,16 @@ EVAL_HBM_BYTES_PER_REGION_8D = 146.13
 # FP64 work the 8D Gaussian evaluation kernel executes per region, from one
 # rocprofv3 counter pass over `bench.py --eval-only` (SQ_INSTS_VALU_FMA_F64,
 # _MUL_F64, _ADD_F64 and SQ_INSTS_VALU, one lane per region, every lane
-# active): 14643 FMA + 1222 MUL + 2426 ADD = 32934 FLOP and 23884 VALU
+# active): 1366 FMA + 233 MUL + 128 ADD = 3093 FLOP and 2216 VALU
 # instructions.  The algorithmic count (90770 FLOP) charges every one of the
-# 1105 points a full d-axis evaluation; the kernel evaluates the Gaussian
-# (a sum over axes in its exponent) from per-region terms -- the points that
-# move at most three coordinates off the centre from term differences, the
-# 256 corners from tabulated partial sums -- so the algorithmic rate would
-# exceed the FP64 peak: the roofline uses the executed FLOPs.
-EVAL_EXEC_FLOP_PER_REGION_8D = 32934
-EVAL_VALU_PER_REGION_8D = 23884
+# 1105 points a full d-axis evaluation; the kernel evaluates the Gaussian as
+# a product over axes -- the centre's value times one factor e^-(change of
+# a_j^2 (x_j - u_j)^2) per changed axis, 81 exponentials per region instead
+# of 1105, and each set's sum over its sign combinations factored per axis
+# -- so the algorithmic rate would exceed the FP64 peak: the roofline uses
+# the executed FLOPs.
+EVAL_EXEC_FLOP_PER_REGION_8D = 3093
+EVAL_VALU_PER_REGION_8D = 2216
 VALU_ISSUE_PEAK = 256 * 2.4e9  # wave-instructions per second (one per CU per clock)
 
 

@@ -128,6 +128,28 @@ struct is_masked : std::false_type {};
 template <class F>
 struct is_masked<F, std::void_t<decltype(F::masked)>> : std::bool_constant<F::masked> {};
 
+// Exponential separable integrands (`static constexpr bool exponential =
+// true`: outer(s + d) = outer(s) * factor(d), with `double factor(double d)`
+// and `bool factor_safe(double s0)`, true when every factor and product met
+// in a region whose centre sums to s0 stays in the normal range) are
+// evaluated as products: the centre's value outer(S0) times one factor per
+// changed axis, so a point of sets 5-8 costs a multiply instead of an outer
+// evaluation.  Regions that fail factor_safe take the generic loop.
+template <class F, class = void>
+struct is_exponential : std::false_type {};
+template <class F>
+struct is_exponential<F, std::void_t<decltype(F::exponential)>>
+    : std::bool_constant<F::exponential> {};
+
+// A multiplicative integrand whose outer is linear (outer(p) = k p, up to
+// the rounding of its result) says so with `static constexpr bool
+// linear_outer = true`.
+template <class F, class = void>
+struct has_linear_outer : std::false_type {};
+template <class F>
+struct has_linear_outer<F, std::void_t<decltype(F::linear_outer)>>
+    : std::bool_constant<F::linear_outer> {};
+
 template <bool Mul>
 struct SepOp {
   // combine the centre's value with a change; the change of term t0 -> t
@@ -176,13 +198,24 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
 // (slices == 1: this wave evaluates all points).  Points are visited in the
 // generic order of every set, and each set's values are summed in that
 // order, as set_sum does; a point x = c - g h changes v_j by -g h_j.
+// Returns false, having done nothing, for an exponential integrand whose
+// region fails factor_safe.
 template <int D, class F>
-__device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+__device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], const double (&hR)[D],
                                                double jac, const pagani_eval_consts& C,
                                                double (&rs)[PAGANI_NRULES], int& bisect)
 {
-  using Op = SepOp<is_multiplicative<F>::value>;
+  constexpr bool kExp = is_exponential<F>::value;
+  constexpr bool kMul = is_multiplicative<F>::value || kExp;
+  using Op = SepOp<kMul>;
   constexpr bool kMasked = is_masked<F>::value;
+  static_assert(!(kMasked && kMul), "masks are supported for additive terms only");
+  // A point's value is the centre's value times the changed axes' factors
+  // (exponential, or multiplicative with a linear outer): a set's sum over
+  // every sign combination on a set of axes then factors per axis, e.g.
+  // sum over the 4 points (+-l1, +-l1) on axes j, k = S0 s_j s_k with
+  // s_j = r(+l1)_j + r(-l1)_j, and the 2^d corners sum to S0 prod_j s5_j.
+  constexpr bool kFactor = kExp || (kMul && has_linear_outer<F>::value);
   double T0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
@@ -194,20 +227,29 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
       if (!f.valid(j, cR[j])) vmask |= 1u << j;
   }
   // the change of axis j's term when its coordinate moves to v
+  // (exponential: the factor of the value)
   auto change = [&](int j, double v) {
     if constexpr (kMasked)
       if (!f.valid(j, v)) return -__builtin_inf();
+    if constexpr (kExp) return f.factor(f.term(j, v) - T0[j]);
     return Op::change(f.term(j, v), T0[j]);
   };
-  // outer, zero when an unchanged axis is invalid at the centre
+  // a point's value from its joined terms (exponential: the value itself);
+  // zero when an unchanged axis is invalid at the centre
   auto gated = [&](std::uint32_t changed, double s) {
     if constexpr (kMasked)
       if (vmask & ~changed) return 0.0;
+    if constexpr (kExp) return s;
     return f.outer(s);
   };
+  // the centre's joined terms (exponential: the centre's value)
   double S0 = T0[0];
 #pragma unroll
-  for (int j = 1; j < D; ++j) S0 = Op::join(S0, T0[j]);
+  for (int j = 1; j < D; ++j) S0 = kExp ? S0 + T0[j] : Op::join(S0, T0[j]);
+  if constexpr (kExp) {
+    if (!f.factor_safe(S0)) return false;
+    S0 = f.outer(S0);
+  }
   // term changes (differences or ratios) at g = +lambda (x = c - lambda h)
   // and g = -lambda
   double d1p[D], d1n[D], d2p[D], d2n[D];
@@ -246,73 +288,107 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
   for (int s = 3; s <= 4; ++s) {
     const double lam = lam34[s - 3];
     double acc = 0.0;
+    if constexpr (kFactor) {
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-      acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(-lam, hR[j], cR[j]))));
-      acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(lam, hR[j], cR[j]))));
+      for (int j = 0; j < D; ++j)
+        acc += change(j, __builtin_fma(-lam, hR[j], cR[j])) +
+               change(j, __builtin_fma(lam, hR[j], cR[j]));
+      acc = gated(0u, S0 * acc);
+    } else {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(-lam, hR[j], cR[j]))));
+        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(lam, hR[j], cR[j]))));
+      }
     }
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
   }
-  // set 5: (+-lambda1, +-lambda1) on axes j < k
-  {
-    double acc = 0.0;
-#pragma unroll 1
+  if constexpr (kFactor) {
+    // sets 5-7 from the per-axis sums s1_j, s2_j of the +-lambda1 and
+    // +-lambda2 factors: set 5 sums S0 s1_j s1_k over j < k, set 6 S0 s1_j
+    // s2_k over j != k, set 7 S0 s1_j s1_k s1_l over j < k < l.  Every sum
+    // below adds non-negative products when the factors are positive.
+    double e1 = 0.0, e2 = 0.0, e3 = 0.0, c12 = 0.0, pre1 = 0.0, pre2 = 0.0;
+#pragma unroll
     for (int j = 0; j < D; ++j) {
-      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-#pragma unroll 1
-      for (int k = j + 1; k < D; ++k) {
-        const std::uint32_t ch = (1u << j) | (1u << k);
-        acc += gated(ch, Op::join(aj[0], d1p[k]));
-        acc += gated(ch, Op::join(aj[1], d1p[k]));
-        acc += gated(ch, Op::join(aj[0], d1n[k]));
-        acc += gated(ch, Op::join(aj[1], d1n[k]));
-      }
+      const double a1 = d1p[j] + d1n[j];
+      const double a2 = d2p[j] + d2n[j];
+      e3 = __builtin_fma(e2, a1, e3);
+      e2 = __builtin_fma(e1, a1, e2);
+      e1 += a1;
+      // pairs (j, k) and (k, j) with k < j
+      c12 = __builtin_fma(a2, pre1, c12);
+      c12 = __builtin_fma(a1, pre2, c12);
+      pre1 += a1;
+      pre2 += a2;
     }
+    const double v5 = gated(0u, S0 * e2), v6 = gated(0u, S0 * c12), v7 = gated(0u, S0 * e3);
 #pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[5][r], acc, rs[r]);
-  }
-  // set 6: (+-lambda1 on axis j, +-lambda2 on axis k != j)
-  {
-    double acc = 0.0;
-#pragma unroll 1
-    for (int j = 0; j < D; ++j) {
-      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-#pragma unroll 1
-      for (int k = 0; k < D; ++k) {
-        if (k == j) continue;
-        const std::uint32_t ch = (1u << j) | (1u << k);
-        acc += gated(ch, Op::join(aj[0], d2p[k]));
-        acc += gated(ch, Op::join(aj[1], d2p[k]));
-        acc += gated(ch, Op::join(aj[0], d2n[k]));
-        acc += gated(ch, Op::join(aj[1], d2n[k]));
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[6][r], acc, rs[r]);
-  }
-  // set 7: (+-lambda1, +-lambda1, +-lambda1) on axes j < k < l
-  {
-    double acc = 0.0;
-#pragma unroll 1
-    for (int j = 0; j < D; ++j) {
-      const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-#pragma unroll 1
-      for (int k = j + 1; k < D; ++k) {
-        const double ajk[4] = {Op::join(aj[0], d1p[k]), Op::join(aj[1], d1p[k]),
-                               Op::join(aj[0], d1n[k]), Op::join(aj[1], d1n[k])};
-#pragma unroll 1
-        for (int l = k + 1; l < D; ++l) {
-          const std::uint32_t ch = (1u << j) | (1u << k) | (1u << l);
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1p[l]));
-#pragma unroll
-          for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1n[l]));
+    for (int r = 0; r < PAGANI_NRULES; ++r)
+      rs[r] = __builtin_fma(C.w[7][r], v7, __builtin_fma(C.w[6][r], v6,
+                                                         __builtin_fma(C.w[5][r], v5, rs[r])));
+  } else {
+    // set 5: (+-lambda1, +-lambda1) on axes j < k
+    {
+      double acc = 0.0;
+  #pragma unroll 1
+      for (int j = 0; j < D; ++j) {
+        const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
+  #pragma unroll 1
+        for (int k = j + 1; k < D; ++k) {
+          const std::uint32_t ch = (1u << j) | (1u << k);
+          acc += gated(ch, Op::join(aj[0], d1p[k]));
+          acc += gated(ch, Op::join(aj[1], d1p[k]));
+          acc += gated(ch, Op::join(aj[0], d1n[k]));
+          acc += gated(ch, Op::join(aj[1], d1n[k]));
         }
       }
+  #pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[5][r], acc, rs[r]);
     }
-#pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
+    // set 6: (+-lambda1 on axis j, +-lambda2 on axis k != j)
+    {
+      double acc = 0.0;
+  #pragma unroll 1
+      for (int j = 0; j < D; ++j) {
+        const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
+  #pragma unroll 1
+        for (int k = 0; k < D; ++k) {
+          if (k == j) continue;
+          const std::uint32_t ch = (1u << j) | (1u << k);
+          acc += gated(ch, Op::join(aj[0], d2p[k]));
+          acc += gated(ch, Op::join(aj[1], d2p[k]));
+          acc += gated(ch, Op::join(aj[0], d2n[k]));
+          acc += gated(ch, Op::join(aj[1], d2n[k]));
+        }
+      }
+  #pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[6][r], acc, rs[r]);
+    }
+    // set 7: (+-lambda1, +-lambda1, +-lambda1) on axes j < k < l
+    {
+      double acc = 0.0;
+  #pragma unroll 1
+      for (int j = 0; j < D; ++j) {
+        const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
+  #pragma unroll 1
+        for (int k = j + 1; k < D; ++k) {
+          const double ajk[4] = {Op::join(aj[0], d1p[k]), Op::join(aj[1], d1p[k]),
+                                 Op::join(aj[0], d1n[k]), Op::join(aj[1], d1n[k])};
+  #pragma unroll 1
+          for (int l = k + 1; l < D; ++l) {
+            const std::uint32_t ch = (1u << j) | (1u << k) | (1u << l);
+  #pragma unroll
+            for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1p[l]));
+  #pragma unroll
+            for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1n[l]));
+          }
+        }
+      }
+  #pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
+    }
   }
   // set 8: the 2^D corners x = c - g h, g_j = +lambda5 where bit j of the
   // corner index is 0 and -lambda5 where it is 1 (the table's order).  The
@@ -320,7 +396,16 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
   // corners adds the high axes' combination.  An integrand whose outer
   // function is long (heavy_outer) takes the table loop instead: unrolled
   // 2^LB times, its outer would not fit the registers.
-  if constexpr (heavy_outer<F>::value) {
+  if constexpr (kFactor) {
+    double prod = S0;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      prod *= change(j, __builtin_fma(-C.lambda5, hR[j], cR[j])) +
+              change(j, __builtin_fma(C.lambda5, hR[j], cR[j]));
+    const double acc = gated(0u, prod);
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+  } else if constexpr (heavy_outer<F>::value) {
     const int p0 = C.set_begin[3];
     const double acc = set_sum<D, F, double>(
         f, C.gtab + static_cast<std::size_t>(C.set_begin[8] - p0) * D, C.set_begin[8],
@@ -334,10 +419,16 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       // the corners change every axis: plain terms, -infinity where invalid
+      // (exponential: the factors against the centre)
       const double vp = __builtin_fma(-C.lambda5, hR[j], cR[j]);
       const double vn = __builtin_fma(C.lambda5, hR[j], cR[j]);
-      tp[j] = f.term(j, vp);
-      tn[j] = f.term(j, vn);
+      if constexpr (kExp) {
+        tp[j] = change(j, vp);
+        tn[j] = change(j, vn);
+      } else {
+        tp[j] = f.term(j, vp);
+        tn[j] = f.term(j, vn);
+      }
       if constexpr (kMasked) {
         if (!f.valid(j, vp)) tp[j] = -__builtin_inf();
         if (!f.valid(j, vn)) tn[j] = -__builtin_inf();
@@ -364,12 +455,19 @@ __device__ __forceinline__ void separable_sums(F& f, const double (&cR)[D], cons
         for (int j = LB + 1; j < D; ++j) hs = Op::join(hs, ((hq >> (j - LB)) & 1) ? tn[j] : tp[j]);
       }
 #pragma unroll
-      for (int q = 0; q < (1 << LB); ++q)
-        acc += f.outer(HB > 0 ? Op::join(lo[q], hs) : lo[q]);
+      for (int q = 0; q < (1 << LB); ++q) {
+        const double t = HB > 0 ? Op::join(lo[q], hs) : lo[q];
+        if constexpr (kExp)
+          acc += t;
+        else
+          acc += f.outer(t);
+      }
     }
+    if constexpr (kExp) acc *= S0;
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
   }
+  return true;
 }
 
 // One wave's share of the evaluation of up to 64 regions, one lane per
@@ -413,10 +511,7 @@ __device__ __forceinline__ void eval_regions_wave(
 
   bool done = false;
   if constexpr (is_separable<F>::value) {
-    if (slices == 1) {
-      separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
-      done = true;
-    }
+    if (slices == 1) done = separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
   }
   if (!done) {
     if (wave == 0) {

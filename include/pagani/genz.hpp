@@ -131,6 +131,7 @@ struct ProductPeak {
     return static_cast<Real>(k2) / den;
   }
   static constexpr bool multiplicative = true;
+  static constexpr bool linear_outer = true;
   __device__ double term(int, double v) const { return 1.0 / (1.0 + v * v); }
   __device__ double outer(double p) const { return static_cast<double>(static_cast<Real>(k2 * p)); }
 };
@@ -174,6 +175,12 @@ struct CornerPeak {
   }
 };
 
+// The largest centre sum s0 for which an exponential family evaluates a
+// region as products: e^s0 and e^-s0 stay normal with room to spare (FP64
+// 2^+-1022 is e^+-708; FP32 2^+-126 is e^+-87).
+template <class Real>
+constexpr double kFactorRange = sizeof(Real) == 8 ? 650.0 : 80.0;
+
 // exp(-sum a_i^2 (x_i - u_i)^2)
 template <class Real, int D>
 struct Gaussian {
@@ -214,6 +221,16 @@ struct Gaussian {
   {
     return static_cast<double>(math::exp_neg(static_cast<Real>(-s)));
   }
+  // outer(s + d) = outer(s) e^-d.  The terms are non-negative, so in a
+  // region whose centre sums to s0 every factor is at most e^s0 and every
+  // point's value at most 1: s0 bounded well inside the exponent range keeps
+  // the centre's value, each factor and every partial product normal.
+  static constexpr bool exponential = true;
+  __device__ double factor(double d) const
+  {
+    return static_cast<double>(math::exp_bounded(static_cast<Real>(-d)));
+  }
+  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<Real>; }
 };
 
 // exp(-sum a_i |x_i - u_i|)
@@ -247,6 +264,13 @@ struct C0 {
   {
     return static_cast<double>(math::exp_neg(static_cast<Real>(-s)));
   }
+  // outer(s + d) = outer(s) e^-d, non-negative terms (see Gaussian)
+  static constexpr bool exponential = true;
+  __device__ double factor(double d) const
+  {
+    return static_cast<double>(math::exp_bounded(static_cast<Real>(-d)));
+  }
+  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<Real>; }
 };
 
 // 0 if any x_i > u_i, else exp(sum a_i x_i)
