@@ -102,17 +102,21 @@ void set_slices(pagani_eval_consts& C, int S)
 
 // The classifier's threshold search (heuristic_classifier.cuh:282-451)
 // needs, per candidate threshold t, the number of errors above t and their
-// sum.  Every candidate the search visits is a fixed function of earlier
-// ones, so when a candidate has not been measured yet, one pass measures it
-// together with the candidates the search would visit next:
-//   - the rest of a get_larger_threshold_results run from t: t_{k+1} =
-//     t_k + |high - t_k| / 2 (:327-346), and
-//   - the lowerings that follow when t passes the memory test but not the
-//     error budget: t_{j+1} = t_j - |low - t_j| / 2 (:404-410).
-// The search is then replayed on the host with the reference's arithmetic,
-// reading measured values from the cache; a candidate outside it costs
-// another pass.  Decisions are those of the sequential search.
+// sum.  The search is a walk over states (t, low, high) with two moves:
+//   - raise, when t fails the memory test: low = t, t += |high - t| / 2
+//     (get_larger_threshold_results, :327-346);
+//   - lower, when t passes it but not the error budget: high = t,
+//     t -= |low - t| / 2 (:404-410).
+// When a candidate has not been measured yet, one pass measures it together
+// with every state reachable from it in kTreeDepth moves (a full binary
+// tree, breadth first).  The search is then replayed on the host with the
+// reference's arithmetic, reading measured values from the cache; a
+// candidate outside it costs another pass.  Decisions are those of the
+// sequential search.
 constexpr int kRaiseLimit = 20;  // :335
+constexpr int kTreeDepth = 3;
+constexpr int kTreeNodes = (2 << kTreeDepth) - 1;
+static_assert(kTreeNodes <= ph::kMaxThresholds, "one pass measures the whole tree");
 
 struct ThresholdCache {
   std::vector<double> t, dot;
@@ -134,16 +138,22 @@ int measure_threshold(pagani_ws* ws, const double* err, std::size_t n, double t,
 {
   if (cache.find(t, c, d)) return PAGANI_OK;
   ph::ThresholdSet T;
+  double lo[kTreeNodes], hi[kTreeNodes];
   int k = 0;
-  double x = t;
-  for (int i = 0; i < kRaiseLimit && k < ph::kMaxThresholds; ++i) {
-    T.t[k++] = x;
-    x += std::fabs(high - x) * .5;
-  }
-  x = t;
-  while (k < ph::kMaxThresholds) {
-    x -= std::fabs(low - x) * .5;
-    T.t[k++] = x;
+  T.t[k] = t;
+  lo[k] = low;
+  hi[k] = high;
+  ++k;
+  for (int i = 0; k < kTreeNodes; ++i) {
+    const double x = T.t[i];
+    T.t[k] = x + std::fabs(hi[i] - x) * .5;  // raise
+    lo[k] = x;
+    hi[k] = hi[i];
+    ++k;
+    T.t[k] = x - std::fabs(lo[i] - x) * .5;  // lower
+    lo[k] = lo[i];
+    hi[k] = x;
+    ++k;
   }
   T.k = k;
   HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, T, ws->d_mpart, ws->d_mout));

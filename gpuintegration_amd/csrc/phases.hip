@@ -290,35 +290,30 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Fold the kMultiBlocks partials of k_multi_threshold: four lanes per
-// threshold, lane j folds blocks j, j+4, j+8, ... in order, then a fixed
-// butterfly over the four.
-constexpr int kMultiLanes = 4;
-static_assert(kMaxThresholds * kMultiLanes <= kThreads, "one group of lanes per threshold");
-
-__global__ void __launch_bounds__(kThreads)
+// Fold the kMultiBlocks partials of k_multi_threshold: one wave per
+// threshold (workgroup k), lane j folds blocks j, j+64, j+128, ... in order,
+// then a fixed butterfly over the wave.
+__global__ void __launch_bounds__(64)
     k_multi_reduce(int nblocks, int nthr, const double* __restrict__ bpart,
                    double* __restrict__ out)
 {
-  const int k = threadIdx.x / kMultiLanes, j = threadIdx.x % kMultiLanes;
+  const int k = blockIdx.x, j = threadIdx.x;
   DD s{0.0, 0.0};
   double c = 0.0;  // counts are integers below 2^53: exact
-  if (k < nthr) {
-    for (int b = j; b < nblocks; b += kMultiLanes) {
-      const double* p = bpart + static_cast<std::size_t>(b) * 3 * kMaxThresholds + 3 * k;
-      s = dd_add(s, DD{p[0], p[1]});
-      c += p[2];
-    }
+  for (int b = j; b < nblocks; b += 64) {
+    const double* p = bpart + static_cast<std::size_t>(b) * 3 * kMaxThresholds + 3 * k;
+    s = dd_add(s, DD{p[0], p[1]});
+    c += p[2];
   }
 #pragma unroll
-  for (int off = kMultiLanes / 2; off > 0; off >>= 1) {
+  for (int off = 32; off > 0; off >>= 1) {
     DD o;
     o.hi = __shfl_xor(s.hi, off, 64);
     o.lo = __shfl_xor(s.lo, off, 64);
     s = dd_add(s, o);
     c += __shfl_xor(c, off, 64);
   }
-  if (k < nthr && j == 0) {
+  if (j == 0) {
     out[k] = c;
     out[nthr + k] = s.hi + s.lo;
   }
@@ -920,16 +915,15 @@ hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err, cons
   if (T.k < 1 || T.k > kMaxThresholds) return hipErrorInvalidValue;
   ThresholdSet U = T;
   for (int k = U.k; k < kMaxThresholds; ++k) U.t[k] = __builtin_inf();
-  if (U.k <= kMaxThresholds / 2)
-    hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds / 2>, dim3(kMultiBlocks), dim3(kThreads),
-                       0, s, n, err, U, bpart);
+  if (U.k <= 16)
+    hipLaunchKernelGGL(k_multi_threshold<16>, dim3(kMultiBlocks), dim3(kThreads), 0, s, n, err,
+                       U, bpart);
   else
     hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds>, dim3(kMultiBlocks), dim3(kThreads), 0,
                        s, n, err, U, bpart);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_multi_reduce, dim3(1), dim3(kThreads), 0, s, kMultiBlocks, T.k, bpart,
-                     out);
+  hipLaunchKernelGGL(k_multi_reduce, dim3(T.k), dim3(64), 0, s, kMultiBlocks, T.k, bpart, out);
   return hipGetLastError();
 }
 

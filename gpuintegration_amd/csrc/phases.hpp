@@ -48,9 +48,8 @@ struct DevCompact {
   std::uint32_t cur, run, has_parents, pad;
 };
 
-// Candidate thresholds measured by one pass of the classifier probe: the
-// search's next 20 raises (heuristic_classifier.cuh:335 caps a run at 20)
-// and the next 20 lowerings, all computable before the pass.
+// Candidate thresholds measured by one pass of the classifier probe (the
+// host measures the states a few moves of the search can reach).
 constexpr int kMaxThresholds = 40;
 struct ThresholdSet {
   double t[kMaxThresholds];
