@@ -209,13 +209,17 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
   constexpr bool kMul = is_multiplicative<F>::value || kExp;
   using Op = SepOp<kMul>;
   constexpr bool kMasked = is_masked<F>::value;
-  static_assert(!(kMasked && kMul), "masks are supported for additive terms only");
   // A point's value is the centre's value times the changed axes' factors
   // (exponential, or multiplicative with a linear outer): a set's sum over
   // every sign combination on a set of axes then factors per axis, e.g.
   // sum over the 4 points (+-l1, +-l1) on axes j, k = S0 s_j s_k with
   // s_j = r(+l1)_j + r(-l1)_j, and the 2^d corners sum to S0 prod_j s5_j.
+  // With masks, a factor is 0 where the moved coordinate is invalid and an
+  // unchanged axis contributes u_j = [valid at the centre]: the sum over the
+  // sets of m changed axes is the x^m coefficient of prod_j (u_j + x s_j).
   constexpr bool kFactor = kExp || (kMul && has_linear_outer<F>::value);
+  static_assert(!(kMasked && kMul && !kFactor), "masks need additive terms or factored sums");
+  const int bisect0 = bisect;
   double T0[D];
 #pragma unroll
   for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
@@ -230,7 +234,7 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
   // (exponential: the factor of the value)
   auto change = [&](int j, double v) {
     if constexpr (kMasked)
-      if (!f.valid(j, v)) return -__builtin_inf();
+      if (!f.valid(j, v)) return kMul ? 0.0 : -__builtin_inf();
     if constexpr (kExp) return f.factor(f.term(j, v) - T0[j]);
     return Op::change(f.term(j, v), T0[j]);
   };
@@ -241,6 +245,16 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
       if (vmask & ~changed) return 0.0;
     if constexpr (kExp) return s;
     return f.outer(s);
+  };
+  // the value of a factored sum of points (no masks applied here)
+  auto value = [&](double p) {
+    if constexpr (kExp) return p;
+    return f.outer(p);
+  };
+  // u_j: 1 when an unchanged axis j keeps the point valid
+  auto unchanged = [&](int j) {
+    if constexpr (kMasked) return ((vmask >> j) & 1u) ? 0.0 : 1.0;
+    return 1.0;
   };
   // the centre's joined terms (exponential: the centre's value)
   double S0 = T0[0];
@@ -289,11 +303,16 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
     const double lam = lam34[s - 3];
     double acc = 0.0;
     if constexpr (kFactor) {
+      double E0 = 1.0;  // x^0 and x^1 coefficients of prod_j (u_j + x s_j)
 #pragma unroll
-      for (int j = 0; j < D; ++j)
-        acc += change(j, __builtin_fma(-lam, hR[j], cR[j])) +
-               change(j, __builtin_fma(lam, hR[j], cR[j]));
-      acc = gated(0u, S0 * acc);
+      for (int j = 0; j < D; ++j) {
+        const double sj = change(j, __builtin_fma(-lam, hR[j], cR[j])) +
+                          change(j, __builtin_fma(lam, hR[j], cR[j]));
+        const double uj = unchanged(j);
+        acc = __builtin_fma(E0, sj, acc * uj);
+        E0 *= uj;
+      }
+      acc = value(S0 * acc);
     } else {
 #pragma unroll
       for (int j = 0; j < D; ++j) {
@@ -307,23 +326,25 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
   if constexpr (kFactor) {
     // sets 5-7 from the per-axis sums s1_j, s2_j of the +-lambda1 and
     // +-lambda2 factors: set 5 sums S0 s1_j s1_k over j < k, set 6 S0 s1_j
-    // s2_k over j != k, set 7 S0 s1_j s1_k s1_l over j < k < l.  Every sum
+    // s2_k over j != k, set 7 S0 s1_j s1_k s1_l over j < k < l -- the x^2,
+    // xy and x^3 coefficients of prod_j (u_j + x s1_j + y s2_j).  Every sum
     // below adds non-negative products when the factors are positive.
-    double e1 = 0.0, e2 = 0.0, e3 = 0.0, c12 = 0.0, pre1 = 0.0, pre2 = 0.0;
+    double e0 = 1.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, g10 = 0.0, g01 = 0.0, g11 = 0.0;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       const double a1 = d1p[j] + d1n[j];
       const double a2 = d2p[j] + d2n[j];
-      e3 = __builtin_fma(e2, a1, e3);
-      e2 = __builtin_fma(e1, a1, e2);
-      e1 += a1;
-      // pairs (j, k) and (k, j) with k < j
-      c12 = __builtin_fma(a2, pre1, c12);
-      c12 = __builtin_fma(a1, pre2, c12);
-      pre1 += a1;
-      pre2 += a2;
+      const double uj = unchanged(j);
+      e3 = __builtin_fma(e2, a1, e3 * uj);
+      e2 = __builtin_fma(e1, a1, e2 * uj);
+      // the pairs (j, k) and (k, j) with k < j
+      g11 = __builtin_fma(g01, a1, __builtin_fma(g10, a2, g11 * uj));
+      g10 = __builtin_fma(e0, a1, g10 * uj);
+      g01 = __builtin_fma(e0, a2, g01 * uj);
+      e1 = g10;
+      e0 *= uj;
     }
-    const double v5 = gated(0u, S0 * e2), v6 = gated(0u, S0 * c12), v7 = gated(0u, S0 * e3);
+    const double v5 = value(S0 * e2), v6 = value(S0 * g11), v7 = value(S0 * e3);
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r)
       rs[r] = __builtin_fma(C.w[7][r], v7, __builtin_fma(C.w[6][r], v6,
@@ -332,10 +353,10 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
     // set 5: (+-lambda1, +-lambda1) on axes j < k
     {
       double acc = 0.0;
-  #pragma unroll 1
+#pragma unroll 1
       for (int j = 0; j < D; ++j) {
         const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-  #pragma unroll 1
+#pragma unroll 1
         for (int k = j + 1; k < D; ++k) {
           const std::uint32_t ch = (1u << j) | (1u << k);
           acc += gated(ch, Op::join(aj[0], d1p[k]));
@@ -344,16 +365,16 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
           acc += gated(ch, Op::join(aj[1], d1n[k]));
         }
       }
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[5][r], acc, rs[r]);
     }
     // set 6: (+-lambda1 on axis j, +-lambda2 on axis k != j)
     {
       double acc = 0.0;
-  #pragma unroll 1
+#pragma unroll 1
       for (int j = 0; j < D; ++j) {
         const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-  #pragma unroll 1
+#pragma unroll 1
         for (int k = 0; k < D; ++k) {
           if (k == j) continue;
           const std::uint32_t ch = (1u << j) | (1u << k);
@@ -363,30 +384,30 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
           acc += gated(ch, Op::join(aj[1], d2n[k]));
         }
       }
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[6][r], acc, rs[r]);
     }
     // set 7: (+-lambda1, +-lambda1, +-lambda1) on axes j < k < l
     {
       double acc = 0.0;
-  #pragma unroll 1
+#pragma unroll 1
       for (int j = 0; j < D; ++j) {
         const double aj[2] = {Op::join(S0, d1p[j]), Op::join(S0, d1n[j])};
-  #pragma unroll 1
+#pragma unroll 1
         for (int k = j + 1; k < D; ++k) {
           const double ajk[4] = {Op::join(aj[0], d1p[k]), Op::join(aj[1], d1p[k]),
                                  Op::join(aj[0], d1n[k]), Op::join(aj[1], d1n[k])};
-  #pragma unroll 1
+#pragma unroll 1
           for (int l = k + 1; l < D; ++l) {
             const std::uint32_t ch = (1u << j) | (1u << k) | (1u << l);
-  #pragma unroll
+#pragma unroll
             for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1p[l]));
-  #pragma unroll
+#pragma unroll
             for (int q = 0; q < 4; ++q) acc += gated(ch, Op::join(ajk[q], d1n[l]));
           }
         }
       }
-  #pragma unroll
+#pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
     }
   }
@@ -402,9 +423,22 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
     for (int j = 0; j < D; ++j)
       prod *= change(j, __builtin_fma(-C.lambda5, hR[j], cR[j])) +
               change(j, __builtin_fma(C.lambda5, hR[j], cR[j]));
-    const double acc = gated(0u, prod);
+    const double acc = value(prod);
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+    if constexpr (kMasked && kExp) {
+      // a masked exponential's factors are bounded only by the box: a
+      // region whose sums left the range takes the generic loop
+      double t = 0.0;
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) t += __builtin_fabs(rs[r]);
+      if (!__builtin_isfinite(t)) {
+#pragma unroll
+        for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = 0.0;
+        bisect = bisect0;
+        return false;
+      }
+    }
   } else if constexpr (heavy_outer<F>::value) {
     const int p0 = C.set_begin[3];
     const double acc = set_sum<D, F, double>(

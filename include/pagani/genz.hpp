@@ -314,13 +314,24 @@ struct Discontinuous {
     return m > Real(0) ? Real(0) : v;
   }
   static constexpr bool masked = true;
-  // at 8D in double the masked separable form spills; the generic loop wins
-  static constexpr bool separable = D < 8 || sizeof(Real) == 4;
   __device__ bool valid(int, double t) const { return t <= 0.0; }
   __device__ double term(int i, double t) const { return static_cast<double>(a[i]) * t; }
   __device__ double outer(double s) const
   {
     return static_cast<double>(math::exp_bounded(static_cast<Real>(c + s)));
+  }
+  // outer(s + d) = outer(s) e^d.  The centre's value e^(c + s0) is kept
+  // well inside the exponent range; the factors are bounded by e^(a_j h_j)
+  // (the kernel sends a region whose sums overflow anyway to the generic
+  // loop).
+  static constexpr bool exponential = true;
+  __device__ double factor(double d) const
+  {
+    return static_cast<double>(math::exp_bounded(static_cast<Real>(d)));
+  }
+  __device__ bool factor_safe(double s0) const
+  {
+    return __builtin_fabs(c + s0) <= kFactorRange<Real> - (sizeof(Real) == 8 ? 100.0 : 20.0);
   }
 };
 
