@@ -141,6 +141,13 @@ template <class F>
 struct is_exponential<F, std::void_t<decltype(F::exponential)>>
     : std::bool_constant<F::exponential> {};
 
+// Cosine integrands (`static constexpr bool cosine = true`, centred and
+// scaled, `double phase_angle()`): f = cos(phase_angle() + sum_j v_j).
+template <class F, class = void>
+struct is_cosine : std::false_type {};
+template <class F>
+struct is_cosine<F, std::void_t<decltype(F::cosine)>> : std::bool_constant<F::cosine> {};
+
 // A multiplicative integrand whose outer is linear (outer(p) = k p, up to
 // the rounding of its result) says so with `static constexpr bool
 // linear_outer = true`.
@@ -504,6 +511,74 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
   return true;
 }
 
+// Rule sums and split axis of one region for a cosine integrand (slices ==
+// 1).  A point that moves v_j by -g_j h_j on a set A of axes has the angle
+// phi0 - sum_A g_j h_j; the rule takes every sign combination of the g_j,
+// and the sum over them is Re(e^(i phi0) prod_A (e^(i l h_j) + e^(-i l h_j)))
+// = cos(phi0) prod_A 2 cos(l h_j).  So a set's sum is cos(phi0) times a
+// symmetric sum of per-axis cosines: per region, one sincos at the centre,
+// a sincos at +-lambda1 and +-lambda2 per axis (the fourth difference needs
+// those points one by one) and a cosine at lambda3..5 per axis.
+__device__ __forceinline__ void sincos_real(double x, double* s, double* c) { ::sincos(x, s, c); }
+__device__ __forceinline__ void sincos_real(float x, float* s, float* c) { ::sincosf(x, s, c); }
+__device__ __forceinline__ double cos_real(double x) { return ::cos(x); }
+__device__ __forceinline__ float cos_real(float x) { return ::cosf(x); }
+
+template <int D, class F, class Real>
+__device__ __forceinline__ void cosine_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+                                            double jac, const pagani_eval_consts& C,
+                                            double (&rs)[PAGANI_NRULES], int& bisect)
+{
+  double phi0 = f.phase_angle();
+#pragma unroll
+  for (int j = 0; j < D; ++j) phi0 += cR[j];
+  Real sr, cr;
+  sincos_real(static_cast<Real>(phi0), &sr, &cr);
+  const double c0 = cr, s0 = sr;
+  const double base = (c0 * jac) * 2 * (1 - C.ratio);
+  double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+  // x^m and xy coefficients of prod_j (1 + x A1_j + y A2_j), A = 2 cos(l h_j)
+  double e1 = 0.0, e2 = 0.0, e3 = 0.0, g01 = 0.0, g11 = 0.0;
+  double a3 = 0.0, a4 = 0.0, p5 = 1.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    Real sa, ca, sb, cb;
+    sincos_real(static_cast<Real>(C.lambda1 * hR[j]), &sa, &ca);
+    sincos_real(static_cast<Real>(C.lambda2 * hR[j]), &sb, &cb);
+    // cos(phi0 -+ l h_j) = c0 cos(l h_j) +- s0 sin(l h_j)
+    const double fa = __builtin_fma(s0, static_cast<double>(sa), c0 * ca);
+    const double fb = __builtin_fma(-s0, static_cast<double>(sa), c0 * ca);
+    const double fc = __builtin_fma(s0, static_cast<double>(sb), c0 * cb);
+    const double fd = __builtin_fma(-s0, static_cast<double>(sb), c0 * cb);
+    s1 += fa + fb;
+    s2 += fc + fd;
+    const double diff =
+        __builtin_fabs(base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
+    if (diff > maxdiff) {
+      maxdiff = diff;
+      bisect = j;
+    }
+    const double A1 = 2.0 * static_cast<double>(ca), A2 = 2.0 * static_cast<double>(cb);
+    e3 = __builtin_fma(e2, A1, e3);
+    e2 = __builtin_fma(e1, A1, e2);
+    g11 = __builtin_fma(g01, A1, __builtin_fma(e1, A2, g11));
+    e1 += A1;
+    g01 += A2;
+    a3 += 2.0 * static_cast<double>(cos_real(static_cast<Real>(C.lambda3 * hR[j])));
+    a4 += 2.0 * static_cast<double>(cos_real(static_cast<Real>(C.lambda4 * hR[j])));
+    p5 *= 2.0 * static_cast<double>(cos_real(static_cast<Real>(C.lambda5 * hR[j])));
+  }
+  const double v[PAGANI_NSETS] = {c0,      s1,      s2,      c0 * a3, c0 * a4,
+                                  c0 * e2, c0 * g11, c0 * e3, c0 * p5};
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r) {
+    double acc = C.w[0][r] * v[0] + C.w[1][r] * v[1] + C.w[2][r] * v[2];
+#pragma unroll
+    for (int s = 3; s < PAGANI_NSETS; ++s) acc = __builtin_fma(C.w[s][r], v[s], acc);
+    rs[r] = acc;
+  }
+}
+
 // One wave's share of the evaluation of up to 64 regions, one lane per
 // region t of a SoA pool with row stride `stride`.  `wave` indexes the
 // `slices` waves that split the batch's points (0 when slices == 1); with
@@ -544,7 +619,12 @@ __device__ __forceinline__ void eval_regions_wave(
   int bisect = maxdim;
 
   bool done = false;
-  if constexpr (is_separable<F>::value) {
+  if constexpr (is_cosine<F>::value && has_origin<F>::value && has_axis_scale<F>::value) {
+    if (slices == 1) {
+      cosine_sums<D, F, Real>(f, cR, hR, jac, C, rs, bisect);
+      done = true;
+    }
+  } else if constexpr (is_separable<F>::value) {
     if (slices == 1) done = separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
   }
   if (!done) {

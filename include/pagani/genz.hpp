@@ -91,6 +91,10 @@ struct Oscillatory {
     return math::cos_(s);
   }
   static constexpr bool heavy_outer = true;  // cos: argument reduction and two polynomials
+  // f = cos(phase_angle() + sum_j v_j): the kernel sums each set's sign
+  // combinations as cos of the centre's angle times per-axis cosines
+  static constexpr bool cosine = true;
+  __device__ double phase_angle() const { return static_cast<double>(phase); }
   __device__ double term(int, double v) const { return v; }
   __device__ double outer(double s) const
   {
