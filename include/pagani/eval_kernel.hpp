@@ -31,6 +31,7 @@
 #include <utility>
 
 #include "pagani.h"
+#include "pagani/trig.hpp"
 
 namespace pagani {
 
@@ -519,19 +520,30 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
 // symmetric sum of per-axis cosines: per region, one sincos at the centre,
 // a sincos at +-lambda1 and +-lambda2 per axis (the fourth difference needs
 // those points one by one) and a cosine at lambda3..5 per axis.
-__device__ __forceinline__ void sincos_real(double x, double* s, double* c) { ::sincos(x, s, c); }
+__device__ __forceinline__ void sincos_real(double x, double* s, double* c)
+{
+  math::sincos_bounded(x, *s, *c);
+}
 __device__ __forceinline__ void sincos_real(float x, float* s, float* c) { ::sincosf(x, s, c); }
-__device__ __forceinline__ double cos_real(double x) { return ::cos(x); }
+__device__ __forceinline__ double cos_real(double x) { return math::cos_bounded(x); }
 __device__ __forceinline__ float cos_real(float x) { return ::cosf(x); }
 
+// Returns false, having done nothing, when an angle could leave the range of
+// the FP64 sine and cosine (math::kTrigRange).
 template <int D, class F, class Real>
-__device__ __forceinline__ void cosine_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+__device__ __forceinline__ bool cosine_sums(F& f, const double (&cR)[D], const double (&hR)[D],
                                             double jac, const pagani_eval_consts& C,
                                             double (&rs)[PAGANI_NRULES], int& bisect)
 {
   double phi0 = f.phase_angle();
+  double hmax = 0.0;
 #pragma unroll
-  for (int j = 0; j < D; ++j) phi0 += cR[j];
+  for (int j = 0; j < D; ++j) {
+    phi0 += cR[j];
+    hmax = __builtin_fmax(hmax, __builtin_fabs(hR[j]));
+  }
+  // every lambda is below 1: the axis angles are at most |h_j|
+  if (!(__builtin_fabs(phi0) <= math::kTrigRange && hmax <= math::kTrigRange)) return false;
   Real sr, cr;
   sincos_real(static_cast<Real>(phi0), &sr, &cr);
   const double c0 = cr, s0 = sr;
@@ -577,6 +589,7 @@ __device__ __forceinline__ void cosine_sums(F& f, const double (&cR)[D], const d
     for (int s = 3; s < PAGANI_NSETS; ++s) acc = __builtin_fma(C.w[s][r], v[s], acc);
     rs[r] = acc;
   }
+  return true;
 }
 
 // One wave's share of the evaluation of up to 64 regions, one lane per
@@ -620,10 +633,7 @@ __device__ __forceinline__ void eval_regions_wave(
 
   bool done = false;
   if constexpr (is_cosine<F>::value && has_origin<F>::value && has_axis_scale<F>::value) {
-    if (slices == 1) {
-      cosine_sums<D, F, Real>(f, cR, hR, jac, C, rs, bisect);
-      done = true;
-    }
+    if (slices == 1) done = cosine_sums<D, F, Real>(f, cR, hR, jac, C, rs, bisect);
   } else if constexpr (is_separable<F>::value) {
     if (slices == 1) done = separable_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
   }
@@ -637,19 +647,19 @@ __device__ __forceinline__ void eval_regions_wave(
       const double f0 = detail::call(f, xc);
       const double base = (f0 * jac) * 2 * (1 - C.ratio);
       double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
+      // f at the centre moved by -g h_j on axis j
+      auto moved = [&](int j, double g) {
         Real x[D];
 #pragma unroll
-        for (int k = 0; k < D; ++k) x[k] = xc[k];
-        x[j] = static_cast<Real>(__builtin_fma(-C.lambda1, hR[j], cR[j]));
-        const double fa = detail::call(f, x);
-        x[j] = static_cast<Real>(__builtin_fma(C.lambda1, hR[j], cR[j]));
-        const double fb = detail::call(f, x);
-        x[j] = static_cast<Real>(__builtin_fma(-C.lambda2, hR[j], cR[j]));
-        const double fc = detail::call(f, x);
-        x[j] = static_cast<Real>(__builtin_fma(C.lambda2, hR[j], cR[j]));
-        const double fd = detail::call(f, x);
+        for (int k = 0; k < D; ++k)
+          x[k] = k == j ? static_cast<Real>(__builtin_fma(-g, hR[k], cR[k])) : xc[k];
+        return detail::call(f, x);
+      };
+      auto axis = [&](int j) {
+        const double fa = moved(j, C.lambda1);
+        const double fb = moved(j, -C.lambda1);
+        const double fc = moved(j, C.lambda2);
+        const double fd = moved(j, -C.lambda2);
         s1 += fa + fb;
         s2 += fc + fd;
         const double diff = __builtin_fabs(
@@ -658,6 +668,15 @@ __device__ __forceinline__ void eval_regions_wave(
           maxdiff = diff;
           bisect = j;
         }
+      };
+      if constexpr (heavy_outer<F>::value) {
+        // a long integrand: one copy of the four evaluations, the axis
+        // chosen by selects
+#pragma unroll 1
+        for (int j = 0; j < D; ++j) axis(j);
+      } else {
+#pragma unroll
+        for (int j = 0; j < D; ++j) axis(j);
       }
 #pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r)

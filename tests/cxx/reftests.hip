@@ -550,6 +550,48 @@ extern "C" int reftest_exp_neg(const double* x, std::size_t n, double* mine, dou
 }
 
 // ---------------------------------------------------------------------------
+// The bounded-argument sine and cosine of the cosine integrands' factored
+// sums (pagani::math::sincos_bounded) beside the device library's.
+
+#include "pagani/trig.hpp"
+
+namespace {
+__global__ void k_sincos(const double* x, double* s, double* c, double* ls, double* lc,
+                         std::size_t n)
+{
+  const std::size_t i = blockIdx.x * static_cast<std::size_t>(blockDim.x) + threadIdx.x;
+  if (i < n) {
+    double a, b;
+    pagani::math::sincos_bounded(x[i], a, b);
+    s[i] = a;
+    c[i] = b;
+    ::sincos(x[i], &a, &b);
+    ls[i] = a;
+    lc[i] = b;
+  }
+}
+}  // namespace
+
+extern "C" int reftest_sincos(const double* x, std::size_t n, double* s, double* c, double* ls,
+                              double* lc)
+{
+  try {
+    DevVec<double> dx(n), ds(n), dc(n), dls(n), dlc(n);
+    dx.put(x, n);
+    hipLaunchKernelGGL(k_sincos, dim3(blocks_for(n)), dim3(256), 0, 0, dx.p, ds.p, dc.p, dls.p,
+                       dlc.p, n);
+    if (hipDeviceSynchronize() != hipSuccess) return -2;
+    ds.get(s, n);
+    dc.get(c, n);
+    dls.get(ls, n);
+    dlc.get(lc, n);
+    return 0;
+  } catch (const std::exception&) {
+    return -1;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // m-Cubes through the reference's C++ surface (include/pagani/mcubes.hpp).
 // The integrands are the ones of cuda/mcubes/demos (Genz2_6D.cu, Genz3_3D.cu,
 // Genz4_5D.cu, Genz5_8D.cu, Genz6_6D.cu, Genz1_8D.cu, SinSum6D.cu, Gauss9D.cu).
