@@ -37,12 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per region of the 8D Gaussian evaluation kernel, from two
 # rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over `bench.py
 # --eval-only`, each scaled by what the counter reads for a 1 GiB device
-# copy: 129.13 B read + 17.00 B written (algorithmic: 128 + 17).
-EVAL_HBM_BYTES_PER_REGION_8D = 146.13
+# copy: 128.20 B read + 17.00 B written (algorithmic: 128 + 17).
+EVAL_HBM_BYTES_PER_REGION_8D = 145.20
 # FP64 work the 8D Gaussian evaluation kernel executes per region, from one
 # rocprofv3 counter pass over `bench.py --eval-only` (SQ_INSTS_VALU_FMA_F64,
 # _MUL_F64, _ADD_F64 and SQ_INSTS_VALU, one lane per region, every lane
-# active): 1366 FMA + 233 MUL + 128 ADD = 3093 FLOP and 2216 VALU
+# active): 1366 FMA + 233 MUL + 128 ADD = 3093 FLOP and 2221 VALU
 # instructions.  The algorithmic count (90770 FLOP) charges every one of the
 # 1105 points a full d-axis evaluation; the kernel evaluates the Gaussian as
 # a product over axes -- the centre's value times one factor e^-(change of
@@ -51,7 +51,7 @@ EVAL_HBM_BYTES_PER_REGION_8D = 146.13
 # -- so the algorithmic rate would exceed the FP64 peak: the roofline uses
 # the executed FLOPs.
 EVAL_EXEC_FLOP_PER_REGION_8D = 3093
-EVAL_VALU_PER_REGION_8D = 2216
+EVAL_VALU_PER_REGION_8D = 2221
 VALU_ISSUE_PEAK = 256 * 2.4e9  # wave-instructions per second (one per CU per clock)
 
 
@@ -318,7 +318,11 @@ def config5(args, pg, W):
     out = []
     for fam, a, u in cases:
         exact = W.exact(fam, a, u)
-        res, dt, st, tr = run_gpu(pg, fam, a, u, args.large_epsrel, 1, 0, pg.FP64,
+        # after one untimed integral on the same workspace the pool is at its
+        # full size: the timed integral allocates nothing (a first integral's
+        # growth reallocates up to hundreds of GB, whose cost depends on how
+        # recently the driver got that memory back)
+        res, dt, st, tr = run_gpu(pg, fam, a, u, args.large_epsrel, 1, 1, pg.FP64,
                                   mem_model="actual")
         rep = phase_report(W, fam, d, res, dt, st, tr, 1)
         out.append(dict(rep, family=fam, abs_err_vs_analytic=abs(res.estimate - exact),
