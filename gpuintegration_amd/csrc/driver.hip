@@ -116,7 +116,7 @@ void set_slices(pagani_eval_consts& C, int S)
 constexpr int kRaiseLimit = 20;  // :335
 constexpr int kTreeDepth = 3;
 constexpr int kTreeNodes = (2 << kTreeDepth) - 1;
-static_assert(kTreeNodes <= ph::kMaxThresholds, "one pass measures the whole tree");
+static_assert(kTreeNodes < ph::kMaxThresholds, "one pass measures the whole tree");
 
 struct ThresholdCache {
   std::vector<double> t, dot;
@@ -156,18 +156,30 @@ int measure_threshold(pagani_ws* ws, const double* err, std::size_t n, double t,
     ++k;
   }
   T.k = k;
-  HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, T, ws->d_mpart, ws->d_mout));
+  // the probe takes its candidates in ascending order; a NaN candidate
+  // (err > NaN holds for no error) is measured as +inf, which counts the same
+  int perm[ph::kMaxThresholds];
+  for (int i = 0; i < k; ++i) perm[i] = i;
+  auto key = [&](int i) { return std::isnan(T.t[i]) ? HUGE_VAL : T.t[i]; };
+  std::sort(perm, perm + k, [&](int a, int b) { return key(a) < key(b) || (key(a) == key(b) && a < b); });
+  ph::ThresholdSet S;
+  S.k = k;
+  for (int i = 0; i < k; ++i) S.t[i] = key(perm[i]);
+  HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, S, ws->d_mpart, ws->d_mout));
   HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * k * sizeof(double),
                              hipMemcpyDeviceToHost, ws->stream));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   ws->stats.classify_passes++;
   for (int i = 0; i < k; ++i) {
-    cache.t.push_back(T.t[i]);
+    const int o = perm[i];  // sorted slot i holds candidate o
+    cache.t.push_back(T.t[o]);
     cache.cnt.push_back(static_cast<std::uint64_t>(ws->h_mout[i]));
     cache.dot.push_back(ws->h_mout[k + i]);
+    if (o == 0) {
+      c = static_cast<std::uint64_t>(ws->h_mout[i]);
+      d = ws->h_mout[k + i];
+    }
   }
-  c = static_cast<std::uint64_t>(ws->h_mout[0]);
-  d = ws->h_mout[k];
   return PAGANI_OK;
 }
 

@@ -228,60 +228,70 @@ __global__ void __launch_bounds__(kThreads)
   write_record(se, sr, sae, sar, c, part, cnt);
 }
 
-// The candidate thresholds of one run of get_larger_threshold_results
-// (heuristic_classifier.cuh:327-346) in a single read of the errors: per
-// threshold, the count of err > t_k (apply_threshold, :282-303) and the sum
-// of those errors (the dot product of evaluate_error_budget, :305-325).
-// Fixed grid, grid-stride order, fixed reduction trees: deterministic.
+// The classifier's candidate thresholds in a single read of the errors: per
+// candidate t_k, the count of err > t_k (apply_threshold,
+// heuristic_classifier.cuh:282-303) and the sum of those errors (the dot
+// product of evaluate_error_budget, :305-325).  The candidates arrive sorted
+// ascending and padded with +inf to K, at most K - 1 of them real.  Each
+// thread keeps K - 1 buckets in LDS: bucket k holds its errors with
+// t_k < err <= t_(k+1), found by a binary search over the candidates, so an
+// error costs a few compares instead of one compare and masked add per
+// candidate; the count and sum of err > t_k are then the thread's buckets
+// k..K-2.  Fixed grid, grid-stride order, per-thread sums in element order,
+// fixed reduction trees: deterministic.
 template <int K>
 __global__ void __launch_bounds__(kThreads)
     k_multi_threshold(std::size_t n, const double* __restrict__ err, const ThresholdSet T,
                       double* __restrict__ bpart)
 {
+  static_assert((K & (K - 1)) == 0, "binary search over a power of two");
+  __shared__ double ts[K];
+  __shared__ double bsum[K - 1][kThreads];
+  __shared__ std::uint32_t bcnt[K - 1][kThreads];
   __shared__ DD sh[kThreads / 64];
   __shared__ std::uint32_t shc[kThreads / 64];
-  std::uint32_t cnt[K];
-  double sum[K];
+  const int tid = threadIdx.x;
+  if (tid < K) ts[tid] = T.t[tid];
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    cnt[k] = 0;
-    sum[k] = 0.0;
+  for (int k = 0; k < K - 1; ++k) {
+    bsum[k][tid] = 0.0;
+    bcnt[k][tid] = 0;
   }
+  __syncthreads();
+  auto add = [&](double r) {
+    // b = #{k : r > t_k} over t_0..t_(K-2) (t_(K-1) is +inf)
+    int b = 0;
+#pragma unroll
+    for (int step = K / 2; step > 0; step >>= 1)
+      if (r > ts[b + step - 1]) b += step;
+    if (b > 0) {
+      bsum[b - 1][tid] += r;
+      bcnt[b - 1][tid] += 1u;
+    }
+  };
   // Grid-stride order; four loads are issued before their values are
   // consumed (in the same order) so more bytes are in flight per wave.
-  // Unused candidates (k >= T.k) hold +inf and count nothing.
   const std::size_t stride = static_cast<std::size_t>(gridDim.x) * kThreads;
-  std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + threadIdx.x;
+  std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + tid;
   for (; t + 3 * stride < n; t += 4 * stride) {
     double r[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) r[i] = err[t + i * stride];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const bool a = r[i] > T.t[k];
-        cnt[k] += a ? 1u : 0u;
-        sum[k] += a ? r[i] : 0.0;
-      }
-    }
+    for (int i = 0; i < 4; ++i) add(r[i]);
   }
-  for (; t < n; t += stride) {
-    const double r = err[t];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const bool a = r > T.t[k];
-      cnt[k] += a ? 1u : 0u;
-      sum[k] += a ? r : 0.0;
-    }
-  }
+  for (; t < n; t += stride) add(err[t]);
+  // suffix sums: candidate k counts the buckets k..K-2
   double* out = bpart + static_cast<std::size_t>(blockIdx.x) * 3 * kMaxThresholds;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  double sum = 0.0;
+  std::uint32_t cnt = 0;
+  for (int k = K - 2; k >= 0; --k) {
+    sum += bsum[k][tid];
+    cnt += bcnt[k][tid];
     if (k < T.k) {  // uniform
-      const DD s = block_reduce(DD{sum[k], 0.0}, sh);
-      const std::uint32_t c = block_count(cnt[k], shc);
-      if (threadIdx.x == 0) {
+      const DD s = block_reduce(DD{sum, 0.0}, sh);
+      const std::uint32_t c = block_count(cnt, shc);
+      if (tid == 0) {
         out[3 * k] = s.hi;
         out[3 * k + 1] = s.lo;
         out[3 * k + 2] = static_cast<double>(c);
@@ -912,15 +922,13 @@ hipError_t threshold_flags(hipStream_t s, std::size_t n, const double* est, cons
 hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err, const ThresholdSet& T,
                            double* bpart, double* out)
 {
-  if (T.k < 1 || T.k > kMaxThresholds) return hipErrorInvalidValue;
+  if (T.k < 1 || T.k >= kMaxThresholds) return hipErrorInvalidValue;
+  for (int k = 1; k < T.k; ++k)
+    if (!(T.t[k - 1] <= T.t[k])) return hipErrorInvalidValue;  // sorted, no NaN
   ThresholdSet U = T;
   for (int k = U.k; k < kMaxThresholds; ++k) U.t[k] = __builtin_inf();
-  if (U.k <= 16)
-    hipLaunchKernelGGL(k_multi_threshold<16>, dim3(kMultiBlocks), dim3(kThreads), 0, s, n, err,
-                       U, bpart);
-  else
-    hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds>, dim3(kMultiBlocks), dim3(kThreads), 0,
-                       s, n, err, U, bpart);
+  hipLaunchKernelGGL(k_multi_threshold<kMaxThresholds>, dim3(kMultiBlocks), dim3(kThreads), 0, s,
+                     n, err, U, bpart);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_multi_reduce, dim3(T.k), dim3(64), 0, s, kMultiBlocks, T.k, bpart, out);

@@ -49,8 +49,9 @@ struct DevCompact {
 };
 
 // Candidate thresholds measured by one pass of the classifier probe (the
-// host measures the states a few moves of the search can reach).
-constexpr int kMaxThresholds = 40;
+// host measures the states a few moves of the search can reach): sorted
+// ascending, at most kMaxThresholds - 1 of them.
+constexpr int kMaxThresholds = 16;
 struct ThresholdSet {
   double t[kMaxThresholds];
   int k;
