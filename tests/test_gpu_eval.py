@@ -137,6 +137,37 @@ def test_eval_random_regions_fp64(pg, gpu, family, d, slices):
     check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, centred_cond(family, a))
 
 
+@pytest.mark.parametrize("family,scale", [("gaussian", 20.0), ("c0", 200.0),
+                                          ("discontinuous", 100.0), ("oscillatory", 2e5)])
+def test_eval_factored_range_fallback(pg, gpu, family, scale):
+    """Parameters large enough that part of the regions leave the range of
+    the factored sums (a Gaussian or C0 centre sum above 650, a
+    discontinuous exponent beyond +-550, oscillatory angles beyond 2^20):
+    in one wave, lanes take the factored sums or the point loop region by
+    region, and every region must still match the oracle."""
+    d, n = 8, 1500
+    a, u = genz_params(family, d, seed=77)
+    a = [scale * (1.0 + 0.1 * k) for k in range(d)]
+    L, H = random_regions(d, n, seed=79)
+    with pg.Workspace(d, 0) as ws:
+        ws.set_slices(1)
+        ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n)
+    oe, oerr, odim, fd, rs = O.eval_regions(d, family, O.params(a, u), L, H, n,
+                                            want_extra=True)
+    if family == "oscillatory":
+        # the angle sums (about 1e6 here) carry 2^-53 of themselves on both
+        # sides: the estimates agree within that conditioning; the error
+        # estimates (differences of nearly equal rule sums) and the split
+        # axes amplify it and are not compared
+        scale_r = O.est_scale(rs)
+        cond = 8 * EPS * sum(a)
+        assert np.all(np.abs(ge - oe) <= (1e-13 + cond) * scale_r + 1e-300)
+        assert np.all(np.isfinite(gerr)) and np.all(gerr >= 0)
+        return
+    check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs,
+                        centred_cond(family, a))
+
+
 @pytest.mark.parametrize("d", [4, 6, 7])
 def test_eval_all_families_remaining_dims(pg, gpu, d):
     n = 2000
