@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "pagani.h"
+#include "ws_internal.hpp"
 
 struct pagani_batch {
   int device = 0;
@@ -234,16 +235,24 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
       }
       return;
     }
-    // Items come in decreasing dimension, so once a worker moves on to a
-    // smaller d it never uses the previous workspace again in this call: its
-    // pool is released, and idle pools never crowd out the running one.
+    // One pool per worker: when a worker moves to another dimension (items
+    // come in decreasing dimension), its other workspaces hand their
+    // buffers to that dimension's workspace and release the rest, so idle
+    // pools never crowd out the running one and the pool grows once per
+    // worker instead of once per dimension.
     int last_d = 0;
     for (std::size_t k; (k = next.fetch_add(1)) < count;) {
       const std::size_t i = order[k];
       const int d = items[i].f.ndim;
-      if (last_d && d != last_d && last_d >= PAGANI_MIN_DIM && last_d <= PAGANI_MAX_DIM &&
-          b->ws[w][last_d])
-        (void)pagani_ws_trim(b->ws[w][last_d]);
+      if (d != last_d && d >= PAGANI_MIN_DIM && d <= PAGANI_MAX_DIM) {
+        pagani_ws*& to = b->ws[w][d];
+        if (!to) to = pagani_ws_create(d, b->device);  // failure: run_item reports it
+        for (int x = PAGANI_MIN_DIM; x <= PAGANI_MAX_DIM; ++x) {
+          if (x == d || !b->ws[w][x]) continue;
+          if (to) (void)pagani::internal::pool_handover(b->ws[w][x], to);
+          (void)pagani_ws_trim(b->ws[w][x]);
+        }
+      }
       last_d = d;
       codes[i] = run_item(b, w, items[i], &results[i]);
     }

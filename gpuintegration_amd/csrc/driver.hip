@@ -40,6 +40,28 @@ void harvest_compact(pagani_ws* ws)
   }
 }
 
+int pool_handover(pagani_ws* from, pagani_ws* to)
+{
+  if (!from || !to || from == to) return PAGANI_EBADARG;
+  if (from->susp.held || to->susp.held)
+    return fail(to, PAGANI_EBADARG, "a suspended run holds the region pool");
+  if (from->device != to->device) return fail(to, PAGANI_EBADARG, "workspaces on two devices");
+  if (hipSetDevice(from->device) != hipSuccess) return fail(to, PAGANI_EHIP, "hipSetDevice");
+  HIP_TRY(from, hipStreamSynchronize(from->stream));
+  HIP_TRY(to, hipStreamSynchronize(to->stream));
+  harvest_compact(from);
+  harvest_compact(to);
+  for_each_pool_pair(from, to, [](auto& a, auto& b) {
+    if (a.cap > b.cap) std::swap(a, b);
+  });
+  for (pagani_ws* w : {from, to}) {
+    w->live_n = 0;
+    w->live_evaluated = false;
+    w->snap_n = 0;
+  }
+  return PAGANI_OK;
+}
+
 // ---------------------------------------------------------------------------
 // evaluation launch data
 

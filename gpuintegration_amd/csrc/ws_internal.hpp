@@ -220,6 +220,38 @@ inline void release(DevBuf<T>& b)
   b.cap = 0;
 }
 
+// Every growable buffer of two workspaces, pairwise (same field, same type).
+template <class F>
+inline void for_each_pool_pair(pagani_ws* a, pagani_ws* b, F&& f)
+{
+  for (int i = 0; i < 2; ++i) {
+    f(a->L[i], b->L[i]);
+    f(a->H[i], b->H[i]);
+  }
+  f(a->est, b->est);
+  f(a->err0, b->err0);
+  f(a->err1, b->err1);
+  f(a->pest, b->pest);
+  f(a->perr, b->perr);
+  f(a->dim, b->dim);
+  f(a->active, b->active);
+  f(a->flags, b->flags);
+  f(a->part, b->part);
+  f(a->fpart, b->fpart);
+  f(a->cnt, b->cnt);
+  f(a->fcnt, b->fcnt);
+  f(a->off, b->off);
+  f(a->foff, b->foff);
+  f(a->snapL, b->snapL);
+  f(a->snapH, b->snapH);
+}
+
+// Move the larger of each pair of pool buffers from `from` to `to` (both on
+// one device, neither suspended); both workspaces' region lists are
+// dropped.  The batch passes one worker's pool on to the workspace of the
+// next dimension instead of freeing it and growing a new one.
+int pool_handover(pagani_ws* from, pagani_ws* to);
+
 // Every growable buffer of the workspace, for trimming and accounting.
 template <class F>
 inline void for_each_pool(pagani_ws* ws, F&& f)
