@@ -136,6 +136,14 @@ struct is_masked<F, std::void_t<decltype(F::masked)>> : std::bool_constant<F::ma
 // evaluated as products: the centre's value outer(S0) times one factor per
 // changed axis, so a point of sets 5-8 costs a multiply instead of an outer
 // evaluation.  Regions that fail factor_safe take the generic loop.
+// An exponential integrand may give the centre's value by `double
+// centre_value(double s0)` (at a wider range than `outer`); else outer(s0).
+template <class F, class = void>
+struct has_centre_value : std::false_type {};
+template <class F>
+struct has_centre_value<F, std::void_t<decltype(std::declval<const F&>().centre_value(0.0))>>
+    : std::true_type {};
+
 template <class F, class = void>
 struct is_exponential : std::false_type {};
 template <class F>
@@ -270,7 +278,10 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
   for (int j = 1; j < D; ++j) S0 = kExp ? S0 + T0[j] : Op::join(S0, T0[j]);
   if constexpr (kExp) {
     if (!f.factor_safe(S0)) return false;
-    S0 = f.outer(S0);
+    if constexpr (has_centre_value<F>::value)
+      S0 = f.centre_value(S0);
+    else
+      S0 = f.outer(S0);
   }
   // term changes (differences or ratios) at g = +lambda (x = c - lambda h)
   // and g = -lambda

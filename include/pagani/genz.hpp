@@ -185,6 +185,19 @@ struct CornerPeak {
 template <class Real>
 constexpr double kFactorRange = sizeof(Real) == 8 ? 650.0 : 80.0;
 
+namespace math {
+// e^x (x <= 709) for the product forms of the exponential families: at
+// the integrand's precision where it holds the value (FP32: |x| <= 80),
+// else in FP64, so that an FP32 integrand's centre values and factors stay
+// normal over the FP64 range of the products.
+__device__ __forceinline__ double exp_factor(double x, double) { return exp_bounded(x); }
+__device__ __forceinline__ double exp_factor(double x, float)
+{
+  return __builtin_fabs(x) <= 80.0 ? static_cast<double>(::expf(static_cast<float>(x)))
+                                   : exp_bounded(x);
+}
+}  // namespace math
+
 // exp(-sum a_i^2 (x_i - u_i)^2)
 template <class Real, int D>
 struct Gaussian {
@@ -227,14 +240,13 @@ struct Gaussian {
   }
   // outer(s + d) = outer(s) e^-d.  The terms are non-negative, so in a
   // region whose centre sums to s0 every factor is at most e^s0 and every
-  // point's value at most 1: s0 bounded well inside the exponent range keeps
-  // the centre's value, each factor and every partial product normal.
+  // point's value at most 1: s0 bounded well inside the FP64 exponent range
+  // keeps the centre's value, each factor and every partial product normal
+  // (an FP32 integrand takes those FP32 cannot hold in FP64).
   static constexpr bool exponential = true;
-  __device__ double factor(double d) const
-  {
-    return static_cast<double>(math::exp_bounded(static_cast<Real>(-d)));
-  }
-  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<Real>; }
+  __device__ double factor(double d) const { return math::exp_factor(-d, Real()); }
+  __device__ double centre_value(double s0) const { return math::exp_factor(-s0, Real()); }
+  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<double>; }
 };
 
 // exp(-sum a_i |x_i - u_i|)
@@ -270,11 +282,9 @@ struct C0 {
   }
   // outer(s + d) = outer(s) e^-d, non-negative terms (see Gaussian)
   static constexpr bool exponential = true;
-  __device__ double factor(double d) const
-  {
-    return static_cast<double>(math::exp_bounded(static_cast<Real>(-d)));
-  }
-  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<Real>; }
+  __device__ double factor(double d) const { return math::exp_factor(-d, Real()); }
+  __device__ double centre_value(double s0) const { return math::exp_factor(-s0, Real()); }
+  __device__ bool factor_safe(double s0) const { return s0 <= kFactorRange<double>; }
 };
 
 // 0 if any x_i > u_i, else exp(sum a_i x_i)
