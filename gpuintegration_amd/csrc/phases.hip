@@ -269,16 +269,18 @@ __global__ void __launch_bounds__(kThreads)
       bcnt[b - 1][tid] += 1u;
     }
   };
-  // Grid-stride order; four loads are issued before their values are
-  // consumed (in the same order) so more bytes are in flight per wave.
+  // Grid-stride order; sixteen loads are issued before their values are
+  // consumed (in the same order): the LDS buckets leave room for three
+  // workgroups per CU, so each lane keeps more bytes in flight.
+  constexpr int kAhead = 16;
   const std::size_t stride = static_cast<std::size_t>(gridDim.x) * kThreads;
   std::size_t t = static_cast<std::size_t>(blockIdx.x) * kThreads + tid;
-  for (; t + 3 * stride < n; t += 4 * stride) {
-    double r[4];
+  for (; t + (kAhead - 1) * stride < n; t += kAhead * stride) {
+    double r[kAhead];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) r[i] = err[t + i * stride];
+    for (int i = 0; i < kAhead; ++i) r[i] = err[t + i * stride];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) add(r[i]);
+    for (int i = 0; i < kAhead; ++i) add(r[i]);
   }
   for (; t < n; t += stride) add(err[t]);
   // suffix sums: candidate k counts the buckets k..K-2
