@@ -561,6 +561,10 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     HIP_TRY(ws, hipMemcpyAsync(ws->H[ws->cur].p, init->H, n * d * sizeof(double),
                                hipMemcpyDeviceToDevice, ws->stream));
   } else {
+    // a fresh uniform start always begins in pair 0, so repeated
+    // integrations alternate the ping-pong pairs identically and a pool
+    // grown by the first finds every buffer at its size in the next
+    ws->cur = 0;
     std::size_t k = init_k ? init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
     n = 1;
     for (int j = 0; j < d; ++j) {
