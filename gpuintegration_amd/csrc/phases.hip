@@ -76,7 +76,8 @@ __device__ __forceinline__ std::uint32_t block_count(std::uint32_t c, std::uint3
 
 // Write the chunk record: four DD sums and a count.
 __device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::uint32_t c,
-                                             double* part, std::uint32_t* cnt)
+                                             double* part, std::uint32_t* cnt,
+                                             std::size_t chunk = blockIdx.x)
 {
   __shared__ DD sh[kThreads / 64];
   __shared__ std::uint32_t shc[kThreads / 64];
@@ -86,7 +87,7 @@ __device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::ui
   s3 = block_reduce(s3, sh);
   c = block_count(c, shc);
   if (threadIdx.x == 0) {
-    double* p = part + static_cast<std::size_t>(blockIdx.x) * kPartDoubles;
+    double* p = part + chunk * kPartDoubles;
     p[0] = s0.hi;
     p[1] = s0.lo;
     p[2] = s1.hi;
@@ -95,7 +96,7 @@ __device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::ui
     p[5] = s2.lo;
     p[6] = s3.hi;
     p[7] = s3.lo;
-    if (cnt) cnt[blockIdx.x] = c;
+    if (cnt) cnt[chunk] = c;
   }
 }
 
@@ -120,9 +121,10 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
                                              const double* __restrict__ pest, double epsrel,
                                              int relerr, std::uint8_t* __restrict__ active,
                                              double* __restrict__ part,
-                                             std::uint32_t* __restrict__ cnt)
+                                             std::uint32_t* __restrict__ cnt,
+                                             std::size_t chunk = blockIdx.x)
 {
-  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  const std::size_t base = chunk * kChunk;
   const std::size_t h = n / 2;
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
@@ -160,7 +162,7 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
       }
     }
   }
-  write_record(se, sr, sae, sar, c, part, cnt);
+  write_record(se, sr, sae, sar, c, part, cnt, chunk);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -169,7 +171,12 @@ __global__ void __launch_bounds__(kThreads)
              std::uint8_t* __restrict__ active, double* __restrict__ part,
              std::uint32_t* __restrict__ cnt)
 {
-  refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt);
+  // workgroups alternate between the two halves of the list, so a region's
+  // sibling (n/2 away) and their parent are read by workgroups running at
+  // the same time and come from the caches the second time
+  const std::size_t b = blockIdx.x, half = (gridDim.x + 1) / 2;
+  const std::size_t chunk = (b & 1) ? half + (b >> 1) : (b >> 1);
+  refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt, chunk);
 }
 
 __global__ void __launch_bounds__(kThreads)
