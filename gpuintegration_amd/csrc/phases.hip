@@ -171,11 +171,23 @@ __global__ void __launch_bounds__(kThreads)
              std::uint8_t* __restrict__ active, double* __restrict__ part,
              std::uint32_t* __restrict__ cnt)
 {
-  // workgroups alternate between the two halves of the list, so a region's
-  // sibling (n/2 away) and their parent are read by workgroups running at
-  // the same time and come from the caches the second time
-  const std::size_t b = blockIdx.x, half = (gridDim.x + 1) / 2;
-  const std::size_t chunk = (b & 1) ? half + (b >> 1) : (b >> 1);
+  // A region's sibling lies n/2 away: workgroups are paired so that a
+  // chunk and its sibling chunk are read by workgroups running at the same
+  // time on the same XCD (workgroups are dealt to the 8 XCDs round robin,
+  // so b and b + 8 share one), and the second read of the siblings and
+  // their parents comes from that XCD's L2.  In groups of 16 workgroups, the
+  // first 8 take 8 chunks of the first half and the next 8 their sibling
+  // chunks; the chunks left over alternate between the halves.
+  const std::size_t b = blockIdx.x, nch = gridDim.x;
+  const std::size_t half = (nch + 1) / 2, groups = (nch - half) / 8;
+  std::size_t chunk;
+  if (b < 16 * groups) {
+    const std::size_t g = b / 16, r = b % 16;
+    chunk = r < 8 ? g * 8 + r : half + g * 8 + (r - 8);
+  } else {
+    const std::size_t t = b - 16 * groups;
+    chunk = (t & 1) ? half + 8 * groups + (t >> 1) : 8 * groups + (t >> 1);
+  }
   refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt, chunk);
 }
 
