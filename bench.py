@@ -9,18 +9,28 @@ and resident on the device; the whole loop (evaluation, refinement,
 classification, filtering, splitting) is inside the timed region.
 value = regions evaluated per second.
 
-Other configurations (each prints its own JSON line):
-  --config 1  5D oscillatory, epsrel 1e-3: the sequential Cuhre-equivalent on
-              one host core (max 1e6 evaluations) beside the GPU
-  --config 3  mixed precision: 6D corner peak and product peak, epsrel
-              1e-3..1e-7, FP32 integrand vs FP64, error vs analytic per run
-  --config 5  memory-bound: 8D cos(sum (i+1) x_i) and 8D discontinuous at
-              epsrel 1e-8 with the library's own memory model
+The headline line also carries, each timed in the same run:
+  generic  the same integral through the C++ drop-in header with the
+           integrand written as a plain functor (only operator(), the
+           reference's formula): the kernel's point-by-point path
+  mixed    FP32 integrand + FP64 sums beside FP64 at epsrel 1e-5
+  mcubes   the m-Cubes VEGAS integrator on the same integrand
+  batch    the 512-integral Genz batch (family and d in {3..8} drawn,
+           epsrel log-uniform in [1e-6, 1e-3]) from one call
+  large    8D cos(sum (i+1) x_i) and 8D discontinuous at epsrel 1e-8 with
+           the library's own memory model (pools of 2e8-5e8 regions)
+  plain_cpu_vs_gpu  5D oscillatory at epsrel 1e-3: the sequential
+           Cuhre-equivalent on one host core beside the GPU
+  cpu_baseline  the oracle's whole PAGANI loop (OpenMP, all host threads)
+           on the headline integral to a stated epsrel cap
+(--quick leaves out batch, large, plain_cpu_vs_gpu, mcubes and the CPU
+baselines.)  Other configurations print their own line: --config 1, 3, 4, 5.
 
-The headline line also carries "mcubes": the m-Cubes VEGAS integrator on the
-same 8D Gaussian (1e9 calls per iteration), samples/s and its kernel's
-algorithmic FP64 rate (--no-mcubes leaves it out, --mcubes-only runs it
-alone).
+The roofline's per-region figures (executed FP64 FLOPs, VALU instructions,
+HBM bytes) are counter measurements read from
+profiles/eval_gaussian8d_counters.json; the launch durations are measured
+here with HIP events on the workspace's stream.  A fraction above 1 means
+those figures no longer describe the kernel, and the run fails.
 """
 import argparse
 import json
@@ -34,25 +44,33 @@ sys.path.insert(0, ROOT)
 METRIC = "regions/s + integrals/s to epsrel 1e-6 (8D Genz), mixed vs FP64, 1 GPU"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per region of the 8D Gaussian evaluation kernel, from two
-# rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over `bench.py
-# --eval-only`, each scaled by what the counter reads for a 1 GiB device
-# copy: 128.20 B read + 17.00 B written (algorithmic: 128 + 17).
-EVAL_HBM_BYTES_PER_REGION_8D = 145.20
-# FP64 work the 8D Gaussian evaluation kernel executes per region, from one
-# rocprofv3 counter pass over `bench.py --eval-only` (SQ_INSTS_VALU_FMA_F64,
-# _MUL_F64, _ADD_F64 and SQ_INSTS_VALU, one lane per region, every lane
-# active): 1366 FMA + 233 MUL + 128 ADD = 3093 FLOP and 2221 VALU
-# instructions.  The algorithmic count (90770 FLOP) charges every one of the
-# 1105 points a full d-axis evaluation; the kernel evaluates the Gaussian as
-# a product over axes -- the centre's value times one factor e^-(change of
-# a_j^2 (x_j - u_j)^2) per changed axis, 81 exponentials per region instead
-# of 1105, and each set's sum over its sign combinations factored per axis
-# -- so the algorithmic rate would exceed the FP64 peak: the roofline uses
-# the executed FLOPs.
-EVAL_EXEC_FLOP_PER_REGION_8D = 3093
-EVAL_VALU_PER_REGION_8D = 2221
 VALU_ISSUE_PEAK = 256 * 2.4e9  # wave-instructions per second (one per CU per clock)
+COUNTERS_PATH = os.path.join(ROOT, "profiles", "eval_gaussian8d_counters.json")
+
+
+def load_counters():
+    with open(COUNTERS_PATH) as f:
+        return json.load(f)
+
+
+def check_fraction(name, frac):
+    """A roofline fraction above one cannot be measured: the per-region
+    counter figures are stale for the kernel that ran."""
+    if frac is not None and frac > 1.0:
+        raise SystemExit(f"bench.py: {name} = {frac:.3f} > 1: the counter figures in "
+                         f"{COUNTERS_PATH} do not describe this kernel; measure them again")
+    return frac
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _sync():
@@ -130,45 +148,105 @@ def phase_report(W, fam, d, res, dt, st, tr, steps):
     }
 
 
-def cpu_baseline(fam, a, u, budget_s):
-    """The oracle's region evaluation on the host cores: repeated passes over
-    the workload's first iteration (the 4^8 uniform split) for ~budget_s."""
+def cpu_baseline(fam, a, u, epsrel):
+    """The oracle's whole PAGANI loop (evaluation OpenMP-parallel over the
+    host threads, the list phases and the classifier sequential) on the
+    headline integral from the same 4^8 start, to `epsrel` (the headline's
+    1e-6 would take the host about 15 minutes).  Regions evaluated per
+    second, like the headline value."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_api as O
     d = len(a)
-    L, H, n = O.uniform_split(d, 4)
     prm = O.params(a, u)
-    O.eval_regions(d, fam, prm, L, H, 4096)  # page in
-    done, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        O.eval_regions(d, fam, prm, L, H, n)
-        done += n
+    t0 = time.perf_counter()
+    r, tr = O.integrate(d, fam, prm, epsrel, 1e-20, init_k=4, trace_cap=800)
     dt = time.perf_counter() - t0
+    regions = int(sum(int(v) for v in tr["n"]))
     return {
-        "value": done / dt,
+        "value": regions / dt,
         "unit": "regions/s",
         "cores": int(O.LIB.orc_num_threads()),
         "kind": "port",
-        "sample": f"oracle rule evaluation (OpenMP, {O.LIB.orc_num_threads()} threads) of the "
-                  f"workload's first iteration, {n} regions x {done // n} passes in {dt:.1f} s",
+        "cpu": cpu_model(),
+        "sample": f"oracle PAGANI loop ({O.LIB.orc_num_threads()} OpenMP threads) on the same "
+                  f"8D Gaussian from the 4^8 split to epsrel {epsrel:g}: {regions} regions in "
+                  f"{len(tr['n'])} iterations, {dt:.1f} s (status {r.status})",
+        "estimate": r.estimate, "errorest": r.errorest,
     }
+
+
+def generic_leg(args, W, counters, steps=2, warmup=1):
+    """The headline integral through the C++ header with a plain functor
+    (libpagani_header_bench.so, compiled as a user translation unit)."""
+    import ctypes
+    import gpuintegration_amd as pg
+    pg.load()
+    lib = ctypes.CDLL(os.path.join(ROOT, "gpuintegration_amd", "libpagani_header_bench.so"))
+    fam, a, u = W.gaussian_8d(args.seed)
+    d = len(a)
+    A = (ctypes.c_double * 8)(*a)
+    U = (ctypes.c_double * 8)(*u)
+    res = pg._Result()
+    st = pg.Stats()
+    secs = ctypes.c_double(0)
+    rc = lib.pagani_header_bench_gaussian8(A, U, ctypes.c_double(args.epsrel),
+                                           ctypes.c_double(1e-20), steps, warmup,
+                                           ctypes.byref(res), ctypes.byref(st), ctypes.byref(secs))
+    if rc != 0:
+        raise SystemExit(f"bench.py: header bench failed with {rc}")
+    st = st.as_dict()
+    dt = secs.value
+    host_regions = st["regions_evaluated"] - st["device_regions"]
+    host_launches = max(st["eval_launches"] - st["device_iterations"], 1)
+    eval_s = st["eval_ms"] / 1e3
+    alg = W.eval_flops_per_region(fam, d)
+    out = {
+        "workload": "same integral; Workspace<double, 8>::integrate through the C++ header, "
+                    "integrand a plain functor exp(-sum a_i^2 (x_i - u_i)^2) (operator() only)",
+        "regions_per_s": st["regions_evaluated"] / dt,
+        "integrals_per_s": steps / dt,
+        "ms_per_integral": dt / steps * 1e3,
+        "estimate": res.estimate, "errorest": res.errorest, "status": res.status,
+        "nregions": res.nregions,
+        "eval_ms_per_integral": st["eval_ms"] / steps,
+        "avg_eval_launch_ms": st["eval_ms"] / host_launches,
+        "eval_regions_per_s": host_regions / eval_s if eval_s else None,
+        "algorithmic_flops_per_region": alg,
+        "algorithmic_tflops": alg * host_regions / eval_s / 1e12 if eval_s else None,
+    }
+    out["algorithmic_frac_fp64_peak"] = check_fraction(
+        "generic algorithmic FP64 fraction",
+        out["algorithmic_tflops"] / FP64_PEAK_TFLOPS if eval_s else None)
+    pc = counters.get("plain", {})
+    if pc.get("exec_fp64_flop_per_region") and eval_s:
+        ex = pc["exec_fp64_flop_per_region"] * host_regions / eval_s / 1e12
+        out["executed_tflops"] = ex
+        out["executed_frac_fp64_peak"] = check_fraction("generic executed FP64 fraction",
+                                                        ex / FP64_PEAK_TFLOPS)
+        out["valu_issue_frac"] = check_fraction(
+            "generic VALU issue fraction",
+            pc["valu_insts_per_region"] / 64 * host_regions / eval_s / VALU_ISSUE_PEAK)
+    return out
 
 
 def config2(args, pg, W):
     fam, a, u = W.gaussian_8d(args.seed)
     d = len(a)
     exact = W.gaussian_exact(a, u)
+    counters = load_counters()
+    fc = counters["factored"]
     res, dt, st, tr = run_gpu(pg, fam, a, u, args.epsrel, args.steps, args.warmup, pg.FP64,
                               init_k=4)
     rep = phase_report(W, fam, d, res, dt, st, tr, args.steps)
     regions = st["regions_evaluated"]
     # the roofline covers the host-driven evaluation launches (timed one by
-    # one); the device loop's small iterations are reported beside it
+    # one with HIP events on the workspace's stream); the device loop's small
+    # iterations are reported beside it
     host_regions = regions - st["device_regions"]
     host_launches = st["eval_launches"] - st["device_iterations"]
     eval_s = st["eval_ms"] / 1e3
-    achieved = EVAL_EXEC_FLOP_PER_REGION_8D * host_regions / eval_s / 1e12 if eval_s else 0.0
-    valu_frac = (EVAL_VALU_PER_REGION_8D / 64 * host_regions / eval_s / VALU_ISSUE_PEAK
+    achieved = fc["exec_fp64_flop_per_region"] * host_regions / eval_s / 1e12 if eval_s else 0.0
+    valu_frac = (fc["valu_insts_per_region"] / 64 * host_regions / eval_s / VALU_ISSUE_PEAK
                  if eval_s else 0.0)
     out = {
         "metric": METRIC,
@@ -202,28 +280,31 @@ def config2(args, pg, W):
         "max_regions": st["max_regions"],
         "phase_ms_per_step": rep["phase_ms_per_integral"],
         "classify_passes_per_step": rep["classify_passes_per_integral"],
-        "noneval_gbs": rep["noneval_gbs"],
+        "noneval_gbs_algorithmic": rep["noneval_gbs"],
         "roofline": {
             "bound": "fp64-valu",
-            "kernel": "pagani::eval_kernel<8, Gaussian<double,8>, double>",
+            "kernel": fc["kernel"],
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "flops_per_region": EVAL_EXEC_FLOP_PER_REGION_8D,
-            "flops_per_region_kind": "executed FP64 FLOPs (counter-measured)",
+            "frac": check_fraction("roofline.frac", achieved / FP64_PEAK_TFLOPS),
+            "flops_per_region": fc["exec_fp64_flop_per_region"],
+            "flops_per_region_kind": "executed FP64 FLOPs (counter-measured, "
+                                     "profiles/eval_gaussian8d_counters.json)",
             "algorithmic_flops_per_region": W.eval_flops_per_region(fam, d),
             "algorithmic_tflops": rep["eval_tflops"],
-            "valu_issue_frac": valu_frac,
+            "valu_issue_frac": check_fraction("roofline.valu_issue_frac", valu_frac),
             "eval_launches": host_launches,
             "avg_launch_ms": st["eval_ms"] / max(host_launches, 1),
             "avg_regions_per_launch": host_regions / max(host_launches, 1),
-            "traffic": EVAL_HBM_BYTES_PER_REGION_8D * host_regions / max(host_launches, 1),
+            "traffic": fc["hbm_bytes_per_region"] * host_regions / max(host_launches, 1),
             "traffic_unit": "bytes per launch (counter-measured per-region HBM bytes x "
                             "average regions per launch)",
             "algorithmic_bytes_per_launch": (16 * d + 17) * host_regions / max(host_launches, 1),
         },
     }
+    if not args.no_generic:
+        out["generic"] = generic_leg(args, W, counters)
     if not args.no_mixed:
         # mixed precision beside FP64 at one stated tolerance both can certify
         eps_m = args.mixed_epsrel
@@ -243,10 +324,15 @@ def config2(args, pg, W):
                          abs_err_vs_analytic=abs(r64.estimate - exact), errorest=r64.errorest,
                          nregions=r64.nregions),
         }
+    if args.quick:
+        return out
     if not args.no_mcubes:
         out["mcubes"] = mcubes_leg(args, pg, W)
+    out["batch"] = config4(args, pg, W, leg=True)
+    out["large"] = config5(args, pg, W)
+    out["plain_cpu_vs_gpu"] = config1(args, pg, W)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(fam, a, u, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(fam, a, u, args.cpu_epsrel)
     return out
 
 
@@ -259,15 +345,19 @@ def config1(args, pg, W):
     prm = O.params(a, u)
     exact = W.exact(fam, a, u)
     # CPU baseline: the oracle's sequential Cuhre-equivalent on one core
+    threads = O.LIB.orc_num_threads()
     O.LIB.orc_set_num_threads(1)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        rc = O.cuhre_seq(d, fam, prm, 1e-3, 1e-20, 1_000_000)
-        reps += 1
-        if time.perf_counter() - t0 > 3.0:
-            break
-    cdt = (time.perf_counter() - t0) / reps
+    try:
+        t0 = time.perf_counter()
+        reps = 0
+        while True:
+            rc = O.cuhre_seq(d, fam, prm, 1e-3, 1e-20, 1_000_000)
+            reps += 1
+            if time.perf_counter() - t0 > 3.0:
+                break
+        cdt = (time.perf_counter() - t0) / reps
+    finally:
+        O.LIB.orc_set_num_threads(threads)
     res, dt, st, tr = run_gpu(pg, fam, a, u, 1e-3, args.steps, args.warmup, pg.FP64,
                               epsabs=1e-20)
     rep = phase_report(W, fam, d, res, dt, st, tr, args.steps)
@@ -334,19 +424,22 @@ def config5(args, pg, W):
             "runs": out}
 
 
-def config4(args, pg, W):
+def config4(args, pg, W, leg=False):
     """512 independent integrals (family, d in {3..8}, epsrel log-uniform in
-    [1e-6, 1e-3]) from one pagani_integrate_batch call."""
+    [1e-6, 1e-3]) from one pagani_integrate_batch call.  As a leg of the
+    headline line: one timed call after a 16-item warm-up."""
     import collections
     items = W.batch(args.batch_count, args.seed)
+    steps = 1 if leg else args.steps
+    warmup = 1 if leg else args.warmup
     b = pg.Batch(0, args.workers)
     b.set_timing(True)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         b.integrate(items[:16])
     _sync()
     b.reset_stats()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         res, codes = b.integrate(items)
     _sync()
     dt = time.perf_counter() - t0
@@ -364,11 +457,11 @@ def config4(args, pg, W):
         row["count"] += 1
         row["converged"] += r.status == 0
         row["rel_err_within_epsrel"] += ok
-    n = len(items) * args.steps
+    n = len(items) * steps
     return {
         "metric": "integrals/s, 512-integral Genz batch (FP64)",
-        "value": n / dt, "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "value": n / dt, "unit": "integrals/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": dt / steps * 1e3,
         "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.batch_count} Genz integrals, family ~ U(6), d ~ U{{3..8}}, "
                                "parameters per family, epsrel = 10^(-3-3U), epsabs 1e-20",
@@ -376,8 +469,9 @@ def config4(args, pg, W):
         "regions_per_s": st["regions_evaluated"] / dt,
         "converged": conv, "rel_err_within_epsrel": within,
         "codes_negative": sum(1 for c in codes if c < 0),
-        "phase_ms_per_step": {k: st[k] / args.steps for k in
-                              ("eval_ms", "refine_ms", "compact_ms", "classify_ms")},
+        "phase_ms_per_step": {k: st[k] / steps for k in
+                              ("eval_ms", "refine_ms", "compact_ms", "classify_ms",
+                               "device_ms", "alloc_ms")},
         "by_family": fam_rows,
     }
 
@@ -494,7 +588,13 @@ def main():
     ap.add_argument("--mixed-epsrel", type=float, default=1e-5)
     ap.add_argument("--large-epsrel", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-epsrel", type=float, default=1e-3,
+                    help="epsrel cap of the oracle's loop in the CPU baseline")
+    ap.add_argument("--quick", action="store_true",
+                    help="headline, generic and mixed legs only")
+    ap.add_argument("--no-generic", action="store_true")
+    ap.add_argument("--generic-only", action="store_true",
+                    help="the generic leg alone (for counter runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-mcubes", action="store_true")
@@ -513,6 +613,9 @@ def main():
         fn = eval_only
     if args.mcubes_only:
         fn = mcubes_leg
+    if args.generic_only:
+        def fn(args, pg, W):
+            return generic_leg(args, W, load_counters(), steps=1, warmup=0)
     print(json.dumps(fn(args, pg, W)), flush=True)
 
 

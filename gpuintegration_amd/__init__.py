@@ -118,6 +118,7 @@ class Stats(ctypes.Structure):
         ("device_iterations", ctypes.c_uint64),
         ("device_regions", ctypes.c_uint64),
         ("device_rounds", ctypes.c_uint64),
+        ("budget_fixes", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -225,6 +226,7 @@ _SIGS = {
     "pagani_ws_load_state": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "pagani_ws_write_regions_csv": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "pagani_ws_set_snapshot_iter": (ctypes.c_int, [_vp, ctypes.c_uint64]),
+    "pagani_ws_set_predict_split": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_snapshot_regions": (ctypes.c_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_vp),
                                                   ctypes.POINTER(_vp)]),
     "pagani_batch_create": (_vp, [ctypes.c_int, ctypes.c_int]),
@@ -501,6 +503,12 @@ class Workspace:
         has finished by then (None: never)."""
         v = (1 << 64) - 1 if it is None else int(it)
         self._check(self.lib.pagani_ws_set_snapshot_iter(self._ws, v))
+
+    def set_predict_split(self, on=True):
+        """Relerr classification by the predict_split rule in every iteration
+        (off before iteration 15 while nothing finished and the list is at
+        most 15M regions, on after), instead of the relerr argument."""
+        self._check(self.lib.pagani_ws_set_predict_split(self._ws, 1 if on else 0))
 
     def snapshot_size(self):
         n = _sz(0)

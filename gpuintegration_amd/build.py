@@ -78,6 +78,20 @@ def build_library(jobs=8):
     return LIB
 
 
+HEADER_BENCH_LIB = os.path.join(HERE, "libpagani_header_bench.so")
+
+
+def build_header_bench():
+    """The benchmark's generic leg: a plain functor through the C++ header,
+    compiled as a user translation unit against libpagani.so."""
+    src = os.path.join(CSRC, "header_bench.hip")
+    tmp = HEADER_BENCH_LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-L" + HERE, "-lpagani",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, HEADER_BENCH_LIB)
+    return HEADER_BENCH_LIB
+
+
 def build_tests():
     src = os.path.join(TESTS_CXX, "reftests.hip")
     if not os.path.exists(src):
@@ -95,7 +109,10 @@ def build_tests():
 
 def build_all():
     build_library()
-    build_tests()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=2) as ex:
+        futs = [ex.submit(build_header_bench), ex.submit(build_tests)]
+        for f in futs:
+            f.result()
 
 
 if __name__ == "__main__":

@@ -192,6 +192,11 @@ int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
       out->integrals += s.integrals;
       out->alloc_ms += s.alloc_ms;
       out->allocations += s.allocations;
+      out->device_ms += s.device_ms;
+      out->device_iterations += s.device_iterations;
+      out->device_regions += s.device_regions;
+      out->device_rounds += s.device_rounds;
+      out->budget_fixes += s.budget_fixes;
     }
   return PAGANI_OK;
 }

@@ -303,6 +303,14 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   }
   int rc;
   const int cur = ws->cur;
+  // From here on the live buffers are overwritten (and may be reallocated):
+  // a run the workspace held is gone, whether or not this load succeeds, so
+  // a failed load leaves nothing that a resume could continue.
+  ws->susp.held = false;
+  ws->live_n = 0;
+  ws->live_evaluated = false;
+  ws->snap_n = 0;
+  ws->trace.clear();
   if ((rc = ensure(ws, ws->L[cur], d * s.n, "region bounds")) ||
       (rc = ensure(ws, ws->H[cur], d * s.n, "region bounds")) ||
       (np && (rc = ensure(ws, ws->pest, np, "parent estimates"))))

@@ -517,7 +517,9 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     if (rc0) return rc0;
     st.epsrel = epsrel;
     st.epsabs = epsabs;
-    st.relerr = relerr;
+    // kRelerrPredictSplit: the flag of each iteration follows the rule of
+    // integrate<IntegT, predict_split=true> (Workspace.cuh:206-210)
+    st.relerr = ws->predict_split ? kRelerrPredictSplit : (relerr ? 1 : 0);
     for (int j = 0; j < d; ++j) {
       st.lows[j] = lows_in ? lows_in[j] : 0.0;
       st.highs[j] = highs_in ? highs_in[j] : 1.0;
@@ -614,6 +616,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     // it needs the host (or finishes).
     bool reduced = false;  // this iteration's sums are already in d_scal
     if (ws->device_loop && !ws->iter_limit && ws->snapshot_iter == ~std::uint64_t(0) &&
+        relerr != kRelerrPredictSplit &&
         n <= ph::kDevLoopMax && mem_ratio(n, d, total, mem_model) <= .1) {
       LoopVars v{it, n, has_parents, &cum, &clf};
       DevOutcome outcome;
@@ -657,9 +660,16 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
 
       // two-level error + relerr flags + the four region sums + counts
       errs = has_parents ? ws->err1.p : ws->err0.p;
+      // predict_split: relerr classification off while the list has at most
+      // 15M regions, before iteration 15 and before any region finished,
+      // on after that (Workspace.cuh:206-210, 415-420)
+      const int relerr_it =
+          relerr == kRelerrPredictSplit
+              ? ((n <= 15000000 && it < 15 && cum.nregions == 0) ? 0 : 1)
+              : relerr;
       record(ws, EV_REF0);
       HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
-                             has_parents ? ws->pest.p : nullptr, epsrel, relerr, ws->active.p,
+                             has_parents ? ws->pest.p : nullptr, epsrel, relerr_it, ws->active.p,
                              ws->part.p, ws->cnt.p));
       HIP_TRY(ws, launch_finalize(ws, n, true, false));
       record(ws, EV_REF1);
@@ -715,6 +725,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       bool recount = false;
       // fix_error_budget_overflow, Workspace.cuh:121-146
       if (cum.errorest + fin_err > std::fabs(cum.estimate + iter_est) * epsrel) {
+        ws->stats.budget_fixes++;
         HIP_TRY(ws, ph::fill_u8(ws->stream, n, ws->active.p, 1));
         fin_err = 0.;
         fin_est = 0.;
@@ -1148,6 +1159,13 @@ int pagani_integrate_genz_resume(pagani_ws* ws, const pagani_genz* f, int precis
   if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
   return integrate_impl(ws, genz_trampoline, &ctx, 0, 0, nullptr, nullptr, 0, nullptr, 0,
                         &ws->susp, nullptr, 0, out);
+}
+
+int pagani_ws_set_predict_split(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->predict_split = enable != 0;
+  return PAGANI_OK;
 }
 
 int pagani_ws_set_snapshot_iter(pagani_ws* ws, uint64_t iteration)

@@ -72,6 +72,10 @@ struct DevLoop {
   TraceRec trace[kDevTrace];
 };
 
+// LoopState::relerr value of a run whose relerr flag follows the
+// predict_split rule, iteration by iteration (0 and 1 are the caller's flag).
+constexpr int kRelerrPredictSplit = 2;
+
 // Everything the iteration loop carries from one iteration to the next,
 // besides the region list itself (L[cur], H[cur]: n regions) and the
 // parents' estimates (pest: n/2 when has_parents).  A run suspended at an
@@ -145,6 +149,8 @@ struct pagani_ws {
   // snapshot of the region list evaluated in iteration snapshot_iter when no
   // region had finished by then (Workspace.cuh:244-248, predict_split)
   std::uint64_t snapshot_iter = ~std::uint64_t(0);
+  // relerr classification by the predict_split rule (Workspace.cuh:206-210)
+  bool predict_split = false;
   pagani::internal::DevBuf<double> snapL, snapH;
   std::size_t snap_n = 0;
 };

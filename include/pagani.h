@@ -165,6 +165,7 @@ typedef struct pagani_stats {
   uint64_t device_iterations; /* iterations run by the device-resident loop */
   uint64_t device_regions;    /* regions it evaluated (included in regions_evaluated) */
   uint64_t device_rounds;     /* its host round trips */
+  uint64_t budget_fixes;      /* iterations that took fix_error_budget_overflow */
 } pagani_stats;
 
 /* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
@@ -304,6 +305,13 @@ int pagani_ws_write_regions_csv(pagani_ws* ws, const char* path);
  * uses 15); UINT64_MAX (the default) disables.  The copy stays in the
  * workspace (device SoA, *n regions) until the next snapshot. */
 int pagani_ws_set_snapshot_iter(pagani_ws* ws, uint64_t iteration);
+/* Workspace::integrate<IntegT, predict_split=true> also overrides the
+ * relerr classification flag every iteration (Workspace.cuh:206-210,
+ * 415-420): off while the list has at most 15M regions, the iteration is
+ * before 15 and no region has finished, on otherwise.  enable = 1 applies
+ * that rule to the following integrate calls instead of their
+ * relerr_classification argument (a suspended run keeps its rule). */
+int pagani_ws_set_predict_split(pagani_ws* ws, int enable);
 int pagani_ws_snapshot_regions(const pagani_ws* ws, size_t* n, const double** L,
                                const double** H);
 

@@ -630,7 +630,10 @@ __device__ __forceinline__ void eval_regions_wave(
       maxr = r;
       maxdim = j;
     }
-    cR[j] = lows[j] + (0.5 * (lo + up)) * range[j];
+    // one rounding in the map to user coordinates (an FMA), so the points
+    // are a fixed function of the region's bounds whatever the compiler
+    // contracts
+    cR[j] = __builtin_fma(0.5 * (lo + up), range[j], lows[j]);
     hR[j] = r * range[j];
     if constexpr (has_origin<F>::value) cR[j] -= f.origin(j);
     if constexpr (has_origin<F>::value && has_axis_scale<F>::value) {

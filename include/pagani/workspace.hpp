@@ -221,6 +221,7 @@ class Workspace {
     pagani_result r;
     // this overload has no caller-owned list to keep a predict_split snapshot in
     pagani_ws_set_snapshot_iter(ws_, ~std::uint64_t(0));
+    pagani_ws_set_predict_split(ws_, predict_split ? 1 : 0);
     const int rc = pagani_integrate_cb(ws_, &pagani::eval_callback<static_cast<int>(ndim), IntegT>,
                                        &f, epsrel, epsabs, vol.lows, vol.highs,
                                        relerr_classification ? 1 : 0, nullptr, 0, &r);
@@ -244,6 +245,7 @@ class Workspace {
     // predict_split: the list of iteration 15 if no region finished before
     // (Workspace.cuh:244-248)
     pagani_ws_set_snapshot_iter(ws_, predict_split ? 15 : ~std::uint64_t(0));
+    pagani_ws_set_predict_split(ws_, predict_split ? 1 : 0);
     const int rc = pagani_integrate_cb(ws_, &pagani::eval_callback<static_cast<int>(ndim), IntegT>,
                                        &f, epsrel, epsabs, vol.lows, vol.highs,
                                        relerr_classification ? 1 : 0, &init, 0, &r);

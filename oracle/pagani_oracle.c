@@ -499,6 +499,7 @@ static void eval_one(const rule_t* R, int fam, const double* prm,
     rs_out[12] = e[2];
     rs_out[13] = e[3];
     rs_out[14] = fdmag;
+    rs_out[15] = 0.0;
   }
   *avg = vol * s[0];
   *err = vol * ((5. * e[1] <= e[2] && 5. * e[2] <= e[3])
@@ -533,6 +534,316 @@ int orc_eval_regions(int d, int fam, const double* prm, const double* lows,
     eval_one(&R, fam, prm, lw, hg, lo, len, &est[t], &err[t], &dim[t],
              fdiff ? fdiff + (size_t)t * d : NULL,
              rsum ? rsum + (size_t)t * ORC_RSUM : NULL);
+  }
+  rule_free(&R);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Evaluation at the library kernel's own points, in extended precision.     */
+/*                                                                          */
+/* The restatement above maps every point as the reference does,            */
+/* x = lows + ((.5 + g) lo + (.5 - g) up) range (Sample.cuh:124-131).  The   */
+/* library kernel forms its coordinates differently (eval_kernel.hpp,        */
+/* eval_regions_wave): the region's centre c = fma(.5 (lo + up), range, lows) */
+/* and width h = (up - lo) range, minus the integrand's origin and times     */
+/* its axis scale when it declares them, and every point v = fma(-g, h, c). */
+/* Those are a few roundings away from the reference's x, and an integrand   */
+/* with a steep exponent (a Gaussian with a = 25) turns them into relative   */
+/* differences of 1e-12 in f.  This mode forms the points exactly as the     */
+/* kernel does (in double, bit for bit), and evaluates f at them and every  */
+/* rule sum in long double, so what is left of a difference from the kernel */
+/* is the kernel's own rounding of f and of its sums.                        */
+/* ------------------------------------------------------------------------ */
+
+/* origin and axis scale the library's Genz functors declare
+ * (include/pagani/genz.hpp); plain integrands have neither. */
+static void kform_shift(int fam, int d, const double* prm, int plain, double* org, double* scl)
+{
+  const double* a = prm;
+  const double* u = prm ? prm + d : NULL;
+  for (int j = 0; j < d; ++j) {
+    org[j] = 0.0;
+    scl[j] = 1.0;
+  }
+  if (plain) return;
+  switch (fam) {
+  case ORC_FAM_OSCILLATORY:
+  case ORC_FAM_CORNER_PEAK:
+    for (int j = 0; j < d; ++j) scl[j] = a[j];
+    break;
+  case ORC_FAM_PRODUCT_PEAK:
+  case ORC_FAM_GAUSSIAN:
+  case ORC_FAM_C0:
+    for (int j = 0; j < d; ++j) {
+      org[j] = u[j];
+      scl[j] = a[j];
+    }
+    break;
+  case ORC_FAM_DISCONTINUOUS:
+    for (int j = 0; j < d; ++j) org[j] = u[j];
+    break;
+  default:
+    break;
+  }
+}
+
+/* f at kernel coordinates v (centred and scaled Genz, or plain x), in long
+ * double.  The constants the library's functors hold in double (the
+ * oscillatory phase 2 pi u_0, the product peak's prod a_i^2, the
+ * discontinuous family's sum a_i u_i) are formed in double as they are. */
+/* *cond receives f's sensitivity to one rounding of each operation of its
+ * inner sum: |f| times the sum of the magnitudes that sum adds up (|sin|
+ * times them for the cosine), the error any FP64 evaluation of f carries in
+ * units of 2^-53. */
+static long double kform_f(int fam, int d, const double* prm, int plain, const double* v,
+                           long double* cond)
+{
+  const double* a = prm;
+  const double* u = prm ? prm + d : NULL;
+  long double s = 0.0L, p = 1.0L, m = 0.0L, f;
+  if (plain) {
+    switch (fam) {
+    case ORC_FAM_OSCILLATORY:
+      s = 2.0L * 3.14159265358979323846264338327950288L * u[0];
+      m = fabsl(s);
+      for (int i = 0; i < d; ++i) {
+        s += (long double)a[i] * v[i];
+        m += fabsl((long double)a[i] * v[i]);
+      }
+      *cond = fabsl(sinl(s)) * m;
+      return cosl(s);
+    case ORC_FAM_PRODUCT_PEAK:
+      for (int i = 0; i < d; ++i) {
+        const long double t = (long double)v[i] - u[i];
+        p *= 1.0L / (1.0L / ((long double)a[i] * a[i]) + t * t);
+        m += 4.0L + 2.0L * fabsl(t) * fmaxl(fabsl(v[i]), fabsl(u[i])) /
+                        (1.0L / ((long double)a[i] * a[i]) + t * t);
+      }
+      *cond = fabsl(p) * m;
+      return p;
+    case ORC_FAM_CORNER_PEAK:
+      s = 1.0L;
+      m = 1.0L;
+      for (int i = 0; i < d; ++i) {
+        s += (long double)a[i] * v[i];
+        m += fabsl((long double)a[i] * v[i]);
+      }
+      f = powl(s, -(long double)(d + 1));
+      *cond = fabsl(f) * (d + 1) * (1.0L + m / fabsl(s));
+      return f;
+    case ORC_FAM_GAUSSIAN:
+      for (int i = 0; i < d; ++i) {
+        const long double t = (long double)v[i] - u[i];
+        s += (long double)a[i] * a[i] * t * t;
+        m += (long double)a[i] * a[i] * fabsl(t) * (fabsl(t) + 2 * fmaxl(fabsl(v[i]), fabsl(u[i])));
+      }
+      f = expl(-s);
+      *cond = f * (m + 1.0L);
+      return f;
+    case ORC_FAM_C0:
+      for (int i = 0; i < d; ++i) {
+        const long double t = fabsl((long double)v[i] - u[i]);
+        s += (long double)a[i] * t;
+        m += (long double)a[i] * (t + fmaxl(fabsl(v[i]), fabsl(u[i])));
+      }
+      f = expl(-s);
+      *cond = f * (m + 1.0L);
+      return f;
+    case ORC_FAM_DISCONTINUOUS:
+      *cond = 0.0L;
+      for (int i = 0; i < d; ++i)
+        if (v[i] > u[i]) return 0.0L;
+      for (int i = 0; i < d; ++i) {
+        s += (long double)a[i] * v[i];
+        m += fabsl((long double)a[i] * v[i]);
+      }
+      f = expl(s);
+      *cond = f * (m + 1.0L);
+      return f;
+    default:
+      f = orc_integrand(fam, d, prm, v);
+      *cond = fabsl(f) * d;
+      return f;
+    }
+  }
+  switch (fam) {
+  case ORC_FAM_OSCILLATORY:
+    s = (long double)(2.0 * M_PI * u[0]);
+    m = fabsl(s);
+    for (int i = 0; i < d; ++i) {
+      s += v[i];
+      m += fabsl((long double)v[i]);
+    }
+    *cond = fabsl(sinl(s)) * m;
+    return cosl(s);
+  case ORC_FAM_PRODUCT_PEAK: {
+    double k2 = 1.0;
+    for (int i = 0; i < d; ++i) k2 *= a[i] * a[i];
+    for (int i = 0; i < d; ++i) p *= 1.0L + (long double)v[i] * v[i];
+    f = (long double)k2 / p;
+    *cond = f * (3.0L * d + 1.0L);
+    return f;
+  }
+  case ORC_FAM_CORNER_PEAK:
+    s = 1.0L;
+    m = 1.0L;
+    for (int i = 0; i < d; ++i) {
+      s += v[i];
+      m += fabsl((long double)v[i]);
+    }
+    f = powl(s, -(long double)(d + 1));
+    *cond = fabsl(f) * (d + 1) * (1.0L + m / fabsl(s));
+    return f;
+  case ORC_FAM_GAUSSIAN:
+    for (int i = 0; i < d; ++i) s += (long double)v[i] * v[i];
+    f = expl(-s);
+    *cond = f * (s + 1.0L);
+    return f;
+  case ORC_FAM_C0:
+    for (int i = 0; i < d; ++i) s += fabsl((long double)v[i]);
+    f = expl(-s);
+    *cond = f * (s + 1.0L);
+    return f;
+  case ORC_FAM_DISCONTINUOUS: {
+    double c = 0.0;
+    for (int i = 0; i < d; ++i) c += a[i] * u[i];
+    *cond = 0.0L;
+    for (int i = 0; i < d; ++i)
+      if (v[i] > 0.0) return 0.0L;
+    s = c;
+    m = fabsl((long double)c);
+    for (int i = 0; i < d; ++i) {
+      s += (long double)a[i] * v[i];
+      m += fabsl((long double)a[i] * v[i]);
+    }
+    f = expl(s);
+    *cond = f * (m + 1.0L);
+    return f;
+  }
+  default:
+    f = orc_integrand(fam, d, prm, v);
+    *cond = fabsl(f) * d;
+    return f;
+  }
+}
+
+static void eval_one_kform(const rule_t* R, int fam, const double* prm, int plain,
+                           const double* lows, const double* highs, const double* lo,
+                           const double* len, double* avg, double* err, int* bisect,
+                           double* fd_out, double* rs_out)
+{
+  const int d = R->d;
+  double c[ORC_MAXDIM], h[ORC_MAXDIM], org[ORC_MAXDIM], scl[ORC_MAXDIM], v[ORC_MAXDIM];
+  long double fv[1 + 4 * ORC_MAXDIM];
+  double vol = 1., jac = 1., maxr = 0.;
+  int maxdim = 0;
+  kform_shift(fam, d, prm, plain, org, scl);
+  for (int j = 0; j < d; ++j) {
+    const double up = lo[j] + len[j];
+    const double r = up - lo[j];
+    const double range = highs[j] - lows[j];
+    vol *= r;
+    jac = jac * range;
+    if (r > maxr) {
+      maxdim = j;
+      maxr = r;
+    }
+    c[j] = fma(0.5 * (lo[j] + up), range, lows[j]);
+    h[j] = r * range;
+    if (!plain) {
+      c[j] -= org[j];
+      c[j] *= scl[j];
+      h[j] *= scl[j];
+    }
+  }
+  long double s[ORC_NRULES] = {0, 0, 0, 0, 0};
+  long double A[ORC_NRULES] = {0, 0, 0, 0, 0};
+  long double C = 0.0L;
+  const int nfd = 1 + 4 * d;
+  for (int p = 0; p < R->F; ++p) {
+    const double* g = R->gen + (size_t)p * d;
+    for (int j = 0; j < d; ++j) v[j] = fma(-g[j], h[j], c[j]);
+    long double cp = 0.0L;
+    const long double fun = kform_f(fam, d, prm, plain, v, &cp) * jac;
+    C += fabsl(cp * jac * R->W[R->set[p]][0]);
+    if (p < nfd) fv[p] = fun;
+    const double* w = R->W[R->set[p]];
+    for (int r = 0; r < ORC_NRULES; ++r) {
+      s[r] += fun * w[r];
+      A[r] += fabsl(fun * w[r]);
+    }
+  }
+  const long double l1 = kLambda[0], l2 = kLambda[1];
+  const long double q = l2 / l1;
+  const long double ratio = q * q;
+  const long double base = fv[0] * 2 * (1 - ratio);
+  long double maxdiff = 0.;
+  long double fdmag = 0.;
+  int bis = maxdim;
+  for (int j = 0; j < d; ++j) {
+    const long double fp1 = fv[1 + 2 * j], fm1 = fv[2 + 2 * j];
+    const long double fp2 = fv[1 + 2 * d + 2 * j], fm2 = fv[2 + 2 * d + 2 * j];
+    const long double fd = fabsl(base + ratio * (fp1 + fm1) - (fp2 + fm2));
+    const long double mag = fabsl(base) + ratio * (fabsl(fp1) + fabsl(fm1)) + fabsl(fp2) +
+                            fabsl(fm2);
+    if (mag > fdmag) fdmag = mag;
+    if (fd_out) fd_out[j] = (double)fd;
+    if (fd > maxdiff) {
+      maxdiff = fd;
+      bis = j;
+    }
+  }
+  long double e[ORC_NRULES];
+  for (int r = 1; r < ORC_NRULES - 1; ++r) {
+    long double me = 0.;
+    for (int k = 0; k < ORC_NSETS; ++k)
+      me = fmaxl(me, fabsl(s[r + 1] + R->scale[k][r] * s[r]) * R->norm[k][r]);
+    e[r] = me;
+  }
+  if (rs_out) {
+    for (int r = 0; r < ORC_NRULES; ++r) {
+      rs_out[r] = (double)s[r];
+      rs_out[ORC_NRULES + r] = (double)A[r];
+    }
+    rs_out[10] = vol;
+    rs_out[11] = (double)e[1];
+    rs_out[12] = (double)e[2];
+    rs_out[13] = (double)e[3];
+    rs_out[14] = (double)fdmag;
+    rs_out[15] = (double)C;
+  }
+  *avg = (double)(vol * s[0]);
+  *err = (double)(vol * ((5. * e[1] <= e[2] && 5. * e[2] <= e[3])
+                             ? 1. * e[1]
+                             : 5. * fmaxl(fmaxl(e[1], e[2]), e[3])));
+  *bisect = bis;
+}
+
+int orc_eval_regions_kform(int d, int fam, const double* prm, int plain, const double* lows,
+                           const double* highs, size_t n, const double* L, const double* H,
+                           double* est, double* err, int* dim, double* fdiff, double* rsum)
+{
+  rule_t R;
+  if (rule_init(&R, d)) {
+    rule_free(&R);
+    return -1;
+  }
+  double lw[ORC_MAXDIM], hg[ORC_MAXDIM];
+  for (int j = 0; j < d; ++j) {
+    lw[j] = lows ? lows[j] : 0.0;
+    hg[j] = highs ? highs[j] : 1.0;
+  }
+  const long long nn = (long long)n;
+#pragma omp parallel for schedule(static) if (nn > 64)
+  for (long long t = 0; t < nn; ++t) {
+    double lo[ORC_MAXDIM], len[ORC_MAXDIM];
+    for (int j = 0; j < d; ++j) {
+      lo[j] = L[(size_t)j * n + (size_t)t];
+      len[j] = H[(size_t)j * n + (size_t)t];
+    }
+    eval_one_kform(&R, fam, prm, plain, lw, hg, lo, len, &est[t], &err[t], &dim[t],
+                   fdiff ? fdiff + (size_t)t * d : NULL, rsum ? rsum + (size_t)t * ORC_RSUM : NULL);
   }
   rule_free(&R);
   return 0;
@@ -851,6 +1162,48 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
                   size_t init_k, int user_regions, orc_result* out,
                   orc_trace* tr)
 {
+  return orc_integrate_ex(d, fam, prm, epsrel, epsabs, lows_in, highs_in, relerr, total_mem,
+                          init_k, user_regions, out, tr, NULL);
+}
+
+void orc_free(void* p) { free(p); }
+
+/* The final list goes to the options (ownership passes to the caller). */
+static void keep_list(orc_opts* o, int d, size_t n, double* L, double* H, double* est,
+                      double* err, const int* dim)
+{
+  o->list_n = n;
+  o->list_L = (double*)malloc(sizeof(double) * (n ? n * d : 1));
+  o->list_H = (double*)malloc(sizeof(double) * (n ? n * d : 1));
+  memcpy(o->list_L, L, sizeof(double) * n * d);
+  memcpy(o->list_H, H, sizeof(double) * n * d);
+  o->list_est = o->list_err = NULL;
+  o->list_dim = NULL;
+  if (est) {
+    o->list_est = (double*)malloc(sizeof(double) * (n ? n : 1));
+    o->list_err = (double*)malloc(sizeof(double) * (n ? n : 1));
+    o->list_dim = (int*)malloc(sizeof(int) * (n ? n : 1));
+    memcpy(o->list_est, est, sizeof(double) * n);
+    memcpy(o->list_err, err, sizeof(double) * n);
+    memcpy(o->list_dim, dim, sizeof(int) * n);
+  }
+}
+
+int orc_integrate_ex(int d, int fam, const double* prm, double epsrel,
+                     double epsabs, const double* lows_in,
+                     const double* highs_in, int relerr, size_t total_mem,
+                     size_t init_k, int user_regions, orc_result* out,
+                     orc_trace* tr, orc_opts* opts)
+{
+  const int keep = opts && opts->keep_list;
+  const int predict_split = opts && opts->predict_split;
+  const size_t max_iters = opts ? opts->max_iters : 0;
+  if (opts) {
+    opts->list_n = 0;
+    opts->list_L = opts->list_H = opts->list_est = opts->list_err = NULL;
+    opts->list_dim = NULL;
+    opts->budget_fixes = 0;
+  }
   if (orc_feval(d) < 0 || !out) return -1;
   double lows[ORC_MAXDIM], highs[ORC_MAXDIM];
   for (int j = 0; j < d; ++j) {
@@ -878,6 +1231,11 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
   int rc = 0;
 
   for (size_t it = 0; it < 700 && n > 0; it++) {
+    if (max_iters && it == max_iters) {
+      if (keep) keep_list(opts, d, n, L, H, NULL, NULL, NULL);
+      rc = 2;
+      goto done;
+    }
     double* est = (double*)malloc(sizeof(double) * n);
     double* err = (double*)malloc(sizeof(double) * n);
     int* dim = (int*)malloc(sizeof(int) * n);
@@ -886,7 +1244,11 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
                      NULL);
     memset(act, 1, n);
     const double iter_est = seq_sum(est, n);
-    if (np != 0) orc_refine(n, est, err, pest, epsrel, relerr, act);
+    /* predict_split overrides the relerr flag per iteration,
+     * Workspace.cuh:206-210 / 415-420 */
+    const int relerr_it =
+        predict_split ? ((n <= 15000000 && it < 15 && cum.nregions == 0) ? 0 : 1) : relerr;
+    if (np != 0) orc_refine(n, est, err, pest, epsrel, relerr_it, act);
     const double iter_err = seq_sum(err, n);
     if (user_regions) cum.iters++;
 
@@ -899,6 +1261,7 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
       cum.errorest += iter_err;
       cum.status = 0;
       cum.nregions += n;
+      if (keep) keep_list(opts, d, n, L, H, est, err, dim);
       free(est), free(err), free(dim), free(act);
       goto done;
     }
@@ -908,6 +1271,7 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
     double fin_err = iter_err - seq_dot(act, err, n);
     /* fix_error_budget_overflow, Workspace.cuh:121-146 */
     if (cum.errorest + fin_err > fabs(cum.estimate + iter_est) * epsrel) {
+      if (opts) opts->budget_fixes++;
       memset(act, 1, n);
       fin_err = 0.;
       fin_est = 0.;
@@ -938,6 +1302,7 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
         cum.estimate += iter_est;
         cum.errorest += iter_err;
         cum.nregions += n;
+        if (keep) keep_list(opts, d, n, L, H, est, err, dim);
         free(est), free(err), free(dim), free(act);
         goto done;
       }
@@ -974,6 +1339,7 @@ int orc_integrate(int d, int fam, const double* prm, double epsrel,
     if (!user_regions) cum.iters++;
   }
   cum.nregions += n;
+  if (keep) keep_list(opts, d, n, L, H, NULL, NULL, NULL);
 done:
   free(L);
   free(H);

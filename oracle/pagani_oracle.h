@@ -72,12 +72,25 @@ double orc_exact(int family, int ndim, const double* params);
  *   [10]    the region volume
  *   [11..13] the null-rule errors e_1, e_2, e_3 before the final branch
  *   [14]    the largest magnitude of the terms of a fourth difference,
- *           which bounds its rounding */
-#define ORC_RSUM 15
+ *           which bounds its rounding
+ *   [15]    (orc_eval_regions_kform only; 0 otherwise) sum_p |w_0 J| c_p, where
+ *           c_p is f's sensitivity to one rounding of its inner sum at point
+ *           p (|f| times the magnitude of the exponent for the exponential
+ *           families, |sin| times the angle's for the oscillatory one, ...):
+ *           the floor of any FP64 evaluation of f at that point */
+#define ORC_RSUM 16
 int orc_eval_regions(int ndim, int family, const double* params,
                      const double* lows, const double* highs, size_t n,
                      const double* L, const double* H, double* est,
                      double* err, int* dim, double* fdiff, double* rsum);
+/* The same evaluation at the library kernel's own points (pagani_oracle.c
+ * explains the difference), with f and every sum in long double.  plain = 1
+ * is an integrand without origin and axis scale: the reference's formula of
+ * the family at x = fma(-g, h, c). */
+int orc_eval_regions_kform(int ndim, int family, const double* params, int plain,
+                           const double* lows, const double* highs, size_t n,
+                           const double* L, const double* H, double* est,
+                           double* err, int* dim, double* fdiff, double* rsum);
 
 /* ---- region list phases -------------------------------------------------- */
 void orc_uniform_split(int ndim, size_t k, double* L, double* H);
@@ -154,6 +167,37 @@ int orc_integrate(int ndim, int family, const double* params, double epsrel,
                   double epsabs, const double* lows, const double* highs,
                   int relerr, size_t total_mem, size_t init_k,
                   int user_regions, orc_result* out, orc_trace* trace);
+
+/* Options of orc_integrate_ex (all zero: orc_integrate).
+ *   predict_split: the relerr flag of every iteration follows the rule of
+ *     Workspace::integrate<IntegT, predict_split=true>
+ *     (Workspace.cuh:206-210, 415-420) instead of `relerr`.
+ *   max_iters > 0: stop before iteration max_iters runs (return value 2);
+ *     the list it would evaluate is the final list.
+ *   keep_list: hand the final list back (malloc'd, freed with orc_free):
+ *     list_n regions, list_L/list_H SoA; when that list was evaluated (the
+ *     run converged or terminated) list_est/list_err/list_dim too, else
+ *     NULL. */
+typedef struct {
+  int predict_split;
+  size_t max_iters;
+  int keep_list;
+  size_t list_n;
+  double* list_L;
+  double* list_H;
+  double* list_est;
+  double* list_err;
+  int* list_dim;
+  size_t budget_fixes; /* out: iterations that took fix_error_budget_overflow */
+} orc_opts;
+
+/* returns 0, 2 when stopped by max_iters, negative on a bad argument */
+int orc_integrate_ex(int ndim, int family, const double* params, double epsrel,
+                     double epsabs, const double* lows, const double* highs,
+                     int relerr, size_t total_mem, size_t init_k,
+                     int user_regions, orc_result* out, orc_trace* trace,
+                     orc_opts* opts);
+void orc_free(void* p);
 
 /* sequential globally adaptive cubature (Cuhre-equivalent CPU baseline):
  * a heap of regions, always bisect the region with the largest error along

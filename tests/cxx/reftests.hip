@@ -220,9 +220,214 @@ int sample(int k, double* est_host, double* err_host, std::size_t cap)
   return rc == 0 ? static_cast<int>(n) : rc;
 }
 
+// cos(sum (i+1) x_i), the 8D sum-of-cosines workload at any dimension
+// (integrands.cuh:1322 has the 8D instance)
+template <int D>
+struct CosSum {
+  template <class... X>
+  __device__ double operator()(X... xs) const
+  {
+    const double x[D] = {static_cast<double>(xs)...};
+    double s = 0.0;
+    for (int i = 0; i < D; ++i) s += (i + 1) * x[i];
+    return cos(s);
+  }
+};
+
+// exp(-10 (|x - .3| + |y - .5| + |z - .7|)): a 3D C0 integrand that runs
+// past iteration 15 under predict_split at epsrel 1e-6
+struct C0Three {
+  __device__ double operator()(double x, double y, double z) const
+  {
+    return exp(-10. * fabs(x - .3) - 10. * fabs(y - .5) - 10. * fabs(z - .7));
+  }
+};
+
+// A whole integration through the header with its per-iteration trace;
+// predict_split selects integrate<F, true>.
+template <int D, class F>
+int run_traced(double epsrel, double epsabs, std::size_t mem_total, int predict_split,
+               int relerr, pagani_result* out, std::uint64_t* tn, std::uint64_t* tm,
+               std::size_t cap, std::size_t* ntrace)
+{
+  Workspace<double, D> pagani;
+  pagani_ws_set_device_loop(pagani.handle(), g_device_loop);
+  if (mem_total) pagani_ws_set_device_mem(pagani.handle(), mem_total);
+  quad::Volume<double, D> vol;
+  const numint::integration_result r =
+      predict_split ? pagani.template integrate<F, true>(F{}, epsrel, epsabs, vol, relerr != 0)
+                    : pagani.integrate(F{}, epsrel, epsabs, vol, relerr != 0);
+  std::memset(out, 0, sizeof(*out));
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->nregions = r.nregions;
+  out->status = r.status;
+  out->iters = r.iters;
+  *ntrace = pagani_ws_trace(pagani.handle(), cap, tn, tm, nullptr, nullptr);
+  return 0;
+}
+
+// The six Genz families as a user would write them: the formulas of the
+// reference's integrands (common/cuda/integrands.cuh) in a functor with only
+// operator(), so the kernel takes its plain path (x = c - g h per point, no
+// origin, no axis scale, no factored sums).
+template <int D, int Fam>
+struct PlainGenz {
+  double a[D], u[D];
+  template <class... X>
+  __device__ double operator()(X... xs) const
+  {
+    const double x[D] = {static_cast<double>(xs)...};
+    double s = 0.0;
+    if constexpr (Fam == PAGANI_GENZ_OSCILLATORY) {
+      s = 2.0 * M_PI * u[0];
+      for (int i = 0; i < D; ++i) s += a[i] * x[i];
+      return cos(s);
+    } else if constexpr (Fam == PAGANI_GENZ_PRODUCT_PEAK) {
+      double v = 1.0;
+      for (int i = 0; i < D; ++i) v *= 1.0 / (1.0 / (a[i] * a[i]) + (x[i] - u[i]) * (x[i] - u[i]));
+      return v;
+    } else if constexpr (Fam == PAGANI_GENZ_CORNER_PEAK) {
+      s = 1.0;
+      for (int i = 0; i < D; ++i) s += a[i] * x[i];
+      return pow(s, -(D + 1.0));
+    } else if constexpr (Fam == PAGANI_GENZ_GAUSSIAN) {
+      for (int i = 0; i < D; ++i) s += a[i] * a[i] * ((x[i] - u[i]) * (x[i] - u[i]));
+      return exp(-s);
+    } else if constexpr (Fam == PAGANI_GENZ_C0) {
+      for (int i = 0; i < D; ++i) s += a[i] * fabs(x[i] - u[i]);
+      return exp(-s);
+    } else {
+      for (int i = 0; i < D; ++i)
+        if (x[i] > u[i]) return 0.0;
+      for (int i = 0; i < D; ++i) s += a[i] * x[i];
+      return exp(s);
+    }
+  }
+};
+
+// One evaluation of n host-given regions (SoA [j*n + t]) with a plain
+// functor, at `slices` waves per batch (0: the library's choice).
+template <int D, int Fam>
+int sample_plain(const double* a, const double* u, std::size_t n, const double* Lh,
+                 const double* Hh, int slices, double* est_h, double* err_h, int* dim_h)
+{
+  PlainGenz<D, Fam> f;
+  for (int i = 0; i < D; ++i) {
+    f.a[i] = a[i];
+    f.u[i] = u[i];
+  }
+  Workspace<double, D> pagani;
+  pagani_ws_set_slices(pagani.handle(), slices);
+  double *L = nullptr, *H = nullptr, *est = nullptr, *err = nullptr;
+  std::uint8_t* dim = nullptr;
+  int rc = 0;
+  if (hipMalloc(&L, n * D * sizeof(double)) != hipSuccess ||
+      hipMalloc(&H, n * D * sizeof(double)) != hipSuccess ||
+      hipMalloc(&est, n * sizeof(double)) != hipSuccess ||
+      hipMalloc(&err, n * sizeof(double)) != hipSuccess || hipMalloc(&dim, n) != hipSuccess)
+    rc = -3;
+  if (rc == 0 && (hipMemcpy(L, Lh, n * D * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(H, Hh, n * D * sizeof(double), hipMemcpyHostToDevice) != hipSuccess))
+    rc = -2;
+  pagani_eval_args args{};
+  args.n = n;
+  args.L = L;
+  args.H = H;
+  args.est = est;
+  args.err = err;
+  args.dim = dim;
+  args.stream = pagani_ws_stream(pagani.handle());
+  if (rc == 0) rc = pagani_ws_eval_consts(pagani.handle(), nullptr, nullptr, n, &args.consts);
+  if (rc == 0) rc = pagani::eval_regions<D, PlainGenz<D, Fam>, double>(f, &args);
+  if (rc == 0 && hipStreamSynchronize(static_cast<hipStream_t>(args.stream)) != hipSuccess) rc = -2;
+  std::vector<std::uint8_t> dh(n);
+  if (rc == 0 && (hipMemcpy(est_h, est, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(err_h, err, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+                  hipMemcpy(dh.data(), dim, n, hipMemcpyDeviceToHost) != hipSuccess))
+    rc = -2;
+  for (std::size_t t = 0; rc == 0 && t < n; ++t) dim_h[t] = dh[t];
+  (void)hipFree(L);
+  (void)hipFree(H);
+  (void)hipFree(est);
+  (void)hipFree(err);
+  (void)hipFree(dim);
+  return rc;
+}
+
+template <int D>
+int sample_plain_family(int fam, const double* a, const double* u, std::size_t n,
+                        const double* L, const double* H, int slices, double* est, double* err,
+                        int* dim)
+{
+  switch (fam) {
+    case PAGANI_GENZ_OSCILLATORY:
+      return sample_plain<D, PAGANI_GENZ_OSCILLATORY>(a, u, n, L, H, slices, est, err, dim);
+    case PAGANI_GENZ_PRODUCT_PEAK:
+      return sample_plain<D, PAGANI_GENZ_PRODUCT_PEAK>(a, u, n, L, H, slices, est, err, dim);
+    case PAGANI_GENZ_CORNER_PEAK:
+      return sample_plain<D, PAGANI_GENZ_CORNER_PEAK>(a, u, n, L, H, slices, est, err, dim);
+    case PAGANI_GENZ_GAUSSIAN:
+      return sample_plain<D, PAGANI_GENZ_GAUSSIAN>(a, u, n, L, H, slices, est, err, dim);
+    case PAGANI_GENZ_C0:
+      return sample_plain<D, PAGANI_GENZ_C0>(a, u, n, L, H, slices, est, err, dim);
+    case PAGANI_GENZ_DISCONTINUOUS:
+      return sample_plain<D, PAGANI_GENZ_DISCONTINUOUS>(a, u, n, L, H, slices, est, err, dim);
+    default:
+      return -1;
+  }
+}
+
 }  // namespace
 
+extern "C" int reftest_sample_plain_genz(int fam, int d, const double* a, const double* u,
+                                         std::size_t n, const double* L, const double* H,
+                                         int slices, double* est, double* err, int* dim)
+{
+  if (!a || !u || !L || !H || !est || !err || !dim || n == 0) return -1;
+  try {
+    switch (d) {
+      case 2: return sample_plain_family<2>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 3: return sample_plain_family<3>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 4: return sample_plain_family<4>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 5: return sample_plain_family<5>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 6: return sample_plain_family<6>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 7: return sample_plain_family<7>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 8: return sample_plain_family<8>(fam, a, u, n, L, H, slices, est, err, dim);
+      default: return -1;
+    }
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
+
 extern "C" void reftest_set_device_loop(int on) { g_device_loop = on ? 1 : 0; }
+
+extern "C" int reftest_integrate_traced(const char* name, double epsrel, double epsabs,
+                                        std::size_t mem_total, int predict_split, int relerr,
+                                        pagani_result* out, std::uint64_t* tn, std::uint64_t* tm,
+                                        std::size_t cap, std::size_t* ntrace)
+{
+  if (!name || !out || !tn || !tm || !ntrace) return -1;
+  try {
+#define PAGANI_TRACED(N, D, F)                                                              \
+  if (!std::strcmp(name, N))                                                                \
+    return run_traced<D, F>(epsrel, epsabs, mem_total, predict_split, relerr, out, tn, tm, \
+                            cap, ntrace);
+    PAGANI_TRACED("F_2_6D", 6, F_2_6D)
+    PAGANI_TRACED("F_3_8D", 8, F_3_8D)
+    PAGANI_TRACED("F_4_5D", 5, F_4_5D)
+    PAGANI_TRACED("F_5_8D", 8, F_5_8D)
+    PAGANI_TRACED("F_6_6D", 6, F_6_6D)
+    PAGANI_TRACED("CosSum5", 5, CosSum<5>)
+    PAGANI_TRACED("CosSum6", 6, CosSum<6>)
+    PAGANI_TRACED("C0Three", 3, C0Three)
+#undef PAGANI_TRACED
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}
 
 extern "C" int reftest_integrate(const char* name, double epsrel, double epsabs,
                                  pagani_result* out)

@@ -30,8 +30,9 @@ FAM = {
 }
 
 # per-region extra record of orc_eval_regions: s_0..s_4, A_0..A_4 (sum |w_r f|),
-# volume, e_1, e_2, e_3, fourth-difference magnitude
-RSUM = 15
+# volume, e_1, e_2, e_3, fourth-difference magnitude, and (kernel-point mode)
+# the sensitivity of f to one rounding of its inner sum, sum_p |w_0 J| c_p
+RSUM = 16
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -61,6 +62,21 @@ class OrcTrace(ctypes.Structure):
         ("m", _sp),
         ("estimate", _dp),
         ("errorest", _dp),
+    ]
+
+
+class OrcOpts(ctypes.Structure):
+    _fields_ = [
+        ("predict_split", ctypes.c_int),
+        ("max_iters", ctypes.c_size_t),
+        ("keep_list", ctypes.c_int),
+        ("list_n", ctypes.c_size_t),
+        ("list_L", _dp),
+        ("list_H", _dp),
+        ("list_est", _dp),
+        ("list_err", _dp),
+        ("list_dim", _ip),
+        ("budget_fixes", ctypes.c_size_t),
     ]
 
 
@@ -174,6 +190,15 @@ def _load():
         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp,
         _dp, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
         ctypes.POINTER(OrcResult), ctypes.POINTER(OrcTrace)]
+    lib.orc_integrate_ex.argtypes = [
+        ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double, _dp,
+        _dp, ctypes.c_int, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int,
+        ctypes.POINTER(OrcResult), ctypes.POINTER(OrcTrace), ctypes.POINTER(OrcOpts)]
+    lib.orc_free.argtypes = [ctypes.c_void_p]
+    lib.orc_free.restype = None
+    lib.orc_eval_regions_kform.argtypes = [
+        ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_size_t, _dp, _dp,
+        _dp, _dp, _ip, _dp, _dp]
     lib.orc_cuhre_seq.argtypes = [
         ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_double,
         ctypes.c_size_t, ctypes.POINTER(OrcResult)]
@@ -296,6 +321,32 @@ def eval_regions(d, fam, prm, L, H, n, lows=None, highs=None, want_extra=False):
     return est, err, dim
 
 
+def eval_regions_kform(d, fam, prm, L, H, n, lows=None, highs=None, plain=False):
+    """Per-region evaluation at the library kernel's own points (centre,
+    origin, axis scale, fma(-g, h, c)) with f and the sums in long double.
+    Returns est, err, dim, fourth differences, the per-region record."""
+    est = np.zeros(n)
+    err = np.zeros(n)
+    dim = np.zeros(n, dtype=np.int32)
+    fd = np.zeros(n * d)
+    rs = np.zeros(n * RSUM)
+    lw = None if lows is None else np.ascontiguousarray(lows, dtype=np.float64)
+    hg = None if highs is None else np.ascontiguousarray(highs, dtype=np.float64)
+    rc = LIB.orc_eval_regions_kform(d, FAM.get(fam, fam), dptr(prm), int(plain), dptr(lw),
+                                    dptr(hg), n, dptr(L), dptr(H), dptr(est), dptr(err),
+                                    iptr(dim), dptr(fd), dptr(rs))
+    assert rc == 0
+    return est, err, dim, fd.reshape(n, d), rs.reshape(n, RSUM)
+
+
+def eval_floor(rs):
+    """vol * sum_p |w_0 J| c_p: the region's estimate moves by this much
+    (times 2^-53 per rounding) for one rounding of the inner sum of f at
+    every point -- the least any FP64 evaluation of the integrand differs
+    from the long-double one (kernel-point mode only)."""
+    return rs[:, 10] * rs[:, 15]
+
+
 def est_scale(rs):
     """vol * sum_p |w_0 f_p|: the magnitude that bounds the rounding of a
     region's estimate."""
@@ -370,8 +421,13 @@ class Result:
 
 
 def integrate(d, fam, prm, epsrel, epsabs, lows=None, highs=None, relerr=True,
-              total_mem=int(16e9), init_k=0, user_regions=False, trace_cap=0):
+              total_mem=int(16e9), init_k=0, user_regions=False, trace_cap=0,
+              predict_split=False, max_iters=0, keep_list=False):
+    """The oracle's Workspace::integrate loop.  With keep_list the final list
+    comes back as a dict (n, L, H and, when it was evaluated, est/err/dim);
+    max_iters > 0 stops before that iteration (status "suspended")."""
     res = OrcResult()
+    opts = OrcOpts(int(predict_split), int(max_iters), int(keep_list), 0)
     tr = None
     arrs = None
     if trace_cap:
@@ -381,11 +437,23 @@ def integrate(d, fam, prm, epsrel, epsabs, lows=None, highs=None, relerr=True,
                       dptr(arrs[2]), dptr(arrs[3]))
     lw = None if lows is None else np.ascontiguousarray(lows, dtype=np.float64)
     hg = None if highs is None else np.ascontiguousarray(highs, dtype=np.float64)
-    rc = LIB.orc_integrate(d, FAM.get(fam, fam), dptr(prm), epsrel, epsabs, dptr(lw), dptr(hg),
-                           int(relerr), int(total_mem), init_k, int(user_regions),
-                           ctypes.byref(res), ctypes.byref(tr) if tr is not None else None)
-    assert rc == 0
+    rc = LIB.orc_integrate_ex(d, FAM.get(fam, fam), dptr(prm), epsrel, epsabs, dptr(lw),
+                              dptr(hg), int(relerr), int(total_mem), init_k, int(user_regions),
+                              ctypes.byref(res), ctypes.byref(tr) if tr is not None else None,
+                              ctypes.byref(opts))
+    assert rc in (0, 2)
     out = Result(res.estimate, res.errorest, res.neval, res.nregions, res.status, res.iters)
+    out.suspended = rc == 2
+    out.budget_fixes = opts.budget_fixes
+    if keep_list:
+        n = opts.list_n
+        lst = dict(n=n)
+        for k, ln in (("L", n * d), ("H", n * d), ("est", n), ("err", n), ("dim", n)):
+            p = getattr(opts, "list_" + k)
+            lst[k] = np.ctypeslib.as_array(p, shape=(ln,)).copy() if p and ln else (
+                None if not p else np.zeros(0))
+            LIB.orc_free(ctypes.cast(p, ctypes.c_void_p))
+        out.final_list = lst
     if trace_cap:
         k = tr.len
         return out, dict(n=arrs[0][:k].astype(np.int64), m=arrs[1][:k].astype(np.int64),

@@ -118,3 +118,50 @@ def test_trim_releases_pool(pg, gpu):
             with pytest.raises(pg.PaganiError):
                 ws.trim()
     assert (r1.estimate, r1.errorest, r1.nregions) == (r2.estimate, r2.errorest, r2.nregions)
+
+
+def test_config4_full_batch(pg, gpu):
+    """The 512-integral batch exactly as benchmarked (families and d in
+    {3..8} drawn, epsrel log-uniform in [1e-6, 1e-3], seed 2024), one call:
+    - every item bit for bit equal to its lone run on a workspace with the
+      batch's memory total;
+    - converged smooth items (oscillatory, product peak, corner peak,
+      Gaussian) within their error estimate of the analytic value, with the
+      per-family counts printed;
+    - C0 and discontinuous items the oracle's loop can run in seconds (d <= 4,
+      or d = 5 at epsrel >= 4e-5): the oracle's status, region count and
+      iteration count, estimates within the combined error estimates."""
+    import oracle_api as O
+    from gpuintegration_amd import workloads as W
+    items = W.batch(512, 2024)
+    hbm = _hbm()
+    with pg.Batch(0, 1) as b:
+        res, codes = b.integrate(items)
+    assert all(c >= 0 for c in codes), [c for c in codes if c < 0]
+    want = _single(pg, items, hbm)
+    for i, (r, w) in enumerate(zip(res, want)):
+        assert (r.estimate, r.errorest, r.nregions, r.iters, r.status) == \
+               (w.estimate, w.errorest, w.nregions, w.iters, w.status), (i, items[i][0])
+    smooth = ("oscillatory", "product_peak", "corner_peak", "gaussian")
+    counts = {}
+    for i, ((fam, a, u, eps), r) in enumerate(zip(items, res)):
+        if fam in smooth and r.status == 0:
+            exact = W.exact(fam, a, u)
+            ok = abs(r.estimate - exact) <= r.errorest
+            counts.setdefault(fam, [0, 0])
+            counts[fam][0] += 1
+            counts[fam][1] += ok
+            assert ok, (i, fam, len(a), eps, r.estimate, exact, r.errorest)
+    print("smooth items within errorest of the analytic value:", counts)
+    checked = 0
+    for i, ((fam, a, u, eps), r) in enumerate(zip(items, res)):
+        if fam not in ("c0", "discontinuous"):
+            continue
+        d = len(a)
+        if not (d <= 4 or (d == 5 and eps >= 4e-5)):
+            continue
+        o = O.integrate(d, fam, O.params(a, u), eps, 1e-20, total_mem=hbm)
+        assert o.status == r.status and o.nregions == r.nregions and o.iters == r.iters, i
+        assert abs(r.estimate - o.estimate) <= r.errorest + o.errorest, i
+        checked += 1
+    assert checked >= 40

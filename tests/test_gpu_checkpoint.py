@@ -113,6 +113,38 @@ def test_checkpoint_rejects_damage_and_mismatch(pg, gpu, tmp_path):
             ws5.load_state(path)
 
 
+def test_failed_load_drops_a_held_run(pg, gpu, tmp_path):
+    """A load that fails after it has begun to overwrite the region pool (a
+    checksum mismatch is found only after the arrays) must not leave the
+    workspace holding the earlier suspended run on damaged bounds: resume
+    refuses, and a good file then continues to the uninterrupted result."""
+    r0, _ = _uninterrupted(pg)
+    path = str(tmp_path / "run.pagani")
+    with pg.Workspace(len(A), 0) as ws:
+        ws.set_device_mem(MEM)
+        ws.set_iteration_limit(4)
+        ws.integrate_genz(FAM, A, U, epsrel=EPS, epsabs=1e-20)
+        assert ws.suspended()
+        ws.save_state(path)
+        bad = bytearray(open(path, "rb").read())
+        bad[len(bad) - 64] ^= 0x10
+        p_bad = str(tmp_path / "bad.pagani")
+        open(p_bad, "wb").write(bytes(bad))
+        with pytest.raises(pg.PaganiError, match="checksum"):
+            ws.load_state(p_bad)
+        assert not ws.suspended()
+        with pytest.raises(pg.PaganiError, match="no suspended run"):
+            ws.resume_genz(FAM, A, U)
+        ws.load_state(path)
+        assert ws.suspended()
+        again, rounds = True, 0
+        while again:
+            r, again = ws.resume_genz(FAM, A, U)
+            rounds += 1
+            assert rounds < 100
+    assert _key(r) == _key(r0)
+
+
 def test_regions_csv(pg, gpu, tmp_path):
     d = 4
     a, u = [8.0, 9.0, 10.0, 11.0], [0.4, 0.5, 0.6, 0.3]
@@ -177,3 +209,59 @@ def test_predict_split_cpp_overload(pg, gpu):
         assert abs(r2.estimate - r1.estimate) <= r1.errorest + r2.errorest
     else:
         assert snap.value == 0
+
+
+def _csv_regions(path, d):
+    rows = np.loadtxt(path, delimiter=",", ndmin=2)
+    est, err = rows[:, 0], rows[:, 1]
+    up, lo = rows[:, 2:2 + 2 * d:2], rows[:, 3:3 + 2 * d:2]
+    return est, err, lo, up, rows[:, -2].astype(int)
+
+
+def test_suspended_list_matches_oracle(pg, gpu, tmp_path):
+    """A run suspended after k iterations holds the oracle's list for
+    iteration k (Sub_regions.cuh:220-239 keeps such a list; quad.h:107-118
+    writes it): the same regions in the same order, bounds bit for bit (the
+    CSV carries 17 significant digits, which round-trip a double); and the
+    list a finished run ends with carries the oracle's per-region estimates
+    at the kernel's points within 1e-13 of their scale."""
+    import oracle_api as O
+    # distinct scales and centres per axis: no region's split axis is a tie
+    # decided by rounding (the oracle maps points as the reference does, the
+    # kernel a few roundings away)
+    d, a, u, eps = 4, [8.0, 9.5, 11.0, 12.5], [0.41, 0.53, 0.62, 0.37], 1e-5
+    prm = O.params(a, u)
+    full, tr = O.integrate(d, "gaussian", prm, eps, 1e-20, trace_cap=800, keep_list=True)
+    iters = len(tr["n"])
+    assert iters > 4
+    for k in (1, 3, iters - 1):
+        o = O.integrate(d, "gaussian", prm, eps, 1e-20, max_iters=k, keep_list=True)
+        lst = o.final_list
+        with pg.Workspace(d, 0) as ws:
+            ws.set_device_mem(MEM)
+            ws.set_iteration_limit(k)
+            ws.integrate_genz("gaussian", a, u, epsrel=eps, epsabs=1e-20)
+            assert ws.suspended()
+            p = str(tmp_path / f"susp{k}.csv")
+            ws.write_regions_csv(p)
+        est, err, lo, up, dim = _csv_regions(p, d)
+        assert len(est) == lst["n"]
+        assert np.all(np.isnan(est)) and np.all(dim == -1)
+        oL = lst["L"].reshape(d, -1).T
+        oU = oL + lst["H"].reshape(d, -1).T
+        assert np.array_equal(lo, oL), f"iteration {k}: lower bounds differ"
+        assert np.array_equal(up, oU), f"iteration {k}: upper bounds differ"
+    # the final, evaluated list of the uninterrupted run
+    with pg.Workspace(d, 0) as ws:
+        ws.set_device_mem(MEM)
+        r = ws.integrate_genz("gaussian", a, u, epsrel=eps, epsabs=1e-20)
+        p = str(tmp_path / "final.csv")
+        ws.write_regions_csv(p)
+    assert r.status == full.status and r.nregions == full.nregions
+    lst = full.final_list
+    est, err, lo, up, dim = _csv_regions(p, d)
+    oL = lst["L"].reshape(d, -1).T
+    assert np.array_equal(lo, oL) and np.array_equal(up, oL + lst["H"].reshape(d, -1).T)
+    oe, oerr, odim, fd, rs = O.eval_regions_kform(d, "gaussian", prm, lst["L"], lst["H"],
+                                                  lst["n"])
+    assert np.all(np.abs(est - oe) <= 1e-13 * O.est_scale(rs) + 1e-300)

@@ -98,9 +98,14 @@ def centred_cond(family, a):
     return 0.0
 
 
-def check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, cond=0.0):
+def check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, cond=0.0,
+                        floor=False):
+    """floor: add the FP64 evaluation floor of the kernel-point oracle,
+    (d + 2) eps times the region's sum of |w_0 J| x f's sensitivity to one
+    rounding of its inner sum (oracle_api.eval_floor), on top of 1e-13."""
     scale = O.est_scale(rs)
-    bad = np.abs(ge - oe) > (1e-13 + cond) * scale + 1e-300
+    extra = (d + 2) * EPS * O.eval_floor(rs) if floor else 0.0
+    bad = np.abs(ge - oe) > (1e-13 + cond) * scale + extra + 1e-300
     assert not bad.any(), (
         f"{family} d={d}: {bad.sum()} estimates off; worst "
         f"{np.max(np.abs(ge - oe) / np.maximum(scale, 1e-300)):.3e} of scale")
@@ -275,3 +280,132 @@ def test_eval_constant_kat(pg, gpu):
             got = lib.reftest_sample(name, k, est.ctypes.data, err.ctypes.data, n)
             assert got == n
             assert np.allclose(est, c / n, rtol=1e-6, atol=1e-300)
+
+
+# ---------------------------------------------------------------------------
+# Parity at the kernel's own points.  The tests above compare with the
+# reference's point map and allow for the conditioning of the steep
+# integrands (centred_cond: up to 42x the 1e-13 bound for the a = 25
+# Gaussian).  Here the oracle forms every point exactly as the kernel does
+# (centre c = fma(.5 (lo + up), range, lows), minus the origin and times the
+# axis scale the functor declares, v = fma(-g, h, c)) and evaluates f and
+# the sums in long double: what is left is the kernel's own rounding.
+#   - The headline's first iteration (the 4^8 split of the 8D Gaussian, at
+#     one wave per batch and at the library's launch shape) and F_4_5D's
+#     parameters are held to 1e-13 * vol * A_0 with no allowance.
+#   - Elsewhere the bound adds the floor any FP64 evaluation of f has at
+#     those points (check_region_parity's `floor`): f's sensitivity to one
+#     rounding of its inner sum, point by point.  It matters only where that
+#     sum is large against f's scale -- a cosine near a zero (measured: one
+#     2D oscillatory region at 1.02e-13), a Gaussian exponent of several
+#     hundred on a stretched box (1.67e-13) -- and is below 1e-14 of the
+#     scale elsewhere.
+
+KFORM_DIMS = [2, 3, 4, 5, 6, 7, 8]
+
+
+def _kform(d, family, a, u, L, H, n, plain=False, lows=None, highs=None):
+    return O.eval_regions_kform(d, family, O.params(a, u), L, H, n, lows=lows, highs=highs,
+                                plain=plain)
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+@pytest.mark.parametrize("d", KFORM_DIMS)
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_kernel_points_fp64(pg, gpu, family, d, slices):
+    n = 2000 if d < 8 else 1000
+    a, u = genz_params(family, d, seed=300 + d)
+    L, H = random_regions(d, n, seed=13 * d + 1)
+    with pg.Workspace(d, 0) as ws:
+        if slices:
+            ws.set_slices(slices)
+        ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n)
+    oe, oerr, odim, fd, rs = _kform(d, family, a, u, L, H, n)
+    check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, floor=True)
+
+
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_kernel_points_config2_split(pg, gpu, slices):
+    """The 8D Gaussian workload's first iteration (4^8 regions) at one wave
+    per batch (the factored set sums of the headline run) and at the
+    library's launch shape."""
+    import torch
+    d, k = 8, 4
+    n = k ** d
+    from gpuintegration_amd.workloads import gaussian_8d
+    _, a, u = gaussian_8d(2024)
+    with pg.Workspace(d, 0) as ws:
+        if slices:
+            ws.set_slices(slices)
+        Ld = torch.empty(n * d, dtype=torch.float64, device=gpu)
+        Hd = torch.empty_like(Ld)
+        ws.uniform_split(k, Ld, Hd)
+        L, H = Ld.cpu().numpy(), Hd.cpu().numpy()
+        ge, gerr, gdim = gpu_eval(ws, gpu, "gaussian", a, u, L, H, n)
+    oe, oerr, odim, fd, rs = _kform(d, "gaussian", a, u, L, H, n)
+    check_region_parity(d, "gaussian", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_kernel_points_f4_5d(pg, gpu, slices):
+    """F_4_5D's parameters (a = 25, u = 0.5; the steepest Gaussian of the
+    reference's tests): a 3^5 split and random sub-boxes."""
+    d = 5
+    a, u = [25.0] * d, [0.5] * d
+    L3, H3, n3 = O.uniform_split(d, 3)
+    Lr, Hr = random_regions(d, 3000, seed=45)
+    with pg.Workspace(d, 0) as ws:
+        if slices:
+            ws.set_slices(slices)
+        for L, H, n in ((L3, H3, n3), (Lr, Hr, 3000)):
+            ge, gerr, gdim = gpu_eval(ws, gpu, "gaussian", a, u, L, H, n)
+            oe, oerr, odim, fd, rs = _kform(d, "gaussian", a, u, L, H, n)
+            check_region_parity(d, "gaussian", ge, gerr, gdim, oe, oerr, odim, fd, rs)
+
+
+def test_eval_kernel_points_general_volume(pg, gpu):
+    d, n = 5, 800
+    lows, highs = [-1.0, 0.0, 2.0, -0.5, 0.25], [1.0, 3.0, 2.5, 0.5, 0.75]
+    L, H = random_regions(d, n, seed=21)
+    with pg.Workspace(d, 0) as ws:
+        for family in FAMILIES:
+            a, u = genz_params(family, d, seed=41)
+            ge, gerr, gdim = gpu_eval(ws, gpu, family, a, u, L, H, n, lows=lows, highs=highs)
+            oe, oerr, odim, fd, rs = _kform(d, family, a, u, L, H, n, lows=np.array(lows),
+                                            highs=np.array(highs))
+            check_region_parity(d, family, ge, gerr, gdim, oe, oerr, odim, fd, rs, floor=True)
+
+
+@pytest.fixture(scope="module")
+def plainlib():
+    import ctypes
+    import os
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cxx",
+                                   "libreftests.so"))
+    lib.reftest_sample_plain_genz.argtypes = [
+        ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_void_p]
+    return lib
+
+
+@pytest.mark.parametrize("family", FAMILIES)
+@pytest.mark.parametrize("d", KFORM_DIMS)
+@pytest.mark.parametrize("slices", [0, 1])
+def test_eval_plain_functor(pg, gpu, plainlib, family, d, slices):
+    """A user's functor with only operator() (the reference's formula of each
+    family, in tests/cxx/reftests.hip) takes the kernel's plain path: every
+    point x = fma(-g, h, c), f called point by point."""
+    n = 1500 if d < 8 else 700
+    a, u = genz_params(family, d, seed=500 + d)
+    L, H = random_regions(d, n, seed=17 * d + 3)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    est, err = np.zeros(n), np.zeros(n)
+    dim = np.zeros(n, dtype=np.int32)
+    rc = plainlib.reftest_sample_plain_genz(pg.FAMILY[family], d, a.ctypes.data, u.ctypes.data,
+                                            n, L.ctypes.data, H.ctypes.data, slices,
+                                            est.ctypes.data, err.ctypes.data, dim.ctypes.data)
+    assert rc == 0
+    oe, oerr, odim, fd, rs = _kform(d, family, a, u, L, H, n, plain=True)
+    check_region_parity(d, family, est, err, dim, oe, oerr, odim, fd, rs, floor=True)

@@ -548,3 +548,127 @@ def test_mt19937_64_standard_value():
     for _ in range(9999):
         g()
     assert g() == 9981545732273789042
+
+
+# ---------------------------------------------------------------------------
+# Evaluation at the library kernel's own points (orc_eval_regions_kform)
+
+def test_kform_rule_exactness():
+    """At any point formation the rule is exact through degree 9: the
+    kernel-point mode integrates random monomials over a sub-box exactly
+    (plain form: x = fma(-g, h, c)), and fails at degree 10 as the
+    reference-map mode does."""
+    rng = np.random.default_rng(5)
+    d = 4
+    L = np.array([0.1, 0.2, 0.05, 0.3])
+    H = np.array([0.5, 0.25, 0.7, 0.4])
+    for _ in range(20):
+        k = rng.multinomial(int(rng.integers(0, 10)), [1 / d] * d).astype(float)
+        prm = O.params(k)
+        exact = np.prod([((L[j] + H[j]) ** (k[j] + 1) - L[j] ** (k[j] + 1)) / (k[j] + 1)
+                         for j in range(d)])
+        est, *_ = O.eval_regions_kform(d, "monomial", prm, L, H, 1, plain=True)
+        assert abs(est[0] - exact) <= 1e-15 * max(abs(exact), 1e-300) + 1e-18
+    k = np.array([10.0, 0, 0, 0])
+    exact = ((L[0] + H[0]) ** 11 - L[0] ** 11) / 11 * np.prod(H[1:])
+    est, *_ = O.eval_regions_kform(d, "monomial", O.params(k), L, H, 1, plain=True)
+    assert abs(est[0] - exact) > 1e-9 * exact
+
+
+@pytest.mark.parametrize("fam", ["oscillatory", "product_peak", "corner_peak", "gaussian", "c0",
+                                 "discontinuous"])
+@pytest.mark.parametrize("plain", [False, True])
+def test_kform_agrees_with_reference_map(fam, plain):
+    """Both modes evaluate the same rule at points a few roundings apart: the
+    estimates agree within that conditioning (a dropped origin, a wrong sign
+    of the axis scale or a transposed centre would not), and the split axes
+    agree away from ties."""
+    d, n = 5, 400
+    rng = np.random.default_rng(11)
+    a = 5 + 10 * rng.random(d) if fam != "oscillatory" else 1 + 0.5 * rng.random(d)
+    u = 0.2 + 0.6 * rng.random(d)
+    H = (10.0 ** rng.uniform(-2.5, -0.5, size=(d, n)))
+    L = rng.random((d, n)) * (1 - H)
+    L, H = L.ravel().copy(), H.ravel().copy()
+    prm = O.params(a, u)
+    e1, _, d1, fd1, rs1 = O.eval_regions(d, fam, prm, L, H, n, want_extra=True)
+    e2, _, d2, _, _ = O.eval_regions_kform(d, fam, prm, L, H, n, plain=plain)
+    cond = 64 * 2.0 ** -53 * float(np.sum(np.asarray(a) ** 2 if fam == "gaussian" else a))
+    scale = O.est_scale(rs1)
+    assert np.all(np.abs(e1 - e2) <= (1e-13 + cond) * scale + 1e-300)
+    srt = np.sort(fd1, axis=1)
+    clear = (srt[:, -1] - srt[:, -2]) > 1e-8 * rs1[:, 14]
+    assert np.all((d1 == d2) | ~clear)
+
+
+def test_kform_discontinuous_mask_at_kernel_points():
+    """The discontinuous family's mask is decided on the kernel's t = x - u:
+    a region whose right edge is u_j exactly keeps its points on the valid
+    side (t <= 0 exactly at the edge) and integrates the exponential."""
+    d = 2
+    a, u = [2.0, 3.0], [0.5, 0.75]
+    L = np.array([0.25, 0.5])
+    H = np.array([0.25, 0.25])
+    est, *_ = O.eval_regions_kform(d, "discontinuous", O.params(a, u), L, H, 1)
+    exact = (np.exp(2 * 0.5) - np.exp(2 * 0.25)) / 2 * (np.exp(3 * 0.75) - np.exp(3 * 0.5)) / 3
+    assert abs(est[0] - exact) <= 1e-10 * exact
+
+
+# ---------------------------------------------------------------------------
+# Loop options of orc_integrate_ex
+
+def test_budget_fix_branch_is_taken():
+    """cos(sum (i+1) x_i): the finished regions' errors overflow the budget in
+    5D and 6D, so fix_error_budget_overflow (Workspace.cuh:121-146) runs; the
+    runs still converge to the closed form."""
+    for d, eps, least in ((5, 1e-5, 1), (6, 1e-4, 1), (6, 1e-6, 1)):
+        a, u = [float(i + 1) for i in range(d)], [0.0] * d
+        prm = O.params(a, u)
+        r = O.integrate(d, "oscillatory", prm, eps, 1e-20)
+        assert r.budget_fixes >= least
+        assert r.status == 0
+        assert abs(r.estimate - O.exact("oscillatory", d, prm)) <= r.errorest
+    # an integrand whose finished errors stay within budget never takes it
+    r = O.integrate(3, "gaussian", O.params([5.0] * 3, [0.5] * 3), 1e-6, 1e-20)
+    assert r.budget_fixes == 0
+
+
+def test_predict_split_rule():
+    """Under the predict_split rule no region finishes by relerr before
+    iteration 15 while nothing has finished (every region of those
+    iterations is split again: m == n), and relerr is on from iteration 15,
+    where regions finish."""
+    d, a, u, eps = 3, [10.0] * 3, [0.3, 0.5, 0.7], 1e-6
+    r, tr = O.integrate(d, "c0", O.params(a, u), eps, 1e-20, trace_cap=800,
+                        predict_split=True)
+    assert len(tr["n"]) > 16
+    assert all(int(m) == int(n) for n, m in zip(tr["n"][:15], tr["m"][:15]))
+    assert int(tr["m"][15]) < int(tr["n"][15])
+    assert r.status == 0
+    assert abs(r.estimate - O.exact("c0", d, O.params(a, u))) <= 2 * r.errorest
+    # with the caller's flag instead, regions finish from the first iterations
+    r2, tr2 = O.integrate(d, "c0", O.params(a, u), eps, 1e-20, trace_cap=800)
+    assert any(int(m) < int(n) for n, m in zip(tr2["n"][1:15], tr2["m"][1:15]))
+
+
+def test_iteration_limit_and_final_list():
+    """max_iters stops before that iteration with the list it would evaluate
+    (the split of the previous one); the full run's final list was evaluated
+    and its estimates sum, with the finished regions', to the result."""
+    d, a, u, eps = 4, [10.0] * 4, [0.4, 0.5, 0.6, 0.45], 1e-5
+    prm = O.params(a, u)
+    full, tr = O.integrate(d, "gaussian", prm, eps, 1e-20, trace_cap=800, keep_list=True)
+    lst = full.final_list
+    assert lst["n"] == int(tr["n"][-1]) and lst["est"] is not None
+    for k in (1, 3, len(tr["n"]) - 1):
+        part = O.integrate(d, "gaussian", prm, eps, 1e-20, max_iters=k, keep_list=True)
+        assert part.suspended
+        pl = part.final_list
+        assert pl["n"] == int(tr["n"][k]) and pl["est"] is None
+        # the list is the oracle's split of iteration k-1: boxes inside the
+        # unit cube whose volumes sum to what was still active
+        Lm, Hm = pl["L"].reshape(d, -1), pl["H"].reshape(d, -1)
+        assert np.all(Lm >= 0) and np.all(Lm + Hm <= 1 + 1e-15)
+    # the evaluated final list: its estimates are those of an evaluation
+    e, r_, _ = O.eval_regions(d, "gaussian", prm, lst["L"], lst["H"], lst["n"])
+    assert np.array_equal(e, lst["est"])
