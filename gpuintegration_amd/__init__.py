@@ -202,6 +202,8 @@ _SIGS = {
                                             _vp, _vp, _dp, _dp, _vp, _vp, _vp]),
     "pagani_refine_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, ctypes.c_double,
                                          ctypes.c_int, _vp, _dp, ctypes.POINTER(_sz)]),
+    "pagani_refine_dev_noalloc": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, ctypes.c_double,
+                                                 ctypes.c_int, _vp, _vp, _vp]),
     "pagani_filter_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp, ctypes.POINTER(_sz)]),
     "pagani_split_dev": (ctypes.c_int, [_vp, _sz, _vp, _vp, _vp, _vp, _vp]),

@@ -39,6 +39,7 @@ HIP_SOURCES = {
     "batch.hip": [],
     "checkpoint.hip": [],
     "mcubes.hip": [],
+    "noalloc.hip": [],
 }
 CXX_SOURCES = {"rule.cpp": ["-O2", "-ffp-contract=off"],
                "mcubes_grid.cpp": ["-O2", "-ffp-contract=off"]}

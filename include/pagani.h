@@ -372,6 +372,23 @@ int pagani_refine_dev(pagani_ws* ws, size_t n, const double* est,
                       const double* parent_est, double epsrel,
                       int relerr_classification, uint8_t* active,
                       double* sums, size_t* num_active);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, serving loops that forbid hipMalloc):
+ * pagani_refine_dev's refinement, flags and sums with every intermediate
+ * record in a fixed 1.5 MB device array of the library (one per GPU, held
+ * until the process ends) instead of workspace buffers that grow on demand.
+ * n is at most PAGANI_NOALLOC_MAX_REGIONS (else PAGANI_EBADARG).  The call
+ * allocates nothing, only enqueues on the workspace's stream
+ * (pagani_ws_stream) and returns: sums_dev[4] = {sum est, sum err_out, sum
+ * active*est, sum active*err_out} and *num_active_dev (device pointers, may
+ * be NULL) are valid once that stream has drained.  Every call on one GPU
+ * shares its array: calls that overlap on one GPU race. */
+#define PAGANI_NOALLOC_MAX_REGIONS ((size_t)1 << 26)
+int pagani_refine_dev_noalloc(pagani_ws* ws, size_t n, const double* est,
+                              const double* err_in, double* err_out,
+                              const double* parent_est, double epsrel,
+                              int relerr_classification, uint8_t* active,
+                              double* sums_dev, uint64_t* num_active_dev);
 /* Sub_regions_filter::filter, Sub_region_filter.cuh:86-139: order-preserving
  * compaction of the active regions (bounds at stride m, parent estimates,
  * split axes).  Returns m in *m_out. */
