@@ -18,7 +18,8 @@ __all__ = [
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpagani.so")
 
-MIN_DIM, MAX_DIM = 2, 8
+MIN_DIM, MAX_DIM = 2, 16
+GENZ_MAX_DIM = 10  # the precompiled Genz descriptor path
 NSETS, NRULES, MAX_SLICES = 9, 5, 8
 FP64, FP32 = 64, 32
 OK, NOT_CONVERGED, EBADARG, EHIP, ENOMEM, EUNSUPPORTED = 0, 1, -1, -2, -3, -4
@@ -95,6 +96,8 @@ class EvalConsts(ctypes.Structure):
         ("lambda3", ctypes.c_double),
         ("lambda4", ctypes.c_double),
         ("lambda5", ctypes.c_double),
+        ("abi_version", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
     ]
 
 

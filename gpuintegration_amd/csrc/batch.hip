@@ -37,6 +37,7 @@ struct pagani_batch {
   std::size_t mem_total = 0;  // 0 = hbm / workers
   int mem_model = PAGANI_MEM_REFERENCE;
   bool timing = false;
+  bool tuned = true;
   // ws[w][d]: worker w's workspace for dimension d
   std::vector<std::array<pagani_ws*, PAGANI_MAX_DIM + 1>> ws;
   std::mutex mu;
@@ -65,8 +66,8 @@ int run_item(pagani_batch* b, int w, const pagani_batch_item& it, pagani_result*
   out->status = 1;
   out->lastPhase = -1;
   const int d = it.f.ndim;
-  if (d < PAGANI_MIN_DIM || d > PAGANI_MAX_DIM) {
-    set_error(b, "batch item ndim outside [2, 8]");
+  if (d < PAGANI_MIN_DIM || d > PAGANI_GENZ_MAX_DIM) {
+    set_error(b, "batch item ndim outside [2, 10]");
     return PAGANI_EBADARG;
   }
   pagani_ws*& ws = b->ws[w][d];
@@ -80,6 +81,7 @@ int run_item(pagani_batch* b, int w, const pagani_batch_item& it, pagani_result*
   pagani_ws_set_device_mem(ws, item_total(b));
   pagani_ws_set_mem_model(ws, b->mem_model);
   pagani_ws_set_timing(ws, b->timing ? 1 : 0);
+  pagani_ws_set_tuned_slices(ws, b->tuned ? 1 : 0);
   const int rc = pagani_integrate_genz(ws, &it.f, it.precision, it.epsrel, it.epsabs, nullptr,
                                        nullptr, it.relerr_classification, it.init_k, out);
   if (rc < 0 && rc != PAGANI_ENOMEM) set_error(b, pagani_ws_last_error(ws));
@@ -169,6 +171,13 @@ int pagani_batch_set_timing(pagani_batch* b, int enable)
   return PAGANI_OK;
 }
 
+int pagani_batch_set_tuned_slices(pagani_batch* b, int enable)
+{
+  if (!b) return PAGANI_EBADARG;
+  b->tuned = enable != 0;
+  return PAGANI_OK;
+}
+
 int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
 {
   if (!b || !out) return PAGANI_EBADARG;
@@ -197,6 +206,7 @@ int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
       out->device_regions += s.device_regions;
       out->device_rounds += s.device_rounds;
       out->budget_fixes += s.budget_fixes;
+      out->graph_launches += s.graph_launches;
     }
   return PAGANI_OK;
 }

@@ -9,16 +9,22 @@
 // whole loop state (the region list, the parents' estimates, the running
 // sums, the classifier's history and the per-iteration trace).
 //
-// File layout, little-endian, version 1 (all integers unsigned unless noted):
+// File layout, little-endian, version 2 (all integers unsigned unless noted):
 //   "PAGANIC1"  u32 version  u32 ndim
-//   u64 n  u64 it  i32 has_parents  i32 user_regions  i32 relerr  i32 mem_model
-//   u64 mem_total  f64 epsrel  f64 epsabs  f64 lows[8]  f64 highs[8]
+//   u64 n  u64 it  i32 has_parents  [i32 split, version 2]  i32 user_regions  i32 relerr
+//   i32 mem_model
+//   u64 mem_total  f64 epsrel  f64 epsabs  f64 lows[16]  f64 highs[16]
 //   result: f64 estimate  f64 errorest  u64 neval  u64 nregions
 //           u64 nFinishedRegions  i32 status  i32 lastPhase  f64 chi_sq  u64 iters
 //   f64 hist[3]  u64 iters_collected
-//   i32 family  i32 genz_ndim  f64 a[8]  f64 u[8]  i32 precision
+//   i32 family  i32 genz_ndim  f64 a[16]  f64 u[16]  i32 precision
 //   u64 ntrace, then ntrace x (u64 n, u64 m, f64 estimate, f64 errorest)
-//   f64 L[ndim*n]  f64 H[ndim*n]  f64 parent_est[has_parents ? n/2 : 0]
+//   f64 L[ndim*rows]  f64 H[ndim*rows]  [u8 pdim[rows] when split]
+//   f64 parent_est[has_parents ? n/2 : 0]
+//   where rows = n for a flat list and n/2 for a list in split form (the
+//   parents' bounds and the axis each is halved along; region t is child
+//   t / rows of parent t mod rows).  Version 1 files (no split field, the
+//   per-axis arrays 8 wide) hold flat lists and still load.
 //   u64 checksum of everything before it (FNV-1a over 64-bit words, the
 //       byte tail zero-padded)
 // Arrays are structure of arrays, [j*n + t], unit-cube coordinates.
@@ -38,7 +44,7 @@ using namespace pagani::internal;
 namespace {
 
 constexpr char kMagic[8] = {'P', 'A', 'G', 'A', 'N', 'I', 'C', '1'};
-constexpr std::uint32_t kVersion = 1;
+constexpr std::uint32_t kVersion = 2;
 constexpr std::size_t kStageBytes = std::size_t(64) << 20;
 
 // FNV-1a over 64-bit words; a partial word at the end of a write is held
@@ -126,28 +132,28 @@ struct Stage {
   }
 };
 
-int write_device_array(pagani_ws* ws, Writer& w, const double* d, std::size_t count, Stage& st)
+template <class T>
+int write_device_array(pagani_ws* ws, Writer& w, const T* d, std::size_t count, Stage& st)
 {
-  const std::size_t per = kStageBytes / sizeof(double);
+  const std::size_t per = kStageBytes / sizeof(T);
   for (std::size_t i = 0; i < count && w.ok; i += per) {
     const std::size_t c = std::min(per, count - i);
-    HIP_TRY(ws, hipMemcpyAsync(st.p, d + i, c * sizeof(double), hipMemcpyDeviceToHost,
-                               ws->stream));
+    HIP_TRY(ws, hipMemcpyAsync(st.p, d + i, c * sizeof(T), hipMemcpyDeviceToHost, ws->stream));
     HIP_TRY(ws, hipStreamSynchronize(ws->stream));
-    w.raw(st.p, c * sizeof(double));
+    w.raw(st.p, c * sizeof(T));
   }
   return w.ok ? PAGANI_OK : fail(ws, PAGANI_EBADARG, "checkpoint write failed");
 }
 
-int read_device_array(pagani_ws* ws, Reader& r, double* d, std::size_t count, Stage& st)
+template <class T>
+int read_device_array(pagani_ws* ws, Reader& r, T* d, std::size_t count, Stage& st)
 {
-  const std::size_t per = kStageBytes / sizeof(double);
+  const std::size_t per = kStageBytes / sizeof(T);
   for (std::size_t i = 0; i < count && r.ok; i += per) {
     const std::size_t c = std::min(per, count - i);
-    r.raw(st.p, c * sizeof(double));
+    r.raw(st.p, c * sizeof(T));
     if (!r.ok) break;
-    HIP_TRY(ws, hipMemcpyAsync(d + i, st.p, c * sizeof(double), hipMemcpyHostToDevice,
-                               ws->stream));
+    HIP_TRY(ws, hipMemcpyAsync(d + i, st.p, c * sizeof(T), hipMemcpyHostToDevice, ws->stream));
     HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   }
   return r.ok ? PAGANI_OK : fail(ws, PAGANI_EBADARG, "checkpoint truncated");
@@ -173,6 +179,7 @@ int pagani_ws_save_state(pagani_ws* ws, const char* path)
   const LoopState& s = ws->susp;
   const std::size_t d = static_cast<std::size_t>(ws->ndim);
   const std::size_t np = s.has_parents ? s.n / 2 : 0;
+  const std::size_t rows = s.split ? s.n / 2 : s.n;
   const std::string tmp = std::string(path) + ".tmp";
   FileCloser fc{std::fopen(tmp.c_str(), "wb")};
   if (!fc.f) return fail(ws, PAGANI_EBADARG, std::string("cannot open ") + tmp);
@@ -183,6 +190,7 @@ int pagani_ws_save_state(pagani_ws* ws, const char* path)
   w.put<std::uint64_t>(s.n);
   w.put<std::uint64_t>(s.it);
   w.put<std::int32_t>(s.has_parents);
+  w.put<std::int32_t>(s.split);
   w.put<std::int32_t>(s.user_regions);
   w.put<std::int32_t>(s.relerr);
   w.put<std::int32_t>(s.mem_model);
@@ -220,8 +228,9 @@ int pagani_ws_save_state(pagani_ws* ws, const char* path)
     return fail(ws, PAGANI_ENOMEM, "pinned staging buffer");
   }
   int rc;
-  if ((rc = write_device_array(ws, w, ws->L[ws->cur].p, d * s.n, st)) ||
-      (rc = write_device_array(ws, w, ws->H[ws->cur].p, d * s.n, st)) ||
+  if ((rc = write_device_array(ws, w, ws->L[ws->cur].p, d * rows, st)) ||
+      (rc = write_device_array(ws, w, ws->H[ws->cur].p, d * rows, st)) ||
+      (s.split && (rc = write_device_array(ws, w, ws->pdim[ws->cur].p, rows, st))) ||
       (rc = write_device_array(ws, w, ws->pest.p, np, st)))
     return rc;
   const std::uint64_t sum = w.ck.finish();
@@ -251,7 +260,8 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   r.raw(magic, sizeof magic);
   if (!r.ok || std::memcmp(magic, kMagic, sizeof magic) != 0)
     return fail(ws, PAGANI_EBADARG, "not a PAGANI checkpoint");
-  if (r.get<std::uint32_t>() != kVersion)
+  const std::uint32_t version = r.get<std::uint32_t>();
+  if (version != 1 && version != kVersion)
     return fail(ws, PAGANI_EBADARG, "unsupported checkpoint version");
   const std::uint32_t d = r.get<std::uint32_t>();
   if (!r.ok || static_cast<int>(d) != ws->ndim)
@@ -260,14 +270,17 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   s.n = r.get<std::uint64_t>();
   s.it = r.get<std::uint64_t>();
   s.has_parents = r.get<std::int32_t>();
+  s.split = version >= 2 ? r.get<std::int32_t>() : 0;
   s.user_regions = r.get<std::int32_t>();
   s.relerr = r.get<std::int32_t>();
   s.mem_model = r.get<std::int32_t>();
   s.mem_total = r.get<std::uint64_t>();
   s.epsrel = r.get<double>();
   s.epsabs = r.get<double>();
-  for (int j = 0; j < PAGANI_MAX_DIM; ++j) s.lows[j] = r.get<double>();
-  for (int j = 0; j < PAGANI_MAX_DIM; ++j) s.highs[j] = r.get<double>();
+  // per-axis arrays: 8 wide in version 1, PAGANI_MAX_DIM (16) from version 2
+  const int axes = version >= 2 ? PAGANI_MAX_DIM : 8;
+  for (int j = 0; j < axes; ++j) s.lows[j] = r.get<double>();
+  for (int j = 0; j < axes; ++j) s.highs[j] = r.get<double>();
   s.cum.estimate = r.get<double>();
   s.cum.errorest = r.get<double>();
   s.cum.neval = r.get<std::uint64_t>();
@@ -281,18 +294,20 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   s.iters_collected = r.get<std::uint64_t>();
   s.genz.family = r.get<std::int32_t>();
   s.genz.ndim = r.get<std::int32_t>();
-  for (int j = 0; j < PAGANI_MAX_DIM; ++j) s.genz.a[j] = r.get<double>();
-  for (int j = 0; j < PAGANI_MAX_DIM; ++j) s.genz.u[j] = r.get<double>();
+  for (int j = 0; j < axes; ++j) s.genz.a[j] = r.get<double>();
+  for (int j = 0; j < axes; ++j) s.genz.u[j] = r.get<double>();
   s.precision = r.get<std::int32_t>();
   const std::uint64_t ntrace = r.get<std::uint64_t>();
   if (!r.ok) return fail(ws, PAGANI_EBADARG, "checkpoint truncated");
   // sizes must match the file before anything is allocated
   const std::uint64_t np = s.has_parents ? s.n / 2 : 0;
+  const std::uint64_t rows = s.split ? s.n / 2 : s.n;
   const long head = std::ftell(fc.f);
   const double want = static_cast<double>(head) + 32.0 * static_cast<double>(ntrace) +
-                      8.0 * (2.0 * d * static_cast<double>(s.n) + static_cast<double>(np)) + 8.0;
-  if (fsize < 0 || static_cast<double>(fsize) != want || (s.has_parents && (s.n & 1)) ||
-      s.n == 0 || (s.has_parents != 0 && s.has_parents != 1))
+                      8.0 * (2.0 * d * static_cast<double>(rows) + static_cast<double>(np)) +
+                      (s.split ? static_cast<double>(rows) : 0.0) + 8.0;
+  if (fsize < 0 || static_cast<double>(fsize) != want || ((s.has_parents || s.split) && (s.n & 1)) ||
+      s.n == 0 || (s.has_parents != 0 && s.has_parents != 1) || (s.split != 0 && s.split != 1))
     return fail(ws, PAGANI_EBADARG, "checkpoint size or header inconsistent");
   std::vector<TraceRec> trace(ntrace);
   for (TraceRec& t : trace) {
@@ -311,8 +326,9 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   ws->live_evaluated = false;
   ws->snap_n = 0;
   ws->trace.clear();
-  if ((rc = ensure(ws, ws->L[cur], d * s.n, "region bounds")) ||
-      (rc = ensure(ws, ws->H[cur], d * s.n, "region bounds")) ||
+  if ((rc = ensure(ws, ws->L[cur], d * rows, "region bounds")) ||
+      (rc = ensure(ws, ws->H[cur], d * rows, "region bounds")) ||
+      (s.split && (rc = ensure(ws, ws->pdim[cur], rows, "split axes"))) ||
       (np && (rc = ensure(ws, ws->pest, np, "parent estimates"))))
     return rc;
   Stage st;
@@ -320,8 +336,9 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
     (void)hipGetLastError();
     return fail(ws, PAGANI_ENOMEM, "pinned staging buffer");
   }
-  if ((rc = read_device_array(ws, r, ws->L[cur].p, d * s.n, st)) ||
-      (rc = read_device_array(ws, r, ws->H[cur].p, d * s.n, st)) ||
+  if ((rc = read_device_array(ws, r, ws->L[cur].p, d * rows, st)) ||
+      (rc = read_device_array(ws, r, ws->H[cur].p, d * rows, st)) ||
+      (s.split && (rc = read_device_array(ws, r, ws->pdim[cur].p, rows, st))) ||
       (rc = read_device_array(ws, r, ws->pest.p, np, st)))
     return rc;
   const std::uint64_t expect = r.ck.finish();
@@ -333,6 +350,7 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   ws->trace = std::move(trace);
   ws->live_n = s.n;
   ws->live_evaluated = false;
+  ws->split = s.split != 0;
   return PAGANI_OK;
 }
 
@@ -346,13 +364,18 @@ int pagani_ws_write_regions_csv(pagani_ws* ws, const char* path)
   if (!ws || !path) return PAGANI_EBADARG;
   if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
   const std::size_t n = ws->live_n, d = static_cast<std::size_t>(ws->ndim);
-  std::vector<double> L(n * d), H(n * d), est, err;
-  std::vector<std::uint8_t> dim;
+  // a list in split form is read as its parents and written out child by
+  // child, each derived as the evaluation derives it
+  const std::size_t rows = ws->split ? n / 2 : n;
+  std::vector<double> L(rows * d), H(rows * d), est, err;
+  std::vector<std::uint8_t> dim, pdim(ws->split ? rows : 0);
   if (n) {
-    HIP_TRY(ws, hipMemcpy(L.data(), ws->L[ws->cur].p, n * d * sizeof(double),
+    HIP_TRY(ws, hipMemcpy(L.data(), ws->L[ws->cur].p, rows * d * sizeof(double),
                           hipMemcpyDeviceToHost));
-    HIP_TRY(ws, hipMemcpy(H.data(), ws->H[ws->cur].p, n * d * sizeof(double),
+    HIP_TRY(ws, hipMemcpy(H.data(), ws->H[ws->cur].p, rows * d * sizeof(double),
                           hipMemcpyDeviceToHost));
+    if (ws->split)
+      HIP_TRY(ws, hipMemcpy(pdim.data(), ws->pdim[ws->cur].p, rows, hipMemcpyDeviceToHost));
     if (ws->live_evaluated) {
       est.resize(n);
       err.resize(n);
@@ -371,8 +394,16 @@ int pagani_ws_write_regions_csv(pagani_ws* ws, const char* path)
       ok = std::fprintf(fc.f, "%.17g,%.17g,", est[t], err[t]) > 0;
     else
       ok = std::fprintf(fc.f, "nan,nan,") > 0;
+    const bool right = ws->split && t >= rows;
+    const std::size_t row = right ? t - rows : t;
     for (std::size_t j = 0; j < d && ok; ++j) {
-      const double lo = L[j * n + t], up = lo + H[j * n + t];
+      double lo = L[j * rows + row], len = H[j * rows + row];
+      if (ws->split && j == pdim[row]) {
+        const double half = len / 2;
+        if (right) lo = lo + half;
+        len = half;
+      }
+      const double up = lo + len;
       ok = std::fprintf(fc.f, "%.17g,%.17g,", up, lo) > 0;
     }
     if (ok)

@@ -95,12 +95,17 @@ __device__ __forceinline__ void decide_step(DevLoop* s, double iter_est, double 
     s->cp.m = m;
     s->cp.cur = s->it.cur;
     s->cp.has_parents = s->it.has_parents;
+    s->cp.split = s->it.split;
     s->cp.run = 1;
+    // the kept regions become the next list's parents: 2m children in
+    // split form in the other pair
     s->it.n = 2 * m;
     s->it.cur ^= 1u;
     s->it.has_parents = 1;
+    s->it.split = 1;
     s->it.slices = static_cast<std::uint32_t>(
-        choose_slices_n(2 * m, s->num_cus, s->slices_override, s->generic));
+        choose_slices_n(2 * m, s->num_cus, s->slices_override, s->generic,
+                        s->has_table ? s->slice_table : nullptr));
     // the round's launches have launch_slices waves per workgroup and
     // grids for round_cap regions: an iteration that needs another slice
     // count (idle waves would take the place of working ones) or a larger

@@ -26,6 +26,12 @@
 
 #include "ws_internal.hpp"
 
+// kTunedSlices[family][ndim][precision][size class]: the launch shapes
+// gpuintegration_amd/tune.py measured (gpuintegration_amd/tuning/
+// eval_slices.json), written out as a C table by the build; family 0 is an
+// evaluation callback, precision 0 FP64 and 1 FP32; 0 = no measurement.
+#include "tuned_slices.inc"
+
 namespace pagani {
 namespace internal {
 
@@ -65,10 +71,22 @@ int pool_handover(pagani_ws* from, pagani_ws* to)
 // ---------------------------------------------------------------------------
 // evaluation launch data
 
+// The tuned table of the workspace's running integrand, or NULL (the rule).
+const std::uint8_t* tuned_table(const pagani_ws* ws)
+{
+  if (!ws->tuned || ws->tune_family < 0 || ws->tune_family > PAGANI_GENZ_DISCONTINUOUS)
+    return nullptr;
+  const std::uint8_t* t =
+      kTunedSlices[ws->tune_family][ws->ndim][ws->tune_precision == PAGANI_FP32 ? 1 : 0];
+  for (int b = 0; b < kSliceBuckets; ++b)
+    if (t[b]) return t;
+  return nullptr;
+}
+
 int choose_slices(const pagani_ws* ws, std::size_t n)
 {
   return choose_slices_n(n, ws->num_cus, ws->slices_override,
-                         ws->rule.feval - ws->rule.set_begin[3]);
+                         ws->rule.feval - ws->rule.set_begin[3], tuned_table(ws));
 }
 
 void set_slices(pagani_eval_consts& C, int S);
@@ -77,6 +95,7 @@ void make_consts(const pagani_ws* ws, const double* lows, const double* highs, s
                  pagani_eval_consts& C)
 {
   std::memset(&C, 0, sizeof(C));
+  C.abi_version = PAGANI_EVAL_ABI_VERSION;
   const int d = ws->ndim;
   const pagani::RuleTables& R = ws->rule;
   std::memcpy(C.w, R.w, sizeof(C.w));
@@ -337,6 +356,30 @@ int grow_keep(pagani_ws* ws, DevBuf<T>& b, std::size_t need, std::size_t keep, c
 // rest as no-op launches, so the length trades round trips for those.
 constexpr int kDevQueueMax = 16;
 
+// Full-length rounds of the device-resident loop captured as HIP graphs:
+// every launch of a round is fixed by the round's length, its grids (the
+// largest list of its slice count) and its slice count, so a later round
+// with the same three replays the graph (one launch instead of 48).  Kept
+// for one segment (the buffers it points at are fixed for the segment).
+struct RoundGraphs {
+  struct Entry {
+    std::uint64_t rcap;
+    int slices;
+    hipGraphExec_t exec;
+  };
+  std::vector<Entry> entries;
+  hipGraphExec_t find(std::uint64_t rcap, int slices) const
+  {
+    for (const Entry& e : entries)
+      if (e.rcap == rcap && e.slices == slices) return e.exec;
+    return nullptr;
+  }
+  ~RoundGraphs()
+  {
+    for (Entry& e : entries) (void)hipGraphExecDestroy(e.exec);
+  }
+};
+
 // What a device-resident segment ended with (see run_device_segment).
 enum DevOutcome { kSegHost, kSegDone, kSegEmpty, kSegLimit };
 
@@ -366,14 +409,17 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   const std::size_t cap = ph::kDevLoopMax;
   const int cur = ws->cur;
   int rc;
-  // the live list and the parents' estimates are kept; everything else is
-  // sized for the capacity
-  if ((rc = grow_keep(ws, ws->L[cur], cap * d, v.n * d, "region bounds")) ||
-      (rc = grow_keep(ws, ws->H[cur], cap * d, v.n * d, "region bounds")) ||
+  // the live list (its own bounds, or its parents' and their halved axes)
+  // and the parents' estimates are kept; everything else is sized for the
+  // capacity
+  const std::size_t rows = ws->split ? v.n / 2 : v.n;
+  if ((rc = grow_keep(ws, ws->L[cur], cap * d, rows * d, "region bounds")) ||
+      (rc = grow_keep(ws, ws->H[cur], cap * d, rows * d, "region bounds")) ||
+      (rc = grow_keep(ws, ws->pdim[cur], cap / 2, ws->split ? rows : 0, "split axes")) ||
       (rc = grow_keep(ws, ws->pest, cap / 2, v.has_parents ? v.n / 2 : 0, "parent estimates")) ||
       (rc = ensure(ws, ws->L[1 - cur], cap * d, "region bounds")) ||
       (rc = ensure(ws, ws->H[1 - cur], cap * d, "region bounds")) ||
-      (rc = ensure(ws, ws->perr, cap / 2, "parent errors")) ||
+      (rc = ensure(ws, ws->pdim[1 - cur], cap / 2, "split axes")) ||
       (rc = ensure(ws, ws->est, cap, "estimates")) || (rc = ensure(ws, ws->err0, cap, "errors")) ||
       (rc = ensure(ws, ws->err1, cap, "errors")) || (rc = ensure(ws, ws->dim, cap, "split axes")) ||
       (rc = ensure(ws, ws->active, cap, "flags")) || (rc = ensure_chunks(ws, cap)))
@@ -386,8 +432,13 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   h.it.run = 1;
   h.it.has_parents = v.has_parents ? 1u : 0u;
   h.it.slices = static_cast<std::uint32_t>(choose_slices(ws, v.n));
+  h.it.split = ws->split ? 1u : 0u;
   h.num_cus = ws->num_cus;
   h.slices_override = ws->slices_override;
+  if (const std::uint8_t* t = tuned_table(ws)) {
+    h.has_table = 1;
+    std::memcpy(h.slice_table, t, sizeof(h.slice_table));
+  }
   h.generic = ws->rule.feval - ws->rule.set_begin[3];
   h.n_cap = cap;
   h.mem_total = v.clf->total;
@@ -412,12 +463,15 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   args.H = ws->H[0].p;
   args.L2 = ws->L[1].p;
   args.H2 = ws->H[1].p;
+  args.pdim = ws->pdim[0].p;
+  args.pdim2 = ws->pdim[1].p;
   args.est = ws->est.p;
   args.err = ws->err0.p;
   args.dim = ws->dim.p;
   args.stream = ws->stream;
   args.dev = &ws->d_loop->it;
   make_consts(ws, lows, highs, cap, args.consts);
+  RoundGraphs graphs;
 
   for (int queued = 2;; queued = std::min(queued * 2, kDevQueueMax)) {
     // A round's launches have as many waves per workgroup as its first
@@ -446,17 +500,45 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
     HIP_TRY(ws, hipMemcpyAsync(ws->d_loop, ws->h_loop, sizeof(DevLoop), hipMemcpyHostToDevice,
                                ws->stream));
     if (ws->timing) record(ws, EV_DEV0);
-    for (int q = 0; q < queued; ++q) {
-      rc = eval(ctx, &args);
-      if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
-      HIP_TRY(ws, ph::step_dev(ws->stream, rcap, ws->d_loop, ws->est.p, ws->err0.p, ws->err1.p,
-                               ws->pest.p, epsrel, relerr, ws->active.p, ws->part.p, ws->cnt.p,
-                               ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p, ws->d_scal,
-                               &ws->d_loop->ticket));
-      HIP_TRY(ws, ph::compact_dev(ws->stream, rcap, &ws->d_loop->cp, d, ws->L[0].p, ws->H[0].p,
-                                  ws->L[1].p, ws->H[1].p, ws->active.p, ws->est.p, ws->err0.p,
-                                  ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,
-                                  ws->perr.p));
+    // the round's launches: `queued` iterations of evaluation, refinement
+    // with reductions and decisions, filter + split
+    auto enqueue = [&]() -> int {
+      for (int q = 0; q < queued; ++q) {
+        const int erc = eval(ctx, &args);
+        if (erc) return fail(ws, erc < 0 ? erc : PAGANI_EHIP, "evaluation callback failed");
+        HIP_TRY(ws, ph::step_dev(ws->stream, rcap, ws->d_loop, ws->est.p, ws->err0.p,
+                                 ws->err1.p, ws->pest.p, epsrel, relerr, ws->active.p, ws->part.p,
+                                 ws->cnt.p, ws->off.p, ws->fpart.p, ws->fcnt.p, ws->foff.p,
+                                 ws->d_scal, &ws->d_loop->ticket));
+        HIP_TRY(ws, ph::compact_dev(ws->stream, rcap, &ws->d_loop->cp, d, ws->L[0].p,
+                                    ws->H[0].p, ws->L[1].p, ws->H[1].p, ws->pdim[0].p,
+                                    ws->pdim[1].p, ws->active.p, ws->est.p, ws->err0.p,
+                                    ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,
+                                    nullptr));
+      }
+      return PAGANI_OK;
+    };
+    if (ws->graphs && queued == kDevQueueMax) {
+      hipGraphExec_t exec = graphs.find(rcap, h.launch_slices);
+      if (!exec) {
+        HIP_TRY(ws, hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+        rc = enqueue();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ws->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess) return hip_fail(ws, e, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) return hip_fail(ws, ie, "hipGraphInstantiate");
+        graphs.entries.push_back({rcap, h.launch_slices, exec});
+      }
+      HIP_TRY(ws, hipGraphLaunch(exec, ws->stream));
+      ws->stats.graph_launches++;
+    } else if ((rc = enqueue())) {
+      return rc;
     }
     if (ws->timing) record(ws, EV_DEV1);
     HIP_TRY(ws, hipMemcpyAsync(ws->h_loop, ws->d_loop, sizeof(DevLoop), hipMemcpyDeviceToHost,
@@ -481,6 +563,7 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   v.n = h.it.n;
   v.has_parents = h.it.has_parents != 0;
   ws->cur = static_cast<int>(h.it.cur);
+  ws->split = h.it.split != 0;
   v.cum->estimate = h.cum_est;
   v.cum->errorest = h.cum_err;
   v.cum->nregions = h.cum_nregions;
@@ -536,6 +619,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
   epsrel = st.epsrel;
   epsabs = st.epsabs;
   relerr = st.relerr;
+  ws->tune_family = st.genz.family;
+  ws->tune_precision = st.precision == PAGANI_FP32 ? PAGANI_FP32 : PAGANI_FP64;
   const double* lows = st.lows;
   const double* highs = st.highs;
   const bool user_regions = st.user_regions != 0;
@@ -553,7 +638,9 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
   std::size_t n = 0;
   if (resume) {
     n = st.n;
+    ws->split = st.split != 0;
   } else if (init) {
+    ws->split = false;
     n = init->n;
     if (n == 0 || !init->L || !init->H) return fail(ws, PAGANI_EBADARG, "empty region list");
     if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
@@ -567,6 +654,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     // integrations alternate the ping-pong pairs identically and a pool
     // grown by the first finds every buffer at its size in the next
     ws->cur = 0;
+    ws->split = false;
     std::size_t k = init_k ? init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
     n = 1;
     for (int j = 0; j < d; ++j) {
@@ -597,6 +685,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       st.it = it;
       st.n = n;
       st.has_parents = has_parents ? 1 : 0;
+      st.split = ws->split ? 1 : 0;
       st.cum = cum;
       st.hist[0] = clf.hist[0];
       st.hist[1] = clf.hist[1];
@@ -645,6 +734,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       args.n = n;
       args.L = ws->L[ws->cur].p;
       args.H = ws->H[ws->cur].p;
+      args.pdim = ws->split ? ws->pdim[ws->cur].p : nullptr;
       args.est = ws->est.p;
       args.err = ws->err0.p;
       args.dim = ws->dim.p;
@@ -699,10 +789,15 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       if ((rc = ensure(ws, ws->snapL, n * d, "snapshot")) ||
           (rc = ensure(ws, ws->snapH, n * d, "snapshot")))
         return rc;
-      HIP_TRY(ws, hipMemcpyAsync(ws->snapL.p, ws->L[ws->cur].p, n * d * sizeof(double),
-                                 hipMemcpyDeviceToDevice, ws->stream));
-      HIP_TRY(ws, hipMemcpyAsync(ws->snapH.p, ws->H[ws->cur].p, n * d * sizeof(double),
-                                 hipMemcpyDeviceToDevice, ws->stream));
+      if (ws->split) {
+        HIP_TRY(ws, ph::split_all(ws->stream, d, n / 2, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                                  ws->pdim[ws->cur].p, ws->snapL.p, ws->snapH.p));
+      } else {
+        HIP_TRY(ws, hipMemcpyAsync(ws->snapL.p, ws->L[ws->cur].p, n * d * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ws->stream));
+        HIP_TRY(ws, hipMemcpyAsync(ws->snapH.p, ws->H[ws->cur].p, n * d * sizeof(double),
+                                   hipMemcpyDeviceToDevice, ws->stream));
+      }
       ws->snap_n = n;
     }
 
@@ -781,10 +876,10 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       }
       if (m > 0) {
         const int nxt = 1 - ws->cur;
-        if ((rc = ensure(ws, ws->L[nxt], 2 * m * d, "region bounds")) ||
-            (rc = ensure(ws, ws->H[nxt], 2 * m * d, "region bounds")) ||
-            (rc = ensure(ws, ws->pest, m, "parent estimates")) ||
-            (rc = ensure(ws, ws->perr, m, "parent errors"))) {
+        if ((rc = ensure(ws, ws->L[nxt], m * d, "region bounds")) ||
+            (rc = ensure(ws, ws->H[nxt], m * d, "region bounds")) ||
+            (rc = ensure(ws, ws->pdim[nxt], m, "split axes")) ||
+            (rc = ensure(ws, ws->pest, m, "parent estimates"))) {
           // the device cannot hold the next split: stop like the reference's
           // memory termination (status 1, all leaves summed)
           ws->trace.push_back({n, 0, cum.estimate + iter_est, tot_err});
@@ -806,12 +901,16 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     } else {
       const int nxt = 1 - ws->cur;
       record(ws, EV_CMP0);
-      HIP_TRY(ws, ph::compact(ws->stream, 1, d, n, m, ws->L[ws->cur].p, ws->H[ws->cur].p,
-                              ws->active.p, ws->est.p, errs, ws->dim.p, ws->off.p, ws->foff.p,
-                              ws->L[nxt].p, ws->H[nxt].p, ws->pest.p, ws->perr.p, nullptr));
+      // the kept regions become the next list's parents (split form)
+      HIP_TRY(ws, ph::compact_parents(ws->stream, d, n, m, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                                      ws->split ? ws->pdim[ws->cur].p : nullptr, ws->active.p,
+                                      ws->est.p, errs, ws->dim.p, ws->off.p, ws->foff.p,
+                                      ws->L[nxt].p, ws->H[nxt].p, ws->pdim[nxt].p, ws->pest.p,
+                                      nullptr));
       record(ws, EV_CMP1);
       ws->compact_pending = ws->timing;
       ws->cur = nxt;
+      ws->split = true;
       n = 2 * m;
       has_parents = true;
     }
@@ -849,6 +948,8 @@ int check_genz(pagani_ws* ws, const pagani_genz* f, int precision)
 {
   if (!f) return fail(ws, PAGANI_EBADARG, "null integrand");
   if (f->ndim != ws->ndim) return fail(ws, PAGANI_EBADARG, "integrand ndim != workspace ndim");
+  if (f->ndim > PAGANI_GENZ_MAX_DIM)
+    return fail(ws, PAGANI_EUNSUPPORTED, "Genz descriptors cover ndim 2..10");
   if (f->family < PAGANI_GENZ_OSCILLATORY || f->family > PAGANI_GENZ_DISCONTINUOUS)
     return fail(ws, PAGANI_EBADARG, "unknown integrand family");
   if (precision != PAGANI_FP64 && precision != PAGANI_FP32)
@@ -899,7 +1000,7 @@ pagani_ws* pagani_ws_create(int ndim, int device)
     return nullptr;
   };
   if (ndim < PAGANI_MIN_DIM || ndim > PAGANI_MAX_DIM)
-    return bail(nullptr, "ndim outside [2, 8]", hipSuccess);
+    return bail(nullptr, "ndim outside [2, 16]", hipSuccess);
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return bail(nullptr, "hipGetDeviceCount", e);
@@ -947,36 +1048,19 @@ pagani_ws* pagani_ws_create(int ndim, int device)
 void pagani_ws_destroy(pagani_ws* ws)
 {
   if (!ws) return;
-  if (ws->stream) hipStreamSynchronize(ws->stream);
-  for (int i = 0; i < 2; ++i) {
-    release(ws->L[i]);
-    release(ws->H[i]);
-  }
-  release(ws->est);
-  release(ws->err0);
-  release(ws->err1);
-  release(ws->pest);
-  release(ws->perr);
-  release(ws->dim);
-  release(ws->active);
-  release(ws->flags);
-  release(ws->part);
-  release(ws->cnt);
-  release(ws->off);
-  release(ws->fpart);
-  release(ws->fcnt);
-  release(ws->foff);
-  if (ws->d_mpart) hipFree(ws->d_mpart);
-  if (ws->d_mout) hipFree(ws->d_mout);
-  if (ws->h_mout) hipHostFree(ws->h_mout);
-  if (ws->d_loop) hipFree(ws->d_loop);
-  if (ws->h_loop) hipHostFree(ws->h_loop);
-  if (ws->d_gtab) hipFree(ws->d_gtab);
-  if (ws->d_scal) hipFree(ws->d_scal);
-  if (ws->h_scal) hipHostFree(ws->h_scal);
+  if (ws->stream) (void)hipStreamSynchronize(ws->stream);
+  for_each_pool(ws, [](auto& b) { release(b); });
+  if (ws->d_mpart) (void)hipFree(ws->d_mpart);
+  if (ws->d_mout) (void)hipFree(ws->d_mout);
+  if (ws->h_mout) (void)hipHostFree(ws->h_mout);
+  if (ws->d_loop) (void)hipFree(ws->d_loop);
+  if (ws->h_loop) (void)hipHostFree(ws->h_loop);
+  if (ws->d_gtab) (void)hipFree(ws->d_gtab);
+  if (ws->d_scal) (void)hipFree(ws->d_scal);
+  if (ws->h_scal) (void)hipHostFree(ws->h_scal);
   for (auto& e : ws->ev)
-    if (e) hipEventDestroy(e);
-  if (ws->stream) hipStreamDestroy(ws->stream);
+    if (e) (void)hipEventDestroy(e);
+  if (ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
 }
 
@@ -1068,9 +1152,24 @@ size_t pagani_ws_trace(const pagani_ws* ws, size_t cap, uint64_t* n, uint64_t* m
   return ws->trace.size();
 }
 
-int pagani_ws_live_regions(const pagani_ws* ws, size_t* n, const double** L, const double** H)
+int pagani_ws_live_regions(pagani_ws* ws, size_t* n, const double** L, const double** H)
 {
   if (!ws || !n || !L || !H) return PAGANI_EBADARG;
+  if (ws->split && ws->live_n > 0) {
+    // write the list out flat into the other pair, which then holds it
+    if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
+    const int d = ws->ndim, nxt = 1 - ws->cur;
+    int rc;
+    if ((rc = ensure(ws, ws->L[nxt], ws->live_n * d, "region bounds")) ||
+        (rc = ensure(ws, ws->H[nxt], ws->live_n * d, "region bounds")))
+      return rc;
+    HIP_TRY(ws, ph::split_all(ws->stream, d, ws->live_n / 2, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                              ws->pdim[ws->cur].p, ws->L[nxt].p, ws->H[nxt].p));
+    HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+    ws->cur = nxt;
+    ws->split = false;
+    if (ws->susp.held) ws->susp.split = 0;
+  }
   *n = ws->live_n;
   *L = ws->L[ws->cur].p;
   *H = ws->H[ws->cur].p;
@@ -1161,6 +1260,20 @@ int pagani_integrate_genz_resume(pagani_ws* ws, const pagani_genz* f, int precis
                         &ws->susp, nullptr, 0, out);
 }
 
+int pagani_ws_set_graphs(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->graphs = enable != 0;
+  return PAGANI_OK;
+}
+
+int pagani_ws_set_tuned_slices(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->tuned = enable != 0;
+  return PAGANI_OK;
+}
+
 int pagani_ws_set_predict_split(pagani_ws* ws, int enable)
 {
   if (!ws) return PAGANI_EBADARG;
@@ -1214,6 +1327,8 @@ int pagani_eval_genz_dev(pagani_ws* ws, const pagani_genz* f, int precision, siz
   args.err = err;
   args.dim = dim;
   args.stream = ws->stream;
+  ws->tune_family = f->family;
+  ws->tune_precision = precision;
   make_consts(ws, lows, highs, n, args.consts);
   record(ws, EV_EVAL0);
   rc = pagani_genz_eval_launch(f, precision, &args);

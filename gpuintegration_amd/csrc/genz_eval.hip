@@ -1,5 +1,5 @@
 // genz_eval.hip -- precompiled evaluation kernels for the Genz descriptor
-// path: 6 families x ndim 2..8 x {FP64, FP32 integrand}.
+// path: 6 families x ndim 2..10 x {FP64, FP32 integrand}.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,6 +30,8 @@ int launch_dim(const pagani_genz* g, const pagani_eval_args* a)
   case 6: return launch<Real, 6>(g, a);
   case 7: return launch<Real, 7>(g, a);
   case 8: return launch<Real, 8>(g, a);
+  case 9: return launch<Real, 9>(g, a);
+  case 10: return launch<Real, 10>(g, a);
   default: return PAGANI_EUNSUPPORTED;
   }
 }

@@ -6,6 +6,7 @@
 #include "phases.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 #include "devloop_decide.hpp"
 
@@ -165,30 +166,32 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
   write_record(se, sr, sae, sar, c, part, cnt, chunk);
 }
 
+// A region's sibling lies n/2 away: workgroups are paired so that a chunk
+// and its sibling chunk are read by workgroups running at the same time on
+// the same XCD (workgroups are dealt to the 8 XCDs round robin, so b and
+// b + 8 share one), and the second read of the siblings and their parent
+// comes from that XCD's L2.  In groups of 16 workgroups, the first 8 take 8
+// chunks of the first half and the next 8 their sibling chunks; the chunks
+// left over alternate between the halves.
+__device__ __forceinline__ std::size_t paired_chunk(std::size_t b, std::size_t nch)
+{
+  const std::size_t half = (nch + 1) / 2, groups = (nch - half) / 8;
+  if (b < 16 * groups) {
+    const std::size_t g = b / 16, r = b % 16;
+    return r < 8 ? g * 8 + r : half + g * 8 + (r - 8);
+  }
+  const std::size_t t = b - 16 * groups;
+  return (t & 1) ? half + 8 * groups + (t >> 1) : 8 * groups + (t >> 1);
+}
+
 __global__ void __launch_bounds__(kThreads)
     k_refine(std::size_t n, const double* __restrict__ est, const double* err_in,
              double* err_out, const double* __restrict__ pest, double epsrel, int relerr,
              std::uint8_t* __restrict__ active, double* __restrict__ part,
              std::uint32_t* __restrict__ cnt)
 {
-  // A region's sibling lies n/2 away: workgroups are paired so that a
-  // chunk and its sibling chunk are read by workgroups running at the same
-  // time on the same XCD (workgroups are dealt to the 8 XCDs round robin,
-  // so b and b + 8 share one), and the second read of the siblings and
-  // their parents comes from that XCD's L2.  In groups of 16 workgroups, the
-  // first 8 take 8 chunks of the first half and the next 8 their sibling
-  // chunks; the chunks left over alternate between the halves.
-  const std::size_t b = blockIdx.x, nch = gridDim.x;
-  const std::size_t half = (nch + 1) / 2, groups = (nch - half) / 8;
-  std::size_t chunk;
-  if (b < 16 * groups) {
-    const std::size_t g = b / 16, r = b % 16;
-    chunk = r < 8 ? g * 8 + r : half + g * 8 + (r - 8);
-  } else {
-    const std::size_t t = b - 16 * groups;
-    chunk = (t & 1) ? half + 8 * groups + (t >> 1) : 8 * groups + (t >> 1);
-  }
-  refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt, chunk);
+  refine_chunk(n, est, err_in, err_out, pest, epsrel, relerr, active, part, cnt,
+               paired_chunk(blockIdx.x, gridDim.x));
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -711,8 +714,9 @@ constexpr int kCompactThreads = 1024;
 // alignRegions (Sub_region_filter.cuh:8-47) and, with split, the bisection
 // of divideIntervalsGPU (Sub_region_splitter.cuh:8-46) fused: the active
 // region t at compacted index idx produces children idx and m+idx.
+template <int D>
 __device__ __forceinline__ void compact_chunk(
-    int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
+    int split, std::size_t n, std::size_t m, const double* __restrict__ L,
     const double* __restrict__ H, const std::uint8_t* __restrict__ active,
     const double* __restrict__ est, const double* __restrict__ err,
     const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
@@ -728,7 +732,7 @@ __device__ __forceinline__ void compact_chunk(
     // the split's reads do not wait for the rank: a region's bounds are
     // requested before the workgroup's scan (whole cache lines are read for
     // a partly active wave in any case)
-    double lv[PAGANI_MAX_DIM], hv[PAGANI_MAX_DIM];
+    double lv[D], hv[D];
     double ev = 0.0, rv = 0.0;
     int b = 0;
     if (a) {
@@ -738,11 +742,9 @@ __device__ __forceinline__ void compact_chunk(
     if (split && a) {
       b = dim[t];
 #pragma unroll
-      for (int j = 0; j < PAGANI_MAX_DIM; ++j) {
-        if (j < d) {
-          lv[j] = L[static_cast<std::size_t>(j) * n + t];
-          hv[j] = H[static_cast<std::size_t>(j) * n + t];
-        }
+      for (int j = 0; j < D; ++j) {
+        lv[j] = L[static_cast<std::size_t>(j) * n + t];
+        hv[j] = H[static_cast<std::size_t>(j) * n + t];
       }
     }
     std::uint32_t rank;
@@ -753,8 +755,7 @@ __device__ __forceinline__ void compact_chunk(
         if (split) {
           const std::size_t n2 = 2 * m;
 #pragma unroll
-          for (int j = 0; j < PAGANI_MAX_DIM; ++j) {
-            if (j >= d) break;
+          for (int j = 0; j < D; ++j) {
             const double l = lv[j];
             const double h = hv[j];
             const std::size_t o = static_cast<std::size_t>(j) * n2 + idx;
@@ -772,7 +773,7 @@ __device__ __forceinline__ void compact_chunk(
             }
           }
         } else {
-          for (int j = 0; j < d; ++j) {
+          for (int j = 0; j < D; ++j) {
             L2[static_cast<std::size_t>(j) * m + idx] = L[static_cast<std::size_t>(j) * n + t];
             H2[static_cast<std::size_t>(j) * m + idx] = H[static_cast<std::size_t>(j) * n + t];
           }
@@ -786,8 +787,9 @@ __device__ __forceinline__ void compact_chunk(
   }
 }
 
+template <int D>
 __global__ void __launch_bounds__(kCompactThreads)
-    k_compact(int split, int d, std::size_t n, std::size_t m, const double* __restrict__ L,
+    k_compact(int split, std::size_t n, std::size_t m, const double* __restrict__ L,
               const double* __restrict__ H, const std::uint8_t* __restrict__ active,
               const double* __restrict__ est, const double* __restrict__ err,
               const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
@@ -795,27 +797,122 @@ __global__ void __launch_bounds__(kCompactThreads)
               double* __restrict__ H2, double* __restrict__ pest, double* __restrict__ perr,
               std::uint8_t* __restrict__ dim2)
 {
-  compact_chunk(split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr, dim2);
+  compact_chunk<D>(split, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr, dim2);
+}
+
+// The active regions of an n-region list (flat, or in split form when pdim
+// is given) become the parents of the next list: region t at compacted
+// index idx writes its own bounds (derived from its parent's in split form,
+// as the evaluation derives them) at [j*m + idx], the axis it is to be
+// halved along (dim[t]) at pdim2[idx], and its estimate (and, when perr is
+// given, its error) at pest/perr[idx].  The chunk's offsets come from finalize, so chunks may
+// run in any order: sibling chunks (n/2 apart, one parent's two children)
+// are processed together, and the second read of a parent hits the L2.
+template <int D>
+__device__ __forceinline__ void compact_parents_chunk(
+    std::size_t n, std::size_t m, const double* __restrict__ L,
+    const double* __restrict__ H, const std::uint8_t* __restrict__ pdim,
+    const std::uint8_t* __restrict__ active, const double* __restrict__ est,
+    const double* __restrict__ err, const std::uint8_t* __restrict__ dim,
+    const std::uint64_t* __restrict__ off, const std::uint64_t* __restrict__ foff,
+    double* __restrict__ L2, double* __restrict__ H2, std::uint8_t* __restrict__ pdim2,
+    double* __restrict__ pest, double* __restrict__ perr, std::size_t chunk)
+{
+  __shared__ std::uint32_t wsum[kCompactThreads / 64];
+  const std::size_t base = chunk * kChunk;
+  std::uint64_t run = off[chunk] + foff[chunk / kFold];
+  const std::size_t h = n / 2;
+  for (int k = 0; k < static_cast<int>(kChunk / kCompactThreads); ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
+    const bool a = t < n && active[t] == 1;
+    double lv[D], hv[D];
+    double ev = 0.0, rv = 0.0;
+    int b = 0;
+    if (a) {
+      ev = est[t];
+      if (perr) rv = err[t];
+      b = dim[t];
+      // the row this region's bounds come from, and the axis its parent
+      // halves (none for a flat list)
+      std::size_t row = t, stride = n;
+      int axis = -1;
+      bool right = false;
+      if (pdim) {
+        right = t >= h;
+        row = right ? t - h : t;
+        stride = h;
+        axis = pdim[row];
+      }
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        double l = L[static_cast<std::size_t>(j) * stride + row];
+        double w = H[static_cast<std::size_t>(j) * stride + row];
+        if (j == axis) {
+          const double half = w / 2;
+          if (right) l = l + half;
+          w = half;
+        }
+        lv[j] = l;
+        hv[j] = w;
+      }
+    }
+    std::uint32_t rank;
+    const std::uint32_t tot = chunk_rank<kCompactThreads>(a, rank, wsum);
+    if (a) {
+      const std::size_t idx = run + rank;
+      if (idx < m) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          L2[static_cast<std::size_t>(j) * m + idx] = lv[j];
+          H2[static_cast<std::size_t>(j) * m + idx] = hv[j];
+        }
+        pdim2[idx] = static_cast<std::uint8_t>(b);
+        pest[idx] = ev;
+        if (perr) perr[idx] = rv;
+      }
+    }
+    run += tot;
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(kCompactThreads)
+    k_compact_parents(std::size_t n, std::size_t m, const double* __restrict__ L,
+                      const double* __restrict__ H, const std::uint8_t* __restrict__ pdim,
+                      const std::uint8_t* __restrict__ active, const double* __restrict__ est,
+                      const double* __restrict__ err, const std::uint8_t* __restrict__ dim,
+                      const std::uint64_t* __restrict__ off,
+                      const std::uint64_t* __restrict__ foff, double* __restrict__ L2,
+                      double* __restrict__ H2, std::uint8_t* __restrict__ pdim2,
+                      double* __restrict__ pest, double* __restrict__ perr)
+{
+  compact_parents_chunk<D>(n, m, L, H, pdim, active, est, err, dim, off, foff, L2, H2, pdim2,
+                           pest, perr, paired_chunk(blockIdx.x, gridDim.x));
 }
 
 // The device-resident loop's filter + split: the descriptor says whether it
-// runs, the counts, which pair holds the parents (the children go to the
-// other) and which error array the refinement wrote.
+// runs, the counts, which pair holds the list and in which form (the next
+// parents go to the other pair), and which error array the refinement
+// wrote.
+template <int D>
 __global__ void __launch_bounds__(kCompactThreads)
-    k_compact_dev(const DevCompact* __restrict__ cp, int d, double* L0, double* H0, double* L1,
-                  double* H1, const std::uint8_t* __restrict__ active,
-                  const double* __restrict__ est, const double* err0, const double* err1,
-                  const std::uint8_t* __restrict__ dim, const std::uint64_t* __restrict__ off,
-                  const std::uint64_t* __restrict__ foff, double* __restrict__ pest,
-                  double* __restrict__ perr)
+    k_compact_dev(const DevCompact* __restrict__ cp, double* L0, double* H0, double* L1,
+                  double* H1, std::uint8_t* P0, std::uint8_t* P1,
+                  const std::uint8_t* __restrict__ active, const double* __restrict__ est,
+                  const double* err0, const double* err1, const std::uint8_t* __restrict__ dim,
+                  const std::uint64_t* __restrict__ off, const std::uint64_t* __restrict__ foff,
+                  double* __restrict__ pest, double* __restrict__ perr)
 {
   if (!cp->run) return;
   const std::size_t n = cp->n, m = cp->m;
-  if (blockIdx.x >= num_chunks(n)) return;
+  const std::size_t nch = num_chunks(n);
+  if (blockIdx.x >= nch) return;
   const bool c1 = cp->cur != 0;
-  compact_chunk(1, d, n, m, c1 ? L1 : L0, c1 ? H1 : H0, active, est,
-                cp->has_parents ? err1 : err0, dim, off, foff, c1 ? L0 : L1, c1 ? H0 : H1, pest,
-                perr, nullptr);
+  compact_parents_chunk<D>(n, m, c1 ? L1 : L0, c1 ? H1 : H0,
+                        cp->split ? (c1 ? P1 : P0) : nullptr, active, est,
+                        cp->has_parents ? err1 : err0, dim, off, foff, c1 ? L0 : L1,
+                        c1 ? H0 : H1, c1 ? P0 : P1, pest, perr,
+                        paired_chunk(blockIdx.x, nch));
 }
 
 __global__ void __launch_bounds__(256)
@@ -873,6 +970,18 @@ __global__ void __launch_bounds__(256) k_fill_u8(std::size_t n, std::uint8_t* p,
 inline dim3 grid_for(std::size_t n, unsigned threads)
 {
   return dim3(static_cast<unsigned>((n + threads - 1) / threads));
+}
+
+// fn(std::integral_constant<int, d>) for d in [PAGANI_MIN_DIM, PAGANI_MAX_DIM]
+template <int D = PAGANI_MIN_DIM, class Fn>
+hipError_t with_dim(int d, Fn&& fn)
+{
+  if constexpr (D > PAGANI_MAX_DIM) {
+    return hipErrorInvalidValue;
+  } else {
+    if (d == D) return fn(std::integral_constant<int, D>{});
+    return with_dim<D + 1>(d, fn);
+  }
 }
 
 }  // namespace
@@ -979,10 +1088,12 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
                    double* pest, double* perr, std::uint8_t* dim2)
 {
   if (n == 0 || m == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kCompactThreads), 0, s,
-                     split, d, n, m, L, H, active, est, err, dim, off, foff, L2, H2, pest, perr,
-                     dim2);
-  return hipGetLastError();
+  return with_dim(d, [&](auto D) {
+    hipLaunchKernelGGL((k_compact<decltype(D)::value>), dim3(static_cast<unsigned>(num_chunks(n))),
+                       dim3(kCompactThreads), 0, s, split, n, m, L, H, active, est, err, dim, off,
+                       foff, L2, H2, pest, perr, dim2);
+    return hipGetLastError();
+  });
 }
 
 hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, const double* est,
@@ -998,15 +1109,35 @@ hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, c
 }
 
 hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, int d,
-                       double* L0, double* H0, double* L1, double* H1,
-                       const std::uint8_t* active, const double* est, const double* err0,
-                       const double* err1, const std::uint8_t* dim, const std::uint64_t* off,
-                       const std::uint64_t* foff, double* pest, double* perr)
+                       double* L0, double* H0, double* L1, double* H1, std::uint8_t* P0,
+                       std::uint8_t* P1, const std::uint8_t* active, const double* est,
+                       const double* err0, const double* err1, const std::uint8_t* dim,
+                       const std::uint64_t* off, const std::uint64_t* foff, double* pest,
+                       double* perr)
 {
-  hipLaunchKernelGGL(k_compact_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))),
-                     dim3(kCompactThreads), 0, s, cp, d, L0, H0, L1, H1, active, est, err0, err1, dim,
-                     off, foff, pest, perr);
-  return hipGetLastError();
+  return with_dim(d, [&](auto D) {
+    hipLaunchKernelGGL((k_compact_dev<decltype(D)::value>),
+                       dim3(static_cast<unsigned>(num_chunks(n_cap))), dim3(kCompactThreads), 0, s,
+                       cp, L0, H0, L1, H1, P0, P1, active, est, err0, err1, dim, off, foff, pest,
+                       perr);
+    return hipGetLastError();
+  });
+}
+
+hipError_t compact_parents(hipStream_t s, int d, std::size_t n, std::size_t m, const double* L,
+                           const double* H, const std::uint8_t* pdim,
+                           const std::uint8_t* active, const double* est, const double* err,
+                           const std::uint8_t* dim, const std::uint64_t* off,
+                           const std::uint64_t* foff, double* L2, double* H2,
+                           std::uint8_t* pdim2, double* pest, double* perr)
+{
+  if (n == 0 || m == 0) return hipSuccess;
+  return with_dim(d, [&](auto D) {
+    hipLaunchKernelGGL((k_compact_parents<decltype(D)::value>),
+                       dim3(static_cast<unsigned>(num_chunks(n))), dim3(kCompactThreads), 0, s, n,
+                       m, L, H, pdim, active, est, err, dim, off, foff, L2, H2, pdim2, pest, perr);
+    return hipGetLastError();
+  });
 }
 
 hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,

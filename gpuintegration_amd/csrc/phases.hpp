@@ -41,11 +41,11 @@ __host__ __device__ inline std::size_t num_folds(std::size_t nchunks)
 constexpr std::size_t kDevLoopMax = kChunk * kFold;
 
 // Filter + split descriptor of the device-resident loop (device memory):
-// the parents' count n and pair, the children's count m, which error array
-// holds the refined errors.
+// the list's count n, pair and form (split: held as parents + halved axes),
+// the count m of regions kept, which error array holds the refined errors.
 struct DevCompact {
   std::uint64_t n, m;
-  std::uint32_t cur, run, has_parents, pad;
+  std::uint32_t cur, run, has_parents, split;
 };
 
 // Candidate thresholds measured by one pass of the classifier probe (the
@@ -82,10 +82,11 @@ hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, c
                     double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* sc,
                     std::uint32_t* ticket);
 hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, int d,
-                       double* L0, double* H0, double* L1, double* H1,
-                       const std::uint8_t* active, const double* est, const double* err0,
-                       const double* err1, const std::uint8_t* dim, const std::uint64_t* off,
-                       const std::uint64_t* foff, double* pest, double* perr);
+                       double* L0, double* H0, double* L1, double* H1, std::uint8_t* P0,
+                       std::uint8_t* P1, const std::uint8_t* active, const double* est,
+                       const double* err0, const double* err1, const std::uint8_t* dim,
+                       const std::uint64_t* off, const std::uint64_t* foff, double* pest,
+                       double* perr);
 
 // RefineError + relerr classification (parent_est == nullptr: iteration 0,
 // err_out may equal err_in).  Partial sums: {est, err_out, act*est, act*err_out}.
@@ -141,7 +142,23 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
                    const std::uint64_t* off, const std::uint64_t* foff, double* L2,
                    double* H2, double* pest, double* perr, std::uint8_t* dim2);
 
-// Children of all m regions (Sub_region_splitter::split without a filter).
+// The iteration loop's filter + split, Sub_region_filter.cuh:8-139 with
+// Sub_region_splitter.cuh:8-87: the m active regions of an n-region list
+// (flat, or in split form when pdim is not NULL) become the parents of the
+// next list, written once -- their bounds at stride m into L2/H2, the axis
+// each is to be halved along (its own fourth-difference choice, dim) into
+// pdim2, their estimates and errors into pest/perr -- and the next list is
+// their 2m children in split form.  The reference writes both children's
+// full bounds (32d bytes per kept region) where this writes 16d + 1.
+hipError_t compact_parents(hipStream_t s, int d, std::size_t n, std::size_t m, const double* L,
+                           const double* H, const std::uint8_t* pdim,
+                           const std::uint8_t* active, const double* est, const double* err,
+                           const std::uint8_t* dim, const std::uint64_t* off,
+                           const std::uint64_t* foff, double* L2, double* H2,
+                           std::uint8_t* pdim2, double* pest, double* perr);
+
+// Children of all m regions (Sub_region_splitter::split without a filter);
+// also writes a list held in split form out flat.
 hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,
                      const std::uint8_t* dim, double* L2, double* H2);
 

@@ -22,6 +22,12 @@ namespace internal {
 
 namespace ph = pagani::phases;
 
+// Tuned launch shapes: slices per power-of-two size class of the list
+// (bucket b holds 2^b <= n < 2^(b+1)), measured by gpuintegration_amd/
+// tune.py per integrand family, dimension and precision; 0 = no
+// measurement (choose_slices_n's rule decides).
+constexpr int kSliceBuckets = 32;
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -60,9 +66,12 @@ struct DevLoop {
   std::int32_t cum_status, status;
   double hist[3];
   std::uint64_t iters_collected;
-  // the evaluation's slice choice (choose_slices) and this round's launch:
-  // its waves per workgroup and the largest list its grids cover
+  // the evaluation's slice choice (choose_slices, with the integrand's
+  // tuned table when has_table) and this round's launch: its waves per
+  // workgroup and the largest list its grids cover
   std::int32_t num_cus, slices_override, generic, launch_slices;
+  std::int32_t has_table, pad_table;
+  std::uint8_t slice_table[kSliceBuckets];
   std::uint64_t round_cap;
   // per round
   std::uint64_t evaluated, iterations, max_n;
@@ -86,6 +95,7 @@ struct LoopState {
   std::uint64_t it = 0;  // index of the next iteration
   std::uint64_t n = 0;   // regions to evaluate next
   int has_parents = 0;
+  int split = 0;         // the list is held in split form
   int user_regions = 0;  // Sub_regions& overload bookkeeping
   int relerr = 1;
   double epsrel = 0, epsabs = 0;
@@ -119,7 +129,9 @@ struct pagani_ws {
   double* d_gtab = nullptr;
 
   pagani::internal::DevBuf<double> L[2], H[2];
+  pagani::internal::DevBuf<std::uint8_t> pdim[2];  // halved axes of a list in split form
   int cur = 0;
+  bool split = false;  // the list in L[cur], H[cur] is in split form (pagani_eval_args)
   pagani::internal::DevBuf<double> est, err0, err1, pest, perr;
   pagani::internal::DevBuf<std::uint8_t> dim, active, flags;
   pagani::internal::DevBuf<double> part, fpart;
@@ -132,8 +144,9 @@ struct pagani_ws {
   double* d_mout = nullptr;
   double* h_mout = nullptr;
   hipEvent_t ev[10] = {};
-  // device-resident loop
+  // device-resident loop, and its full-length rounds as HIP graphs
   bool device_loop = true;
+  bool graphs = true;
   pagani::internal::DevLoop* d_loop = nullptr;
   pagani::internal::DevLoop* h_loop = nullptr;
   bool compact_pending = false;
@@ -151,6 +164,11 @@ struct pagani_ws {
   std::uint64_t snapshot_iter = ~std::uint64_t(0);
   // relerr classification by the predict_split rule (Workspace.cuh:206-210)
   bool predict_split = false;
+  // launch shapes from the tuned table (pagani_ws_set_tuned_slices), for
+  // the integrand of the running call: its family (0: a callback) and
+  // precision
+  bool tuned = true;
+  int tune_family = 0, tune_precision = PAGANI_FP64;
   pagani::internal::DevBuf<double> snapL, snapH;
   std::size_t snap_n = 0;
 };
@@ -233,6 +251,7 @@ inline void for_each_pool_pair(pagani_ws* a, pagani_ws* b, F&& f)
   for (int i = 0; i < 2; ++i) {
     f(a->L[i], b->L[i]);
     f(a->H[i], b->H[i]);
+    f(a->pdim[i], b->pdim[i]);
   }
   f(a->est, b->est);
   f(a->err0, b->err0);
@@ -266,6 +285,8 @@ inline void for_each_pool(pagani_ws* ws, F&& f)
   f(ws->L[1]);
   f(ws->H[0]);
   f(ws->H[1]);
+  f(ws->pdim[0]);
+  f(ws->pdim[1]);
   f(ws->est);
   f(ws->err0);
   f(ws->err1);
@@ -319,7 +340,7 @@ enum { EV_EVAL0, EV_EVAL1, EV_REF0, EV_REF1, EV_CMP0, EV_CMP1, EV_CLS0, EV_CLS1,
 
 inline void record(pagani_ws* ws, int i)
 {
-  if (ws->timing) hipEventRecord(ws->ev[i], ws->stream);
+  if (ws->timing) (void)hipEventRecord(ws->ev[i], ws->stream);
 }
 
 inline double elapsed(pagani_ws* ws, int a, int b)
@@ -332,13 +353,27 @@ inline double elapsed(pagani_ws* ws, int a, int b)
   return ms;
 }
 
-// Waves per batch of 64 regions for an n-region evaluation: more waves
+__host__ __device__ inline int slice_bucket(std::size_t n)
+{
+  int b = 0;
+  while (b + 1 < kSliceBuckets && (n >> (b + 1)) != 0) ++b;
+  return b;
+}
+
+// Waves per batch of 64 regions for an n-region evaluation: the tuned
+// table's entry for n's size class when there is one; otherwise more waves
 // share a batch while there are too few batches to give every CU four
 // waves per SIMD and each wave would still get at least 16 generic points.
+// Both are non-increasing in n (the device loop's rounds rely on it).
 __host__ __device__ inline int choose_slices_n(std::size_t n, int num_cus, int override_,
-                                               int generic)
+                                               int generic,
+                                               const std::uint8_t* table = nullptr)
 {
   if (override_ > 0) return override_ < PAGANI_MAX_SLICES ? override_ : PAGANI_MAX_SLICES;
+  if (table) {
+    const int t = table[slice_bucket(n)];
+    if (t > 0) return t < PAGANI_MAX_SLICES ? t : PAGANI_MAX_SLICES;
+  }
   const std::size_t batches = (n + 63) / 64;
   const std::size_t target = static_cast<std::size_t>(num_cus) * 4 * 4;
   int s = 1;
@@ -350,9 +385,11 @@ __host__ __device__ inline std::size_t mem_required(std::size_t n, int d, int mo
 {
   const std::size_t nd = n * static_cast<std::size_t>(d);
   if (model == PAGANI_MEM_ACTUAL) {
-    // live bounds (16d) + both children's bounds (32d) + their est/err0/err1
-    // (48) and dim/active/flags (6) + the parents' est/err (16)
-    return 16 * nd + 32 * nd + 48 * n + 6 * n + 16 * n;
+    // lists are held in split form (parents' bounds + halved axis): the
+    // list's n/2 parents (8d per region) + the next list's n parents (16d)
+    // + the next list's est/err0/err1 (48) and dim/active/flags (6) + both
+    // lists' halved axes (1.5) + the next parents' estimates (8)
+    return 8 * nd + 16 * nd + 48 * n + 6 * n + 2 * n + 8 * n;
   }
   // num_doubles_needed / num_ints_needed / device_mem_required_for_full_split
   const std::size_t dbl = 8 * n + 10 * nd;

@@ -32,7 +32,8 @@ extern "C" {
 #endif
 
 #define PAGANI_MIN_DIM 2
-#define PAGANI_MAX_DIM 8
+#define PAGANI_MAX_DIM 16      /* workspaces and the C++ header (Workspace<double, ndim>) */
+#define PAGANI_GENZ_MAX_DIM 10 /* the precompiled Genz descriptor path */
 #define PAGANI_NSETS 9
 #define PAGANI_NRULES 5
 #define PAGANI_MAX_SLICES 8
@@ -73,13 +74,22 @@ typedef struct pagani_result {
 } pagani_result;
 
 /* Precompiled Genz integrand descriptor (POD).  The reference's fixed
- * functors F_k_dD (common/cuda/integrands.cuh:1303-1934) are instances. */
+ * functors F_k_dD (common/cuda/integrands.cuh:1303-1934) are instances.
+ * ndim 2..PAGANI_GENZ_MAX_DIM; a and u hold ndim entries. */
 typedef struct pagani_genz {
   int family;
   int ndim;
   double a[PAGANI_MAX_DIM];
   double u[PAGANI_MAX_DIM];
 } pagani_genz;
+
+/* Layout version of pagani_eval_consts / pagani_eval_args and of the region
+ * lists they describe.  The library stamps it into every launch's constants
+ * and pagani::eval_regions (compiled into the caller's code) refuses a
+ * launch whose stamp differs from the version it was compiled with, so a
+ * caller built against other headers fails with PAGANI_EUNSUPPORTED instead
+ * of reading a list laid out differently. */
+#define PAGANI_EVAL_ABI_VERSION 2
 
 /* Uniform data of one evaluation launch: the degree-9 rule (Rule.cuh),
  * the integration volume and the launch shape.  Built by the library; an
@@ -99,6 +109,8 @@ typedef struct pagani_eval_consts {
   int slices;                      /* waves sharing one batch of 64 regions */
   int slice_begin[PAGANI_MAX_SLICES + 1]; /* generic-point slice bounds     */
   double lambda3, lambda4, lambda5; /* generator magnitudes of sets 3, 4, 8 */
+  uint32_t abi_version;             /* PAGANI_EVAL_ABI_VERSION of the library */
+  uint32_t pad;
 } pagani_eval_consts;
 
 /* Iteration descriptor of the device-resident loop, in device memory: the
@@ -111,13 +123,25 @@ typedef struct pagani_dev_iter {
   uint32_t run;         /* 0: skip */
   uint32_t has_parents; /* the refinement has parents' estimates */
   uint32_t slices;      /* waves per batch of 64 regions (see consts.slices) */
+  uint32_t split;       /* the list is in split form (see pagani_eval_args) */
+  uint32_t pad;
 } pagani_dev_iter;
 
 /* One evaluation request: regions [0, n) of the SoA pool.  When dev is not
  * NULL the request belongs to the device-resident loop: n is an upper bound,
  * the count and the pair (L, H) or (L2, H2) are read from *dev on the
  * device, and the launch must not depend on them (pagani::eval_regions
- * handles both forms). */
+ * handles both forms).
+ *
+ * A list is flat or in split form.  Flat: L, H hold the n regions' own
+ * bounds, [j*n + t].  Split form (pdim not NULL; in the device-resident loop,
+ * dev->split, with pdim2 for the pair (L2, H2)): the list is the two
+ * children of each of n/2 parents, L, H hold the parents' bounds
+ * [j*(n/2) + i] and pdim[i] the axis parent i is halved along; region t is
+ * child t / (n/2) of parent t mod (n/2), the left half along that axis
+ * for child 0 and the right half for child 1 (Sub_region_splitter.cuh:8-46
+ * writes those children out; this library derives them where they are
+ * read). */
 typedef struct pagani_eval_args {
   size_t n;
   const double* L;
@@ -130,6 +154,8 @@ typedef struct pagani_eval_args {
   const pagani_dev_iter* dev;
   const double* L2;
   const double* H2;
+  const uint8_t* pdim;
+  const uint8_t* pdim2;
 } pagani_eval_args;
 
 /* An evaluation callback launches the region-evaluation kernel for one
@@ -166,6 +192,7 @@ typedef struct pagani_stats {
   uint64_t device_regions;    /* regions it evaluated (included in regions_evaluated) */
   uint64_t device_rounds;     /* its host round trips */
   uint64_t budget_fixes;      /* iterations that took fix_error_budget_overflow */
+  uint64_t graph_launches;    /* device-loop rounds replayed from a HIP graph */
 } pagani_stats;
 
 /* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
@@ -211,10 +238,11 @@ size_t pagani_ws_device_mem(const pagani_ws* ws);
 /* Bytes the classifier's memory model charges for splitting n regions:
  * PAGANI_MEM_REFERENCE (default) is device_mem_required_for_full_split
  * (heuristic_classifier.cuh:96-133, 8(10d+8)+16 bytes per region);
- * PAGANI_MEM_ACTUAL is this library's own footprint (the live bounds, the
- * children's bounds, the children's per-region arrays and the parents'
- * estimates: 48d+70 bytes per region), which lets a pool grow about 1.6x
- * larger on the same device before classification is forced. */
+ * PAGANI_MEM_ACTUAL is this library's own footprint (lists are held as
+ * parents plus the axis each is halved along: the list's parents, the next
+ * list's parents, the next list's per-region arrays and the parents'
+ * estimates: 24d+64 bytes per region), which lets a pool grow about 2.7x
+ * larger at 8D on the same device before classification is forced. */
 #define PAGANI_MEM_REFERENCE 0
 #define PAGANI_MEM_ACTUAL 1
 int pagani_ws_set_mem_model(pagani_ws* ws, int model);
@@ -225,8 +253,20 @@ int pagani_ws_set_mem_model(pagani_ws* ws, int model);
  * iteration.  Results are identical either way; the host takes over for
  * the heuristic classifier, the error-budget fix and larger lists. */
 int pagani_ws_set_device_loop(pagani_ws* ws, int enable);
+/* Full-length rounds of the device-resident loop (16 iterations, 48
+ * launches) are captured once as a HIP graph and replayed for later rounds
+ * of the same shape (enable = 1, the default); results are identical
+ * either way.  No reference counterpart. */
+int pagani_ws_set_graphs(pagani_ws* ws, int enable);
 /* waves per batch of 64 regions in the evaluation kernel; 0 = automatic */
 int pagani_ws_set_slices(pagani_ws* ws, int slices);
+/* No reference counterpart (the reference fixes n x 64 threads,
+ * PaganiUtils.cuh:313-331).  Automatic launch shapes come from the table
+ * the autotuner (gpuintegration_amd/tune.py) measured for the integrand's
+ * family, dimension and precision, per size class of the list, where it
+ * has an entry (enable = 1, the default), else from a fixed rule (enable =
+ * 0, or an integrand without measurements, such as a callback). */
+int pagani_ws_set_tuned_slices(pagani_ws* ws, int enable);
 int pagani_ws_set_timing(pagani_ws* ws, int enable);
 int pagani_ws_get_stats(const pagani_ws* ws, pagani_stats* out);
 int pagani_ws_reset_stats(pagani_ws* ws);
@@ -243,10 +283,10 @@ size_t pagani_ws_trace(const pagani_ws* ws, size_t cap, uint64_t* n, uint64_t* m
                        double* estimate, double* errorest);
 /* The region list the last integrate call ended with (the reference leaves
  * it in the caller's Sub_regions, Sub_region_filter.cuh:129-135,
- * Sub_region_splitter.cuh:81-84): *n regions, device SoA pointers owned by
- * the workspace and valid until its next call. */
-int pagani_ws_live_regions(const pagani_ws* ws, size_t* n, const double** L,
-                           const double** H);
+ * Sub_region_splitter.cuh:81-84): *n regions, flat device SoA pointers
+ * owned by the workspace and valid until its next call (a list held in
+ * split form is written out flat first, on the workspace's stream). */
+int pagani_ws_live_regions(pagani_ws* ws, size_t* n, const double** L, const double** H);
 
 /* ---- integration ------------------------------------------------------ */
 /* Workspace::integrate(integrand, epsrel, epsabs, vol, relerr_classification)
@@ -344,6 +384,8 @@ const char* pagani_batch_last_error(const pagani_batch* b);
 int pagani_batch_set_device_mem(pagani_batch* b, size_t total_bytes);
 int pagani_batch_set_mem_model(pagani_batch* b, int model);
 int pagani_batch_set_timing(pagani_batch* b, int enable);
+/* pagani_ws_set_tuned_slices for every workspace of the batch */
+int pagani_batch_set_tuned_slices(pagani_batch* b, int enable);
 /* statistics summed over all workspaces of the batch */
 int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out);
 int pagani_batch_reset_stats(pagani_batch* b);
@@ -531,7 +573,8 @@ int pagani_mc_integrate_cb(pagani_mc* mc, pagani_mc_sample_fn sample, void* ctx,
                            double epsrel, double epsabs, const double* lows,
                            const double* highs, const pagani_mc_params* params,
                            pagani_mc_result* out);
-/* The same for a precompiled Genz integrand (ndim 2..8, FP64). */
+/* The same for a precompiled Genz integrand (ndim 2..8, FP64; m-Cubes'
+ * precompiled samplers stop at 8 dimensions). */
 int pagani_mc_integrate_genz(pagani_mc* mc, const pagani_genz* f, double epsrel,
                              double epsabs, const double* lows, const double* highs,
                              const pagani_mc_params* params, pagani_mc_result* out);

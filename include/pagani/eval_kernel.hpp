@@ -603,27 +603,69 @@ __device__ __forceinline__ bool cosine_sums(F& f, const double (&cR)[D], const d
   return true;
 }
 
+// Region t's left coordinate and length along axis j.  Flat list (pdim
+// NULL): row t of n.  Split form: row t mod n/2 of the n/2 parents, halved
+// along the parent's axis b = pdim[row] -- lo, h / 2 for child 0 and
+// lo + h / 2, h / 2 for child 1, the operations Sub_region_splitter.cuh:
+// 8-46 performs, so the bounds are those a written-out split would hold.
+struct RegionRow {
+  std::size_t row, stride;
+  int axis;  // the halved axis, -1 for a flat list
+  bool right;
+  __device__ __forceinline__ RegionRow(const std::uint8_t* __restrict__ pdim, std::size_t n,
+                                       std::size_t t)
+  {
+    if (pdim) {
+      const std::size_t m = n / 2;
+      right = t >= m;
+      row = right ? t - m : t;
+      stride = m;
+      axis = pdim[row];
+    } else {
+      right = false;
+      row = t;
+      stride = n;
+      axis = -1;
+    }
+  }
+  __device__ __forceinline__ void bounds(const double* __restrict__ L,
+                                         const double* __restrict__ H, int j, double& lo,
+                                         double& len) const
+  {
+    lo = L[static_cast<std::size_t>(j) * stride + row];
+    len = H[static_cast<std::size_t>(j) * stride + row];
+    if (j == axis) {
+      const double half = len / 2;
+      if (right) lo = lo + half;
+      len = half;
+    }
+  }
+};
+
 // One wave's share of the evaluation of up to 64 regions, one lane per
-// region t of a SoA pool with row stride `stride`.  `wave` indexes the
-// `slices` waves that split the batch's points (0 when slices == 1); with
-// more than one, every wave of the workgroup passes one barrier.
-// `lows`, `range`, `jac` describe the integration box.
+// region t of an n-region SoA list (split form when pdim is not NULL).
+// `wave` indexes the `slices` waves that split the batch's points (0 when
+// slices == 1); with more than one, every wave of the workgroup passes one
+// barrier.  `lows`, `range`, `jac` describe the integration box.
 template <int D, class F, class Real>
 __device__ __forceinline__ void eval_regions_wave(
-    F& f, const double* __restrict__ L, const double* __restrict__ H, std::size_t stride,
-    std::size_t t, bool live, const double* lows, const double* range, double jac,
-    const pagani_eval_consts& C, int slices, int wave, int lane, double* part,
-    double* __restrict__ est, double* __restrict__ err, std::uint8_t* __restrict__ dim)
+    F& f, const double* __restrict__ L, const double* __restrict__ H,
+    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t, bool live,
+    const double* lows, const double* range, double jac, const pagani_eval_consts& C, int slices,
+    int wave, int lane, double* part, double* __restrict__ est, double* __restrict__ err,
+    std::uint8_t* __restrict__ dim)
 {
   // Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
   // width in user coordinates.
   double cR[D], hR[D];
   double vol = 1.0, maxr = 0.0;
   int maxdim = 0;
+  const RegionRow rr(pdim, n, t);
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    const double lo = L[static_cast<std::size_t>(j) * stride + t];
-    const double up = lo + H[static_cast<std::size_t>(j) * stride + t];
+    double lo, len;
+    rr.bounds(L, H, j, lo, len);
+    const double up = lo + len;
     const double r = up - lo;
     vol *= r;
     if (r > maxr) {
@@ -758,7 +800,8 @@ __device__ __forceinline__ void eval_regions_wave(
 }  // namespace detail
 
 // Kernel: workgroup b evaluates regions [64 b, 64 b + 64) of an n-region
-// SoA pool with S = slices waves sharing the batch's points.  For a
+// SoA list (flat, or in split form when P is given) with S = slices waves
+// sharing the batch's points.  For a
 // host-driven launch `it` is NULL and n, the pool (L0, H0) and S = C.slices
 // come from the arguments.  In the device-resident loop they come from *it
 // (nothing at all if !it->run), and the launch is sized for the round's
@@ -771,6 +814,7 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     eval_kernel(F f, const pagani_dev_iter* __restrict__ it, std::size_t n_arg,
                 const double* __restrict__ L0, const double* __restrict__ H0,
                 const double* __restrict__ L1, const double* __restrict__ H1,
+                const std::uint8_t* __restrict__ P0, const std::uint8_t* __restrict__ P1,
                 double* __restrict__ est, double* __restrict__ err,
                 std::uint8_t* __restrict__ dim, const pagani_eval_consts C)
 {
@@ -778,6 +822,7 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
   std::size_t n = n_arg;
   const double* L = L0;
   const double* H = H0;
+  const std::uint8_t* P = P0;
   int slices = C.slices;
   if (it) {
     if (!it->run) return;
@@ -785,7 +830,9 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     if (it->cur) {
       L = L1;
       H = H1;
+      P = P1;
     }
+    if (!it->split) P = nullptr;
     slices = static_cast<int>(it->slices);
   }
   if (static_cast<std::size_t>(blockIdx.x) * 64 >= n) return;
@@ -800,8 +847,8 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
   const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
   const bool live = t0 < n;
   const std::size_t t = live ? t0 : n - 1;
-  detail::eval_regions_wave<D, F, Real>(f, L, H, n, t, live, C.lows, C.range, C.jac, C, slices,
-                                        wave, lane, part, est, err, dim);
+  detail::eval_regions_wave<D, F, Real>(f, L, H, P, n, t, live, C.lows, C.range, C.jac, C,
+                                        slices, wave, lane, part, est, err, dim);
 }
 
 // Launch the evaluation of args->n regions for integrand f.  Returns 0 or
@@ -810,8 +857,11 @@ template <int D, class F, class Real = double>
 int eval_regions(const F& f, const pagani_eval_args* args)
 {
   static_assert(D >= PAGANI_MIN_DIM && D <= PAGANI_MAX_DIM, "unsupported ndim");
+  // a library built from other headers lays its lists out differently
+  if (args->consts.abi_version != PAGANI_EVAL_ABI_VERSION) return PAGANI_EUNSUPPORTED;
   if (args->consts.ndim != D) return PAGANI_EBADARG;
   if (args->n == 0) return PAGANI_OK;
+  if (!args->dev && args->pdim && (args->n % 2)) return PAGANI_EBADARG;  // pairs of children
   // waves per workgroup: the slice count (an upper bound of the device
   // loop's counts for its launches)
   const int waves = args->consts.slices;
@@ -822,7 +872,8 @@ int eval_regions(const F& f, const pagani_eval_args* args)
   hipLaunchKernelGGL((eval_kernel<D, F, Real>), dim3(static_cast<unsigned>(blocks)),
                      dim3(64 * waves), lds, static_cast<hipStream_t>(args->stream), f, args->dev,
                      args->n, args->L, args->H, args->dev ? args->L2 : args->L,
-                     args->dev ? args->H2 : args->H, args->est, args->err, args->dim,
+                     args->dev ? args->H2 : args->H, args->pdim,
+                     args->dev ? args->pdim2 : args->pdim, args->est, args->err, args->dim,
                      args->consts);
   return hipGetLastError() == hipSuccess ? PAGANI_OK : PAGANI_EHIP;
 }

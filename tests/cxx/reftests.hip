@@ -243,6 +243,17 @@ struct C0Three {
   }
 };
 
+// cos(s + t + ... + b): the reference's 10D oscillatory demo integrand
+// (integrands.cuh:1144-1167, Oscillatory_10D; driven by
+// cuda/pagani/demos/oscillatory_integrands.cu over [pi, 1.1 pi]^10)
+struct Oscillatory10 {
+  __device__ double operator()(double s, double t, double u, double v, double w, double x,
+                               double y, double z, double c, double b) const
+  {
+    return cos(s + t + u + v + w + x + y + z + c + b);
+  }
+};
+
 // A whole integration through the header with its per-iteration trace;
 // predict_split selects integrate<F, true>.
 template <int D, class F>
@@ -394,6 +405,8 @@ extern "C" int reftest_sample_plain_genz(int fam, int d, const double* a, const 
       case 6: return sample_plain_family<6>(fam, a, u, n, L, H, slices, est, err, dim);
       case 7: return sample_plain_family<7>(fam, a, u, n, L, H, slices, est, err, dim);
       case 8: return sample_plain_family<8>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 9: return sample_plain_family<9>(fam, a, u, n, L, H, slices, est, err, dim);
+      case 10: return sample_plain_family<10>(fam, a, u, n, L, H, slices, est, err, dim);
       default: return -1;
     }
   } catch (const std::exception&) {
@@ -402,6 +415,35 @@ extern "C" int reftest_sample_plain_genz(int fam, int d, const double* a, const 
 }
 
 extern "C" void reftest_set_device_loop(int on) { g_device_loop = on ? 1 : 0; }
+
+// The demo's call: Sub_regions with 2 partitions per axis (get_partitions_
+// per_axis, new_time_and_call.cuh:116-126), epsabs 1e-40, the box
+// [low, high]^10, with the per-iteration trace.
+extern "C" int reftest_oscillatory10(double low, double high, double epsrel, pagani_result* out,
+                                     std::uint64_t* tn, std::uint64_t* tm, std::size_t cap,
+                                     std::size_t* ntrace)
+{
+  if (!out || !tn || !tm || !ntrace) return -1;
+  try {
+    Workspace<double, 10> workspace;
+    pagani_ws_set_device_mem(workspace.handle(), static_cast<std::size_t>(16e9));
+    quad::Volume<double, 10> vol(low, high);
+    Sub_regions<double, 10> sub_regions(2);
+    const numint::integration_result r =
+        workspace.template integrate<Oscillatory10, false, false>(Oscillatory10{}, sub_regions,
+                                                                   epsrel, 1.0e-40, vol, true);
+    std::memset(out, 0, sizeof(*out));
+    out->estimate = r.estimate;
+    out->errorest = r.errorest;
+    out->nregions = r.nregions;
+    out->status = r.status;
+    out->iters = r.iters;
+    *ntrace = pagani_ws_trace(workspace.handle(), cap, tn, tm, nullptr, nullptr);
+    return 0;
+  } catch (const std::exception&) {
+    return -2;
+  }
+}
 
 extern "C" int reftest_integrate_traced(const char* name, double epsrel, double epsabs,
                                         std::size_t mem_total, int predict_split, int relerr,

@@ -52,7 +52,7 @@ def test_cos_sum_8d_full_pool(pg, gpu):
           f"nregions {r.nregions}, estimate {r.estimate!r}, errorest {r.errorest:.3g}")
     assert max(n) >= 300_000_000
     assert r.status == 1
-    per_region = 48 * d + 70  # the library's footprint model (PAGANI_MEM_ACTUAL)
+    per_region = 24 * d + 64  # the library's footprint model (PAGANI_MEM_ACTUAL)
     assert per_region * n[-1] > total / 2
     assert abs(r.estimate - COS8_EXACT) <= r.errorest
     assert abs(r.estimate - COS8_EXACT) <= 1e-9 * COS8_EXACT

@@ -43,7 +43,7 @@ def test_test_helper_library_loads():
         assert hasattr(lib, n)
 
 
-@pytest.mark.parametrize("d", range(2, 9))
+@pytest.mark.parametrize("d", range(2, 17))
 def test_rule_tables_bit_identical_to_oracle(pg, d):
     w, sc, nm, lam, sb, g = pg.rule_tables(d)
     ow, olam, osc, onm, cnt = O.rule_tables(d)
@@ -56,7 +56,7 @@ def test_rule_tables_bit_identical_to_oracle(pg, d):
 
 def test_rule_tables_reject_bad_dims(pg):
     lib = pg.load()
-    for d in (0, 1, 9, 17):
+    for d in (0, 1, 17, 18):
         assert lib.pagani_rule_tables(d, None, None, None, None, None, None) < 0
 
 
