@@ -83,8 +83,20 @@ def _sync():
 
 
 def noneval_bytes(trace, d):
-    """Algorithmic HBM bytes of refine + compaction + split over a run:
-    n (16d + 33) + m (32d + 16) per iteration."""
+    """Algorithmic HBM bytes of this library's refinement and filter + split
+    over a run, per iteration of n regions keeping m: the refinement reads
+    each region's estimate and error (16), its share of the parents'
+    estimates (4), writes the refined error (8) and the flag (1); the
+    compaction reads every flag (1) and, per kept region, its parent's row
+    of bounds (16d), the parent's halved axis, its own axis (1 + 1) and
+    estimate (8), and writes it as a parent of the next list (16d bounds +
+    1 axis + 8 estimate): 30 n + (32d + 19) m."""
+    return sum(int(n) * 30 + int(m) * (32 * d + 19) for n, m in zip(trace["n"], trace["m"]))
+
+
+def reference_noneval_bytes(trace, d):
+    """The bytes the reference's phases move for the same trajectory
+    (n (16d + 33) + m (32d + 16): both children written out in full)."""
     return sum(int(n) * (16 * d + 33) + int(m) * (32 * d + 16)
                for n, m in zip(trace["n"], trace["m"]))
 
@@ -121,6 +133,7 @@ def phase_report(W, fam, d, res, dt, st, tr, steps):
     # loop's iterations are timed as whole rounds (device_ms)
     flops = W.eval_flops_per_region(fam, d) * (regions - st["device_regions"])
     ne_bytes = noneval_bytes(tr, d) * steps
+    ref_bytes = reference_noneval_bytes(tr, d) * steps
     ne_s = (st["refine_ms"] + st["compact_ms"]) / 1e3
     return {
         "regions_per_s": regions / dt,
@@ -136,7 +149,10 @@ def phase_report(W, fam, d, res, dt, st, tr, steps):
         "eval_tflops": flops / eval_s / 1e12 if eval_s > 0 else None,
         "eval_frac_fp64_peak": flops / eval_s / 1e12 / FP64_PEAK_TFLOPS if eval_s > 0 else None,
         "noneval_gbs": ne_bytes / ne_s / 1e9 if ne_s > 0 else None,
-        "noneval_frac_hbm": ne_bytes / ne_s / 1e9 / HBM_PEAK_GBS if ne_s > 0 else None,
+        "noneval_frac_hbm": check_fraction("non-eval HBM fraction",
+                                           ne_bytes / ne_s / 1e9 / HBM_PEAK_GBS)
+        if ne_s > 0 else None,
+        "noneval_reference_bytes_per_s": ref_bytes / ne_s if ne_s > 0 else None,
         "phase_ms_per_integral": {k: st[k] / steps for k in
                                   ("eval_ms", "refine_ms", "compact_ms", "classify_ms")},
         "classify_passes_per_integral": st["classify_passes"] / steps,
@@ -229,7 +245,21 @@ def generic_leg(args, W, counters, steps=2, warmup=1):
     return out
 
 
+def calibration_copy(mib):
+    """One device-to-device copy of `mib` MiB: the reference point that scales
+    the FETCH_SIZE / WRITE_SIZE counters of a run to bytes."""
+    import torch
+    nbytes = mib << 20
+    src = torch.zeros(nbytes, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
 def config2(args, pg, W):
+    if args.calibrate_copy:
+        calibration_copy(args.copy_mib)
     fam, a, u = W.gaussian_8d(args.seed)
     d = len(a)
     exact = W.gaussian_exact(a, u)
@@ -329,6 +359,13 @@ def config2(args, pg, W):
     if not args.no_mcubes:
         out["mcubes"] = mcubes_leg(args, pg, W)
     out["batch"] = config4(args, pg, W, leg=True)
+    # the same batch with the fixed launch-shape rule instead of the
+    # autotuner's table
+    rule = config4(args, pg, W, leg=True, tuned=False)
+    out["launch_shape"] = {"batch_integrals_per_s_tuned": out["batch"]["value"],
+                           "batch_integrals_per_s_rule": rule["value"],
+                           "batch_eval_ms_tuned": out["batch"]["phase_ms_per_step"]["eval_ms"],
+                           "batch_eval_ms_rule": rule["phase_ms_per_step"]["eval_ms"]}
     out["large"] = config5(args, pg, W)
     out["plain_cpu_vs_gpu"] = config1(args, pg, W)
     if not args.no_cpu_baseline:
@@ -424,7 +461,7 @@ def config5(args, pg, W):
             "runs": out}
 
 
-def config4(args, pg, W, leg=False):
+def config4(args, pg, W, leg=False, tuned=True):
     """512 independent integrals (family, d in {3..8}, epsrel log-uniform in
     [1e-6, 1e-3]) from one pagani_integrate_batch call.  As a leg of the
     headline line: one timed call after a 16-item warm-up."""
@@ -434,6 +471,7 @@ def config4(args, pg, W, leg=False):
     warmup = 1 if leg else args.warmup
     b = pg.Batch(0, args.workers)
     b.set_timing(True)
+    b.set_tuned_slices(tuned)
     for _ in range(warmup):
         b.integrate(items[:16])
     _sync()
@@ -583,6 +621,8 @@ def main():
     ap.add_argument("--eval-k", type=int, default=6)
     ap.add_argument("--eval-launches", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="start the headline with a device copy of --copy-mib MiB (counter runs)")
     ap.add_argument("--workers", type=int, default=1)
     ap.add_argument("--epsrel", type=float, default=1e-6)
     ap.add_argument("--mixed-epsrel", type=float, default=1e-5)

@@ -122,6 +122,7 @@ class Stats(ctypes.Structure):
         ("device_regions", ctypes.c_uint64),
         ("device_rounds", ctypes.c_uint64),
         ("budget_fixes", ctypes.c_uint64),
+        ("graph_launches", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -186,6 +187,8 @@ _SIGS = {
     "pagani_ws_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_device_loop": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_tuned_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "pagani_ws_reset_stats": (ctypes.c_int, [_vp]),
@@ -240,6 +243,7 @@ _SIGS = {
     "pagani_batch_set_device_mem": (ctypes.c_int, [_vp, _sz]),
     "pagani_batch_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_batch_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_batch_set_tuned_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_batch_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "pagani_batch_reset_stats": (ctypes.c_int, [_vp]),
     "pagani_integrate_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_BatchItem), _sz,
@@ -423,6 +427,15 @@ class Workspace:
 
     def set_slices(self, s):
         self._check(self.lib.pagani_ws_set_slices(self._ws, int(s)))
+
+    def set_graphs(self, on=True):
+        """Replay full-length device-loop rounds from HIP graphs (default on)."""
+        self._check(self.lib.pagani_ws_set_graphs(self._ws, 1 if on else 0))
+
+    def set_tuned_slices(self, on=True):
+        """Launch shapes from the autotuner's table where it has an entry
+        (default on); off: the fixed rule."""
+        self._check(self.lib.pagani_ws_set_tuned_slices(self._ws, 1 if on else 0))
 
     def set_timing(self, on=True):
         self._check(self.lib.pagani_ws_set_timing(self._ws, 1 if on else 0))
@@ -645,6 +658,9 @@ class Batch:
 
     def set_timing(self, on=True):
         self._check(self.lib.pagani_batch_set_timing(self._b, 1 if on else 0))
+
+    def set_tuned_slices(self, on=True):
+        self._check(self.lib.pagani_batch_set_tuned_slices(self._b, 1 if on else 0))
 
     def stats(self):
         s = Stats()

@@ -82,6 +82,10 @@ int run_item(pagani_batch* b, int w, const pagani_batch_item& it, pagani_result*
   pagani_ws_set_mem_model(ws, b->mem_model);
   pagani_ws_set_timing(ws, b->timing ? 1 : 0);
   pagani_ws_set_tuned_slices(ws, b->tuned ? 1 : 0);
+  // a graph capture on one worker's stream is broken by another worker's
+  // device-wide synchronisation (a buffer freed while it grows): graphs
+  // only when one worker runs
+  pagani_ws_set_graphs(ws, b->workers == 1 ? 1 : 0);
   const int rc = pagani_integrate_genz(ws, &it.f, it.precision, it.epsrel, it.epsabs, nullptr,
                                        nullptr, it.relerr_classification, it.init_k, out);
   if (rc < 0 && rc != PAGANI_ENOMEM) set_error(b, pagani_ws_last_error(ws));

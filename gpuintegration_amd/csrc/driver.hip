@@ -352,6 +352,34 @@ int grow_keep(pagani_ws* ws, DevBuf<T>& b, std::size_t need, std::size_t keep, c
   return PAGANI_OK;
 }
 
+// The largest list size up to cap whose launches keep n's slice count:
+// with a tuned table, the end of the run of size classes from n's own that
+// share its entry (the table need not be monotone); with the rule, which is
+// non-increasing in n, a binary search.
+std::uint64_t round_cap(const pagani_ws* ws, std::uint64_t n, std::uint64_t cap, int slices)
+{
+  if (ws->slices_override <= 0)
+    if (const std::uint8_t* t = tuned_table(ws)) {
+      int b = slice_bucket(n);
+      if (t[b] > 0) {
+        while (b + 1 < kSliceBuckets && t[b + 1] == t[b] &&
+               (std::uint64_t(1) << (b + 1)) <= cap)
+          ++b;
+        const std::uint64_t end = b + 1 < 64 ? (std::uint64_t(2) << b) - 1 : cap;
+        return std::max<std::uint64_t>(n, std::min(end, cap));
+      }
+    }
+  std::uint64_t lo = n, hi = cap;
+  while (lo < hi) {
+    const std::uint64_t mid = lo + (hi - lo + 1) / 2;
+    if (choose_slices(ws, mid) == slices)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
 // Iterations queued per round at most: a round that stops early leaves the
 // rest as no-op launches, so the length trades round trips for those.
 constexpr int kDevQueueMax = 16;
@@ -483,17 +511,7 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
     // count.
     h.it.run = 1;
     h.launch_slices = static_cast<std::int32_t>(h.it.slices);
-    {
-      std::uint64_t lo = h.it.n, hi = cap;
-      while (lo < hi) {  // largest n <= cap with the same slice count
-        const std::uint64_t mid = lo + (hi - lo + 1) / 2;
-        if (choose_slices(ws, mid) == h.launch_slices)
-          lo = mid;
-        else
-          hi = mid - 1;
-      }
-      h.round_cap = lo;
-    }
+    h.round_cap = round_cap(ws, h.it.n, cap, h.launch_slices);
     set_slices(args.consts, h.launch_slices);
     args.n = h.round_cap;
     const std::size_t rcap = h.round_cap;

@@ -364,7 +364,8 @@ __host__ __device__ inline int slice_bucket(std::size_t n)
 // table's entry for n's size class when there is one; otherwise more waves
 // share a batch while there are too few batches to give every CU four
 // waves per SIMD and each wave would still get at least 16 generic points.
-// Both are non-increasing in n (the device loop's rounds rely on it).
+// The rule is non-increasing in n; a table need not be (round_cap in
+// driver.hip finds how far a launch shape holds either way).
 __host__ __device__ inline int choose_slices_n(std::size_t n, int num_cus, int override_,
                                                int generic,
                                                const std::uint8_t* table = nullptr)

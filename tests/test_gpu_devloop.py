@@ -123,3 +123,51 @@ def test_device_loop_user_functors(pg, gpu):
             assert got[0] == got[1], name
     finally:
         lib.reftest_set_device_loop(1)
+
+
+def test_device_loop_round_graphs(pg, gpu):
+    """Full-length rounds replayed from HIP graphs give the launches' own
+    results bit for bit: a run with many small iterations (2D discontinuous
+    at 1e-14: about 50 iterations of at most 850 regions, two full rounds
+    of 16), with graphs, without, and on the host loop."""
+    fam, a, u, eps = "discontinuous", [5.0, 6.0], [0.4, 0.6], 1e-14
+    runs = {}
+    for name, device, graphs in (("graphs", True, True), ("launches", True, False),
+                                 ("host", False, False)):
+        with pg.Workspace(2, 0) as ws:
+            ws.set_device_loop(device)
+            ws.set_graphs(graphs)
+            r = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-300)
+            runs[name] = (r, ws.trace(), ws.stats())
+    key = lambda r: (r.estimate, r.errorest, r.nregions, r.iters, r.status)
+    assert key(runs["graphs"][0]) == key(runs["launches"][0]) == key(runs["host"][0])
+    for k in ("n", "m", "estimate", "errorest"):
+        assert list(runs["graphs"][1][k]) == list(runs["launches"][1][k]) == \
+            list(runs["host"][1][k]), k
+    assert runs["graphs"][2]["graph_launches"] >= 2
+    assert runs["launches"][2]["graph_launches"] == 0
+
+
+def test_tuned_launch_shapes(pg, gpu):
+    """The autotuner's table picks launch shapes only.  With it on and off
+    the host loop and the device loop still agree bit for bit; between the
+    table and the fixed rule a region's sums may round differently (another
+    slice count splits a batch's points differently among waves, and one
+    wave per batch takes the factored sums), so the two runs agree within
+    their error estimates."""
+    for fam, d, eps in (("gaussian", 5, 1e-6), ("corner_peak", 6, 1e-6)):
+        a, u = _params(fam, d, seed=d + 5)
+        out = {}
+        for tuned in (True, False):
+            for device in (True, False):
+                with pg.Workspace(d, 0) as ws:
+                    ws.set_tuned_slices(tuned)
+                    ws.set_device_loop(device)
+                    r = ws.integrate_genz(fam, a, u, epsrel=eps, epsabs=1e-20)
+                    out[tuned, device] = (r.estimate, r.errorest, r.nregions, r.iters,
+                                          r.status)
+        for tuned in (True, False):
+            assert out[tuned, True] == out[tuned, False], (fam, tuned)
+        t, f = out[True, True], out[False, True]
+        assert t[4] == f[4] == 0
+        assert abs(t[0] - f[0]) <= t[1] + f[1]
