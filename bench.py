@@ -327,10 +327,15 @@ def config2(args, pg, W):
             "eval_launches": host_launches,
             "avg_launch_ms": st["eval_ms"] / max(host_launches, 1),
             "avg_regions_per_launch": host_regions / max(host_launches, 1),
-            "traffic": fc["hbm_bytes_per_region"] * host_regions / max(host_launches, 1),
-            "traffic_unit": "bytes per launch (counter-measured per-region HBM bytes x "
-                            "average regions per launch)",
-            "algorithmic_bytes_per_launch": (16 * d + 17) * host_regions / max(host_launches, 1),
+            # the host-driven launches evaluate split-form lists: a parent's
+            # row of bounds (16d) serves its two children, plus the halved
+            # axis (1 per parent), and each child writes estimate, error and
+            # axis (17)
+            "traffic": fc["split_hbm_bytes_per_region"] * host_regions / max(host_launches, 1),
+            "traffic_unit": "bytes per launch (counter-measured per-region HBM bytes of the "
+                            "headline's split-form launches x average regions per launch)",
+            "algorithmic_bytes_per_launch": (8 * d + 17.5) * host_regions / max(host_launches, 1),
+            "traffic_vs_algorithmic": fc["split_hbm_bytes_per_region"] / (8 * d + 17.5),
         },
     }
     if not args.no_generic:
@@ -593,13 +598,31 @@ def eval_only(args, pg, W):
     torch.cuda.synchronize()
     dst.copy_(src)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.eval_launches):
-        ws.eval_genz(fam, a, u, n, L, H, est, err, dim)
-    dt = time.perf_counter() - t0
+    if args.plain:
+        # the plain functor's kernel, compiled in libpagani_header_bench.so
+        import ctypes
+        lib = ctypes.CDLL(os.path.join(ROOT, "gpuintegration_amd", "libpagani_header_bench.so"))
+        lib.pagani_header_bench_eval_gaussian8.argtypes = [ctypes.c_void_p] * 3 + [
+            ctypes.c_size_t] + [ctypes.c_void_p] * 5 + [ctypes.c_int]
+        A = (ctypes.c_double * d)(*a)
+        U = (ctypes.c_double * d)(*u)
+        ptrs = [t.data_ptr() for t in (L, H, est, err, dim)]
+        kernel = "pagani::eval_kernel<8, PlainGaussian8, double>"
+        t0 = time.perf_counter()
+        rc = lib.pagani_header_bench_eval_gaussian8(ws._ws, A, U, n, *ptrs, args.eval_launches)
+        dt = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"pagani_header_bench_eval_gaussian8 returned {rc}")
+    else:
+        kernel = "pagani::eval_kernel<8, Gaussian<double,8>, double>"
+        t0 = time.perf_counter()
+        for _ in range(args.eval_launches):
+            ws.eval_genz(fam, a, u, n, L, H, est, err, dim)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
     ws.close()
     return {
-        "mode": "eval-only", "kernel": "pagani::eval_kernel<8, Gaussian<double,8>, double>",
+        "mode": "eval-only", "kernel": kernel,
         "regions_per_launch": n, "launches": args.eval_launches,
         "algorithmic_bytes_per_launch": n * (16 * d + 17),
         "flops_per_launch": n * W.eval_flops_per_region(fam, d),
@@ -618,6 +641,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--eval-only", action="store_true",
                     help="time the evaluation kernel alone (for counter runs)")
+    ap.add_argument("--plain", action="store_true",
+                    help="with --eval-only: the plain functor's kernel (operator() only, "
+                         "compiled through the C++ header) instead of the Genz descriptor's")
     ap.add_argument("--eval-k", type=int, default=6)
     ap.add_argument("--eval-launches", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=1024)

@@ -235,6 +235,9 @@ _SIGS = {
     "pagani_ws_write_regions_csv": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "pagani_ws_set_snapshot_iter": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "pagani_ws_set_predict_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_debug": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
+                                           ctypes.c_int]),
+    "pagani_rule_point": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "pagani_ws_snapshot_regions": (ctypes.c_int, [_vp, ctypes.POINTER(_sz), ctypes.POINTER(_vp),
                                                   ctypes.POINTER(_vp)]),
     "pagani_batch_create": (_vp, [ctypes.c_int, ctypes.c_int]),
@@ -319,6 +322,16 @@ def rule_tables(ndim):
                            g.ctypes.data_as(_dp))
     return (w.reshape(9, 5), sc.reshape(9, 5), nm.reshape(9, 5), lam, list(sb),
             g.reshape(-1, ndim))
+
+
+def rule_point(ndim, p, axis):
+    """Generator coordinate `axis` of rule point p in the library's point
+    order (the point of a region with centre c and width w is c - g w)."""
+    g = ctypes.c_double(0.0)
+    rc = load().pagani_rule_point(int(ndim), int(p), int(axis), ctypes.byref(g))
+    if rc != 0:
+        raise PaganiError(rc, "pagani_rule_point")
+    return g.value
 
 
 @dataclass
@@ -527,6 +540,17 @@ class Workspace:
         (off before iteration 15 while nothing finished and the list is at
         most 15M regions, on after), instead of the relerr argument."""
         self._check(self.lib.pagani_ws_set_predict_split(self._ws, 1 if on else 0))
+
+    def set_debug(self, level, directory=".", id="default", append=False):
+        """The reference's debug recorders for the following integrate calls:
+        level 1 writes the per-iteration CSV (and, with an id, the per-phase
+        time breakdown), 2 also every region's estimate and error, 3 also the
+        generator table and, for lists under 1024 regions, every rule point.
+        id=None gives the plain integrate overload's files (cuda_iters.csv);
+        append=True appends without header lines (collect_mult_runs)."""
+        self._check(self.lib.pagani_ws_set_debug(
+            self._ws, int(level), os.fsencode(directory),
+            None if id is None else id.encode(), 1 if append else 0))
 
     def snapshot_size(self):
         n = _sz(0)

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -697,6 +698,15 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
   std::uint64_t iters_this_call = 0;
   double* errs = ws->err0.p;
 
+  // the reference's debug recorders (pagani_ws_set_debug); they hold files
+  // open for this call only
+  DebugRecorder rec;
+  if (ws->dbg.level > 0) {
+    std::string why;
+    if (!rec.open(ws->dbg, ws->stream, d, epsrel, ws->rule, lows, highs, &why))
+      return fail(ws, PAGANI_EBADARG, why);
+  }
+
   std::uint64_t it = st.it;
   for (; it < 700 && n > 0; it++) {
     if (ws->iter_limit && iters_this_call == ws->iter_limit) {
@@ -723,7 +733,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     // it needs the host (or finishes).
     bool reduced = false;  // this iteration's sums are already in d_scal
     if (ws->device_loop && !ws->iter_limit && ws->snapshot_iter == ~std::uint64_t(0) &&
-        relerr != kRelerrPredictSplit &&
+        relerr != kRelerrPredictSplit && rec.level() == 0 &&
         n <= ph::kDevLoopMax && mem_ratio(n, d, total, mem_model) <= .1) {
       LoopVars v{it, n, has_parents, &cum, &clf};
       DevOutcome outcome;
@@ -758,10 +768,23 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       args.dim = ws->dim.p;
       args.stream = ws->stream;
       make_consts(ws, lows, highs, n, args.consts);
+      if (rec.points(n)) {
+        if ((rc = ensure(ws, ws->dfe, n * static_cast<std::size_t>(ws->rule.feval),
+                         "point values")))
+          return rc;
+        args.fevals = ws->dfe.p;
+      }
+      rec.mark();
       record(ws, EV_EVAL0);
       rc = eval(ctx, &args);
       if (rc) return fail(ws, rc < 0 ? rc : PAGANI_EHIP, "evaluation callback failed");
       record(ws, EV_EVAL1);
+      if (rec.level() > 0) {
+        rec.phase(it, "apply_cubature_rules");
+        HIP_TRY(ws, rec.regions(it, n, ws->est.p, ws->err0.p));
+        HIP_TRY(ws, rec.points(n, args.L, args.H, args.pdim, ws->est.p, ws->err0.p, args.fevals));
+        rec.mark();
+      }
       ws->stats.regions_evaluated += n;
       ws->stats.eval_launches++;
       ws->stats.max_regions = std::max<std::uint64_t>(ws->stats.max_regions, n);
@@ -801,6 +824,9 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     double act_err = ws->h_scal->sum[3];
     std::size_t m = ws->h_scal->count;
     if (user_regions) cum.iters++;
+    rec.phase(it, "two_level_errorest");
+    rec.iteration(it, cum.estimate + iter_est, cum.errorest + iter_err, cum.estimate,
+                  cum.errorest, n, cum.nregions);
 
     // predict_split's snapshot of the list just evaluated, Workspace.cuh:244-248
     if (it == ws->snapshot_iter && cum.nregions == 0) {
@@ -836,6 +862,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       double fin_est = iter_est - act_est;
       double fin_err = iter_err - act_err;
       bool recount = false;
+      rec.phase(it, "compute_finished_estimates");
       // fix_error_budget_overflow, Workspace.cuh:121-146
       if (cum.errorest + fin_err > std::fabs(cum.estimate + iter_est) * epsrel) {
         ws->stats.budget_fixes++;
@@ -844,6 +871,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         fin_est = 0.;
         recount = true;
       }
+      rec.phase(it, "fix_error_budget_overflow");
       // heuristic_classify, Workspace.cuh:74-118
       const bool necessary = mem_ratio(n, d, total, mem_model) > 1.;
       bool terminate;
@@ -885,6 +913,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         if ((rc = finalize_read(ws, n, true, false))) return rc;
         m = ws->h_scal->count;
       }
+      rec.phase(it, "heuristic_classify");
       if (terminate) {
         ws->trace.push_back({n, 0, cum.estimate + iter_est, tot_err});
         cum.estimate += iter_est;
@@ -931,6 +960,12 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       ws->split = true;
       n = 2 * m;
       has_parents = true;
+    }
+    if (rec.level() > 0) {
+      // Sub_regions_filter::filter and Sub_region_splitter::split are one
+      // kernel here
+      rec.phase(it, "region_filtering");
+      rec.folded(it, "region_splitting");
     }
     if (!user_regions) cum.iters++;
   }
@@ -1002,6 +1037,20 @@ int pagani_rule_tables(int ndim, double* w, double* scale, double* norm, double*
     for (int s = 0; s <= PAGANI_NSETS; ++s) set_begin[s] = R.set_begin[s];
   if (gtab) std::memcpy(gtab, R.gtab.data(), R.gtab.size() * sizeof(double));
   return R.feval;
+}
+
+int pagani_rule_point(int ndim, int p, int axis, double* g)
+{
+  static std::mutex mu;
+  static pagani::RuleTables tables[PAGANI_MAX_DIM + 1];
+  if (!g || ndim < PAGANI_MIN_DIM || ndim > PAGANI_MAX_DIM || axis < 0 || axis >= ndim)
+    return PAGANI_EBADARG;
+  std::lock_guard<std::mutex> lock(mu);
+  pagani::RuleTables& R = tables[ndim];
+  if (R.feval == 0 && !pagani::build_rule(ndim, R)) return PAGANI_EBADARG;
+  if (p < 0 || p >= R.feval) return PAGANI_EBADARG;
+  *g = point_generator(R, p, axis);
+  return PAGANI_OK;
 }
 
 static thread_local std::string g_create_error;
@@ -1296,6 +1345,17 @@ int pagani_ws_set_predict_split(pagani_ws* ws, int enable)
 {
   if (!ws) return PAGANI_EBADARG;
   ws->predict_split = enable != 0;
+  return PAGANI_OK;
+}
+
+int pagani_ws_set_debug(pagani_ws* ws, int level, const char* dir, const char* id, int append)
+{
+  if (!ws || level < 0 || level > 3) return PAGANI_EBADARG;
+  ws->dbg.level = level;
+  ws->dbg.dir = dir && *dir ? dir : ".";
+  ws->dbg.plain = id == nullptr;
+  ws->dbg.id = id ? id : "";
+  ws->dbg.append = append != 0;
   return PAGANI_OK;
 }
 

@@ -74,3 +74,34 @@ extern "C" int pagani_header_bench_gaussian8(const double* a, const double* u, d
     return PAGANI_EHIP;
   }
 }
+
+// The plain path's evaluation kernel alone, for hardware-counter runs:
+// `launches` evaluations of the n regions (L, H: SoA bounds on the device,
+// unit-cube integration box) with the launch constants the workspace gives
+// a callback, on the workspace's stream; blocks until they finish.
+extern "C" int pagani_header_bench_eval_gaussian8(pagani_ws* ws, const double* a, const double* u,
+                                                  size_t n, const double* L, const double* H,
+                                                  double* est, double* err, uint8_t* dim,
+                                                  int launches)
+{
+  if (!ws || !a || !u || !L || !H || !est || !err || !dim || launches < 1) return PAGANI_EBADARG;
+  PlainGaussian8 f;
+  for (int i = 0; i < 8; ++i) {
+    f.a[i] = a[i];
+    f.u[i] = u[i];
+  }
+  pagani_eval_args args{};
+  int rc = pagani_ws_eval_consts(ws, nullptr, nullptr, n, &args.consts);
+  if (rc) return rc;
+  args.n = n;
+  args.L = L;
+  args.H = H;
+  args.est = est;
+  args.err = err;
+  args.dim = dim;
+  args.stream = pagani_ws_stream(ws);
+  for (int l = 0; l < launches; ++l)
+    if ((rc = pagani::eval_regions<8, PlainGaussian8>(f, &args))) return rc;
+  return hipStreamSynchronize(static_cast<hipStream_t>(args.stream)) == hipSuccess ? PAGANI_OK
+                                                                                : PAGANI_EHIP;
+}

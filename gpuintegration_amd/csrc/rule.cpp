@@ -196,4 +196,17 @@ bool build_rule(int d, RuleTables& R)
   return static_cast<int>(R.gtab.size()) == (F - R.set_begin[3]) * d;
 }
 
+double point_generator(const RuleTables& R, int p, int axis)
+{
+  if (p == 0) return 0.0;
+  if (p < R.set_begin[3]) {
+    const bool two = p >= R.set_begin[2];
+    const int k = p - (two ? R.set_begin[2] : R.set_begin[1]);
+    if (k / 2 != axis) return 0.0;
+    const double lam = two ? R.lambda[1] : R.lambda[0];
+    return (k & 1) ? -lam : lam;
+  }
+  return R.gtab[static_cast<std::size_t>(p - R.set_begin[3]) * R.ndim + axis];
+}
+
 }  // namespace pagani

@@ -30,6 +30,12 @@ struct RuleTables {
 // Points per region for ndim (quad.h:68-75).
 int feval_points(int ndim);
 
+// Generator coordinate `axis` of rule point p in the library's point order:
+// the centre, then +-lambda1 (set 1) and +-lambda2 (set 2) per axis,
+// axis-major with + before -, then the table of sets 3-8.  The point of a
+// region with centre c and width w is c - g w.
+double point_generator(const RuleTables& R, int p, int axis);
+
 // Build the tables; returns false for an unsupported ndim.
 bool build_rule(int ndim, RuleTables& out);
 

@@ -15,6 +15,7 @@
 
 #include "pagani.h"
 #include "phases.hpp"
+#include "recorder.hpp"
 #include "rule.hpp"
 
 namespace pagani {
@@ -171,6 +172,10 @@ struct pagani_ws {
   int tune_family = 0, tune_precision = PAGANI_FP64;
   pagani::internal::DevBuf<double> snapL, snapH;
   std::size_t snap_n = 0;
+  // debug recording of the following integrate calls (pagani_ws_set_debug)
+  // and the device buffer of its point values
+  pagani::internal::DebugConfig dbg;
+  pagani::internal::DevBuf<double> dfe;
 };
 
 namespace pagani {
@@ -269,6 +274,7 @@ inline void for_each_pool_pair(pagani_ws* a, pagani_ws* b, F&& f)
   f(a->foff, b->foff);
   f(a->snapL, b->snapL);
   f(a->snapH, b->snapH);
+  f(a->dfe, b->dfe);
 }
 
 // Move the larger of each pair of pool buffers from `from` to `to` (both on
@@ -303,6 +309,7 @@ inline void for_each_pool(pagani_ws* ws, F&& f)
   f(ws->foff);
   f(ws->snapL);
   f(ws->snapH);
+  f(ws->dfe);
 }
 
 inline int ensure_chunks(pagani_ws* ws, std::size_t n)

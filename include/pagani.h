@@ -89,7 +89,7 @@ typedef struct pagani_genz {
  * launch whose stamp differs from the version it was compiled with, so a
  * caller built against other headers fails with PAGANI_EUNSUPPORTED instead
  * of reading a list laid out differently. */
-#define PAGANI_EVAL_ABI_VERSION 2
+#define PAGANI_EVAL_ABI_VERSION 3
 
 /* Uniform data of one evaluation launch: the degree-9 rule (Rule.cuh),
  * the integration volume and the launch shape.  Built by the library; an
@@ -156,6 +156,11 @@ typedef struct pagani_eval_args {
   const double* H2;
   const uint8_t* pdim;
   const uint8_t* pdim2;
+  /* Optional (the debug recorder's point dump; host-driven launches only):
+   * when not NULL, the value of every rule point, fevals[t * feval + p], with
+   * p in the library's point order (pagani_rule_point).  pagani::eval_regions
+   * writes them with a second kernel. */
+  double* fevals;
 } pagani_eval_args;
 
 /* An evaluation callback launches the region-evaluation kernel for one
@@ -352,6 +357,40 @@ int pagani_ws_set_snapshot_iter(pagani_ws* ws, uint64_t iteration);
  * that rule to the following integrate calls instead of their
  * relerr_classification argument (a suspended run keeps its rule). */
 int pagani_ws_set_predict_split(pagani_ws* ws, int enable);
+/* The reference's debug recorders (Workspace<T, ndim, debug,
+ * use_custom, collect_mult_runs>: Workspace.cuh:171-179, 199-352, 377-431;
+ * PaganiUtils.cuh:50-241).  For the following integrate calls, level
+ *   0  records nothing (the default);
+ *   1  with id not NULL (the Sub_regions& overload, whose `optional`
+ *      argument is the id), writes <dir>/cuda_pagani_iters.csv, one line
+ *      per iteration after the error refinement ("it, estimate, errorest,
+ *      festimate, ferrorest, nregions, nfregions": the running totals with
+ *      and without this iteration's regions, the list size, the regions
+ *      finished before it), and <dir>/cuda_pagani_time_breakdown.csv ("id,
+ *      ndim, epsrel, it, name, time"): milliseconds per phase under the
+ *      reference's names apply_cubature_rules, two_level_errorest,
+ *      compute_finished_estimates, fix_error_budget_overflow,
+ *      heuristic_classify, region_filtering and region_splitting (host time
+ *      between stream synchronisations at the phase boundaries; the filter
+ *      and the split are one kernel here, timed as region_filtering, with
+ *      region_splitting 0); with id NULL (the plain overload) writes
+ *      <dir>/cuda_iters.csv ("it, estimate, errorest, nregions") only;
+ *   2  also <dir>/cuda_regions.csv: every region's estimate and rule error
+ *      as evaluated ("it, reg, estimate, errorest"; reg in list order);
+ *   3  also <dir>/cuda_generators.csv (the rule's generator table, "i, gen",
+ *      i = feval * axis + point) and, for lists under 1024 regions,
+ *      <dir>/cuda_fevals.csv: every rule point of every region (region and
+ *      box bounds, the point, the value, the region's estimate and error).
+ * append != 0 appends without header lines (collect_mult_runs).
+ * Values carry 17 significant digits.  Recording runs the host loop only (no
+ * device-resident rounds or graphs) with one stream sync per iteration; the
+ * results are those of an unrecorded run.  dir NULL means ".". */
+int pagani_ws_set_debug(pagani_ws* ws, int level, const char* dir, const char* id, int append);
+/* Generator coordinate g_axis of rule point p (library order: the centre;
+ * +-lambda1 and +-lambda2 per axis, axis-major, + before -; then the table of
+ * sets 3-8), for ndim-D rules; the point of a region with centre c and width
+ * w is c - g w.  Returns 0, or PAGANI_EBADARG. */
+int pagani_rule_point(int ndim, int p, int axis, double* g);
 int pagani_ws_snapshot_regions(const pagani_ws* ws, size_t* n, const double** L,
                                const double** H);
 

@@ -210,72 +210,43 @@ __device__ __forceinline__ double set_sum(F& f, const double* __restrict__ g,
   return acc;
 }
 
-// Rule sums and split axis of one region for a separable integrand
-// (slices == 1: this wave evaluates all points).  Points are visited in the
-// generic order of every set, and each set's values are summed in that
-// order, as set_sum does; a point x = c - g h changes v_j by -g h_j.
-// Returns false, having done nothing, for an exponential integrand whose
-// region fails factor_safe.
+// Rule sums and split axis of one region for a factored separable
+// integrand (exponential, or multiplicative with a linear outer), slices ==
+// 1.  A point's value is the centre's value times the changed axes'
+// factors, so every set's sum over its sign combinations factors per axis:
+// e.g. the 4 points (+-l1, +-l1) on axes j, k sum to S0 s_j s_k with s_j =
+// r(+l1)_j + r(-l1)_j, and the 2^d corners to S0 prod_j s5_j.  With masks, a
+// factor is 0 where the moved coordinate is invalid and an unchanged axis
+// contributes u_j = [valid at the centre]: the sum over the sets of m
+// changed axes is the x^m coefficient of prod_j (u_j + x s_j).
+// One pass over the axes after the centre: axis j's ten factors (+-lambda1
+// .. +-lambda5) are used as soon as they are formed -- the fourth difference,
+// the axis sums of sets 1-4, the running coefficients of sets 5-7 and the
+// corners' product -- so no per-axis array outlives its axis and the region
+// needs few registers.  Every sum adds its terms in axis order.
 template <int D, class F>
-__device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], const double (&hR)[D],
-                                               double jac, const pagani_eval_consts& C,
-                                               double (&rs)[PAGANI_NRULES], int& bisect)
+__device__ __forceinline__ bool factored_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+                                              double jac, const pagani_eval_consts& C,
+                                              double (&rs)[PAGANI_NRULES], int& bisect)
 {
   constexpr bool kExp = is_exponential<F>::value;
   constexpr bool kMul = is_multiplicative<F>::value || kExp;
   using Op = SepOp<kMul>;
   constexpr bool kMasked = is_masked<F>::value;
-  // A point's value is the centre's value times the changed axes' factors
-  // (exponential, or multiplicative with a linear outer): a set's sum over
-  // every sign combination on a set of axes then factors per axis, e.g.
-  // sum over the 4 points (+-l1, +-l1) on axes j, k = S0 s_j s_k with
-  // s_j = r(+l1)_j + r(-l1)_j, and the 2^d corners sum to S0 prod_j s5_j.
-  // With masks, a factor is 0 where the moved coordinate is invalid and an
-  // unchanged axis contributes u_j = [valid at the centre]: the sum over the
-  // sets of m changed axes is the x^m coefficient of prod_j (u_j + x s_j).
-  constexpr bool kFactor = kExp || (kMul && has_linear_outer<F>::value);
-  static_assert(!(kMasked && kMul && !kFactor), "masks need additive terms or factored sums");
   const int bisect0 = bisect;
-  double T0[D];
-#pragma unroll
-  for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
-  // axes invalid at the centre: a point is zero unless it changes them all
+  // the centre's joined terms (exponential: the centre's value) and the
+  // axes invalid at the centre (a point is zero unless it changes them all)
   std::uint32_t vmask = 0;
-  if constexpr (kMasked) {
+  double S0 = f.term(0, cR[0]);
 #pragma unroll
-    for (int j = 0; j < D; ++j)
+  for (int j = 0; j < D; ++j) {
+    if (j > 0) {
+      const double t = f.term(j, cR[j]);
+      S0 = kExp ? S0 + t : Op::join(S0, t);
+    }
+    if constexpr (kMasked)
       if (!f.valid(j, cR[j])) vmask |= 1u << j;
   }
-  // the change of axis j's term when its coordinate moves to v
-  // (exponential: the factor of the value)
-  auto change = [&](int j, double v) {
-    if constexpr (kMasked)
-      if (!f.valid(j, v)) return kMul ? 0.0 : -__builtin_inf();
-    if constexpr (kExp) return f.factor(f.term(j, v) - T0[j]);
-    return Op::change(f.term(j, v), T0[j]);
-  };
-  // a point's value from its joined terms (exponential: the value itself);
-  // zero when an unchanged axis is invalid at the centre
-  auto gated = [&](std::uint32_t changed, double s) {
-    if constexpr (kMasked)
-      if (vmask & ~changed) return 0.0;
-    if constexpr (kExp) return s;
-    return f.outer(s);
-  };
-  // the value of a factored sum of points (no masks applied here)
-  auto value = [&](double p) {
-    if constexpr (kExp) return p;
-    return f.outer(p);
-  };
-  // u_j: 1 when an unchanged axis j keeps the point valid
-  auto unchanged = [&](int j) {
-    if constexpr (kMasked) return ((vmask >> j) & 1u) ? 0.0 : 1.0;
-    return 1.0;
-  };
-  // the centre's joined terms (exponential: the centre's value)
-  double S0 = T0[0];
-#pragma unroll
-  for (int j = 1; j < D; ++j) S0 = kExp ? S0 + T0[j] : Op::join(S0, T0[j]);
   if constexpr (kExp) {
     if (!f.factor_safe(S0)) return false;
     if constexpr (has_centre_value<F>::value)
@@ -283,92 +254,206 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
     else
       S0 = f.outer(S0);
   }
-  // term changes (differences or ratios) at g = +lambda (x = c - lambda h)
-  // and g = -lambda
-  double d1p[D], d1n[D], d2p[D], d2n[D];
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    d1p[j] = change(j, __builtin_fma(-C.lambda1, hR[j], cR[j]));
-    d1n[j] = change(j, __builtin_fma(C.lambda1, hR[j], cR[j]));
-    d2p[j] = change(j, __builtin_fma(-C.lambda2, hR[j], cR[j]));
-    d2n[j] = change(j, __builtin_fma(C.lambda2, hR[j], cR[j]));
-  }
-  // sets 0-2 and the fourth difference, Sample.cuh:176-218
+  auto gated = [&](std::uint32_t changed, double s) {
+    if constexpr (kMasked)
+      if (vmask & ~changed) return 0.0;
+    if constexpr (kExp) return s;
+    return f.outer(s);
+  };
+  auto value = [&](double p) {
+    if constexpr (kExp) return p;
+    return f.outer(p);
+  };
   const double f0 = gated(0u, S0);
   const double base = (f0 * jac) * 2 * (1 - C.ratio);
   double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+  double acc3 = 0.0, E3 = 1.0, acc4 = 0.0, E4 = 1.0;  // sets 3, 4: x^1, x^0 coefficients
+  // sets 5-7: the x^2, xy and x^3 coefficients of prod_j (u_j + x s1_j + y
+  // s2_j), with the lower ones they are built from
+  double e0 = 1.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, g10 = 0.0, g01 = 0.0, g11 = 0.0;
+  double prod = S0;  // set 8
 #pragma unroll
   for (int j = 0; j < D; ++j) {
-    const double fa = gated(1u << j, Op::join(S0, d1p[j]));
-    const double fb = gated(1u << j, Op::join(S0, d1n[j]));
-    const double fc = gated(1u << j, Op::join(S0, d2p[j]));
-    const double fd = gated(1u << j, Op::join(S0, d2n[j]));
-    s1 += fa + fb;
-    s2 += fc + fd;
-    const double diff =
-        __builtin_fabs(base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
-    if (diff > maxdiff) {
-      maxdiff = diff;
-      bisect = j;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < PAGANI_NRULES; ++r)
-    rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
-  // sets 3-4: +-lambda3, +-lambda4 on one axis
-  const double lam34[2] = {C.lambda3, C.lambda4};
-#pragma unroll
-  for (int s = 3; s <= 4; ++s) {
-    const double lam = lam34[s - 3];
-    double acc = 0.0;
-    if constexpr (kFactor) {
-      double E0 = 1.0;  // x^0 and x^1 coefficients of prod_j (u_j + x s_j)
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        const double sj = change(j, __builtin_fma(-lam, hR[j], cR[j])) +
-                          change(j, __builtin_fma(lam, hR[j], cR[j]));
-        const double uj = unchanged(j);
-        acc = __builtin_fma(E0, sj, acc * uj);
-        E0 *= uj;
-      }
-      acc = value(S0 * acc);
-    } else {
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(-lam, hR[j], cR[j]))));
-        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(lam, hR[j], cR[j]))));
+    const double t0 = f.term(j, cR[j]);
+    // the factor (exponential) or term ratio of axis j moved to v
+    auto change = [&](double v) {
+      if constexpr (kMasked)
+        if (!f.valid(j, v)) return 0.0;
+      if constexpr (kExp) return f.factor(f.term(j, v) - t0);
+      return Op::change(f.term(j, v), t0);
+    };
+    const double uj = kMasked ? (((vmask >> j) & 1u) ? 0.0 : 1.0) : 1.0;
+    const double d1p = change(__builtin_fma(-C.lambda1, hR[j], cR[j]));
+    const double d1n = change(__builtin_fma(C.lambda1, hR[j], cR[j]));
+    const double d2p = change(__builtin_fma(-C.lambda2, hR[j], cR[j]));
+    const double d2n = change(__builtin_fma(C.lambda2, hR[j], cR[j]));
+    // sets 0-2 and the fourth difference, Sample.cuh:176-218
+    {
+      const double fa = gated(1u << j, Op::join(S0, d1p));
+      const double fb = gated(1u << j, Op::join(S0, d1n));
+      const double fc = gated(1u << j, Op::join(S0, d2p));
+      const double fd = gated(1u << j, Op::join(S0, d2n));
+      s1 += fa + fb;
+      s2 += fc + fd;
+      const double diff =
+          __builtin_fabs(base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
+      if (diff > maxdiff) {
+        maxdiff = diff;
+        bisect = j;
       }
     }
-#pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
-  }
-  if constexpr (kFactor) {
-    // sets 5-7 from the per-axis sums s1_j, s2_j of the +-lambda1 and
-    // +-lambda2 factors: set 5 sums S0 s1_j s1_k over j < k, set 6 S0 s1_j
-    // s2_k over j != k, set 7 S0 s1_j s1_k s1_l over j < k < l -- the x^2,
-    // xy and x^3 coefficients of prod_j (u_j + x s1_j + y s2_j).  Every sum
-    // below adds non-negative products when the factors are positive.
-    double e0 = 1.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, g10 = 0.0, g01 = 0.0, g11 = 0.0;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      const double a1 = d1p[j] + d1n[j];
-      const double a2 = d2p[j] + d2n[j];
-      const double uj = unchanged(j);
+    // sets 3-4: +-lambda3, +-lambda4 on one axis
+    {
+      const double s3 = change(__builtin_fma(-C.lambda3, hR[j], cR[j])) +
+                        change(__builtin_fma(C.lambda3, hR[j], cR[j]));
+      acc3 = __builtin_fma(E3, s3, acc3 * uj);
+      E3 *= uj;
+      const double s4 = change(__builtin_fma(-C.lambda4, hR[j], cR[j])) +
+                        change(__builtin_fma(C.lambda4, hR[j], cR[j]));
+      acc4 = __builtin_fma(E4, s4, acc4 * uj);
+      E4 *= uj;
+    }
+    // sets 5-7 (the pairs (j, k) and (k, j) with k < j)
+    {
+      const double a1 = d1p + d1n;
+      const double a2 = d2p + d2n;
       e3 = __builtin_fma(e2, a1, e3 * uj);
       e2 = __builtin_fma(e1, a1, e2 * uj);
-      // the pairs (j, k) and (k, j) with k < j
       g11 = __builtin_fma(g01, a1, __builtin_fma(g10, a2, g11 * uj));
       g10 = __builtin_fma(e0, a1, g10 * uj);
       g01 = __builtin_fma(e0, a2, g01 * uj);
       e1 = g10;
       e0 *= uj;
     }
-    const double v5 = value(S0 * e2), v6 = value(S0 * g11), v7 = value(S0 * e3);
+    // set 8: the 2^D corners, g_j = +-lambda5 on every axis
+    prod *= change(__builtin_fma(-C.lambda5, hR[j], cR[j])) +
+            change(__builtin_fma(C.lambda5, hR[j], cR[j]));
+  }
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r)
+    rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
+  const double v3 = value(S0 * acc3), v4 = value(S0 * acc4);
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[3][r], v3, rs[r]);
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[4][r], v4, rs[r]);
+  const double v5 = value(S0 * e2), v6 = value(S0 * g11), v7 = value(S0 * e3);
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r)
+    rs[r] = __builtin_fma(C.w[7][r], v7, __builtin_fma(C.w[6][r], v6,
+                                                       __builtin_fma(C.w[5][r], v5, rs[r])));
+  const double v8 = value(prod);
+#pragma unroll
+  for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], v8, rs[r]);
+  if constexpr (kMasked && kExp) {
+    // a masked exponential's factors are bounded only by the box: a region
+    // whose sums left the range takes the generic loop
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < PAGANI_NRULES; ++r) t += __builtin_fabs(rs[r]);
+    if (!__builtin_isfinite(t)) {
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = 0.0;
+      bisect = bisect0;
+      return false;
+    }
+  }
+  return true;
+}
+
+// Rule sums and split axis of one region for a separable integrand
+// (slices == 1: this wave evaluates all points).  Points are visited in the
+// generic order of every set, and each set's values are summed in that
+// order, as set_sum does; a point x = c - g h changes v_j by -g h_j.
+// Returns false, having done nothing, for an exponential integrand whose
+// region fails factor_safe.  Factored integrands (exponential, or
+// multiplicative with a linear outer) take factored_sums; the rest join the
+// centre's terms with the changed axes' term changes point by point.
+template <int D, class F>
+__device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], const double (&hR)[D],
+                                               double jac, const pagani_eval_consts& C,
+                                               double (&rs)[PAGANI_NRULES], int& bisect)
+{
+  constexpr bool kExp = is_exponential<F>::value;
+  constexpr bool kMul = is_multiplicative<F>::value || kExp;
+  constexpr bool kMasked = is_masked<F>::value;
+  constexpr bool kFactor = kExp || (kMul && has_linear_outer<F>::value);
+  static_assert(!(kMasked && kMul && !kFactor), "masks need additive terms or factored sums");
+  if constexpr (kFactor) {
+    return factored_sums<D, F>(f, cR, hR, jac, C, rs, bisect);
+  } else {
+    using Op = SepOp<kMul>;
+    double T0[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) T0[j] = f.term(j, cR[j]);
+    // axes invalid at the centre: a point is zero unless it changes them all
+    std::uint32_t vmask = 0;
+    if constexpr (kMasked) {
+#pragma unroll
+      for (int j = 0; j < D; ++j)
+        if (!f.valid(j, cR[j])) vmask |= 1u << j;
+    }
+    // the change of axis j's term when its coordinate moves to v
+    auto change = [&](int j, double v) {
+      if constexpr (kMasked)
+        if (!f.valid(j, v)) return kMul ? 0.0 : -__builtin_inf();
+      return Op::change(f.term(j, v), T0[j]);
+    };
+    // a point's value from its joined terms; zero when an unchanged axis is
+    // invalid at the centre
+    auto gated = [&](std::uint32_t changed, double s) {
+      if constexpr (kMasked)
+        if (vmask & ~changed) return 0.0;
+      return f.outer(s);
+    };
+    double S0 = T0[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) S0 = Op::join(S0, T0[j]);
+    // term changes at g = +lambda (x = c - lambda h) and g = -lambda
+    double d1p[D], d1n[D], d2p[D], d2n[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      d1p[j] = change(j, __builtin_fma(-C.lambda1, hR[j], cR[j]));
+      d1n[j] = change(j, __builtin_fma(C.lambda1, hR[j], cR[j]));
+      d2p[j] = change(j, __builtin_fma(-C.lambda2, hR[j], cR[j]));
+      d2n[j] = change(j, __builtin_fma(C.lambda2, hR[j], cR[j]));
+    }
+    // sets 0-2 and the fourth difference, Sample.cuh:176-218
+    const double f0 = gated(0u, S0);
+    const double base = (f0 * jac) * 2 * (1 - C.ratio);
+    double s1 = 0.0, s2 = 0.0, maxdiff = 0.0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const double fa = gated(1u << j, Op::join(S0, d1p[j]));
+      const double fb = gated(1u << j, Op::join(S0, d1n[j]));
+      const double fc = gated(1u << j, Op::join(S0, d2p[j]));
+      const double fd = gated(1u << j, Op::join(S0, d2n[j]));
+      s1 += fa + fb;
+      s2 += fc + fd;
+      const double diff =
+          __builtin_fabs(base + C.ratio * (fa * jac + fb * jac) - (fc * jac + fd * jac));
+      if (diff > maxdiff) {
+        maxdiff = diff;
+        bisect = j;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < PAGANI_NRULES; ++r)
-      rs[r] = __builtin_fma(C.w[7][r], v7, __builtin_fma(C.w[6][r], v6,
-                                                         __builtin_fma(C.w[5][r], v5, rs[r])));
-  } else {
+      rs[r] = C.w[0][r] * f0 + C.w[1][r] * s1 + C.w[2][r] * s2;
+    // sets 3-4: +-lambda3, +-lambda4 on one axis
+    const double lam34[2] = {C.lambda3, C.lambda4};
+#pragma unroll
+    for (int s = 3; s <= 4; ++s) {
+      const double lam = lam34[s - 3];
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(-lam, hR[j], cR[j]))));
+        acc += gated(1u << j, Op::join(S0, change(j, __builtin_fma(lam, hR[j], cR[j]))));
+      }
+#pragma unroll
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[s][r], acc, rs[r]);
+    }
     // set 5: (+-lambda1, +-lambda1) on axes j < k
     {
       double acc = 0.0;
@@ -429,98 +514,67 @@ __device__ __forceinline__ bool separable_sums(F& f, const double (&cR)[D], cons
 #pragma unroll
       for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[7][r], acc, rs[r]);
     }
-  }
-  // set 8: the 2^D corners x = c - g h, g_j = +lambda5 where bit j of the
-  // corner index is 0 and -lambda5 where it is 1 (the table's order).  The
-  // low LB axes' combined terms are tabulated once; each group of 2^LB
-  // corners adds the high axes' combination.  An integrand whose outer
-  // function is long (heavy_outer) takes the table loop instead: unrolled
-  // 2^LB times, its outer would not fit the registers.
-  if constexpr (kFactor) {
-    double prod = S0;
+    // set 8: the 2^D corners x = c - g h, g_j = +lambda5 where bit j of the
+    // corner index is 0 and -lambda5 where it is 1 (the table's order).  The
+    // low LB axes' combined terms are tabulated once; each group of 2^LB
+    // corners adds the high axes' combination.  An integrand whose outer
+    // function is long (heavy_outer) takes the table loop instead: unrolled
+    // 2^LB times, its outer would not fit the registers.
+    if constexpr (heavy_outer<F>::value) {
+      const int p0 = C.set_begin[3];
+      const double acc = set_sum<D, F, double>(
+          f, C.gtab + static_cast<std::size_t>(C.set_begin[8] - p0) * D, C.set_begin[8],
+          C.set_begin[9], cR, hR);
 #pragma unroll
-    for (int j = 0; j < D; ++j)
-      prod *= change(j, __builtin_fma(-C.lambda5, hR[j], cR[j])) +
-              change(j, __builtin_fma(C.lambda5, hR[j], cR[j]));
-    const double acc = value(prod);
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+    } else {
+      constexpr int LB = D < 4 ? D : 4;
+      constexpr int HB = D - LB;
+      double tp[D], tn[D];
 #pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
-    if constexpr (kMasked && kExp) {
-      // a masked exponential's factors are bounded only by the box: a
-      // region whose sums left the range takes the generic loop
-      double t = 0.0;
-#pragma unroll
-      for (int r = 0; r < PAGANI_NRULES; ++r) t += __builtin_fabs(rs[r]);
-      if (!__builtin_isfinite(t)) {
-#pragma unroll
-        for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = 0.0;
-        bisect = bisect0;
-        return false;
-      }
-    }
-  } else if constexpr (heavy_outer<F>::value) {
-    const int p0 = C.set_begin[3];
-    const double acc = set_sum<D, F, double>(
-        f, C.gtab + static_cast<std::size_t>(C.set_begin[8] - p0) * D, C.set_begin[8],
-        C.set_begin[9], cR, hR);
-#pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
-  } else {
-    constexpr int LB = D < 4 ? D : 4;
-    constexpr int HB = D - LB;
-    double tp[D], tn[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      // the corners change every axis: plain terms, -infinity where invalid
-      // (exponential: the factors against the centre)
-      const double vp = __builtin_fma(-C.lambda5, hR[j], cR[j]);
-      const double vn = __builtin_fma(C.lambda5, hR[j], cR[j]);
-      if constexpr (kExp) {
-        tp[j] = change(j, vp);
-        tn[j] = change(j, vn);
-      } else {
+      for (int j = 0; j < D; ++j) {
+        // the corners change every axis: plain terms, -infinity where invalid
+        const double vp = __builtin_fma(-C.lambda5, hR[j], cR[j]);
+        const double vn = __builtin_fma(C.lambda5, hR[j], cR[j]);
         tp[j] = f.term(j, vp);
         tn[j] = f.term(j, vn);
+        if constexpr (kMasked) {
+          if (!f.valid(j, vp)) tp[j] = -__builtin_inf();
+          if (!f.valid(j, vn)) tn[j] = -__builtin_inf();
+        }
       }
-      if constexpr (kMasked) {
-        if (!f.valid(j, vp)) tp[j] = -__builtin_inf();
-        if (!f.valid(j, vn)) tn[j] = -__builtin_inf();
-      }
-    }
-    double lo[1 << LB];
-    lo[0] = tp[0];
-    lo[1] = tn[0];
+      double lo[1 << LB];
+      lo[0] = tp[0];
+      lo[1] = tn[0];
 #pragma unroll
-    for (int j = 1; j < LB; ++j) {
+      for (int j = 1; j < LB; ++j) {
 #pragma unroll
-      for (int q = 0; q < (1 << j); ++q) {
-        lo[q + (1 << j)] = Op::join(lo[q], tn[j]);
-        lo[q] = Op::join(lo[q], tp[j]);
+        for (int q = 0; q < (1 << j); ++q) {
+          lo[q + (1 << j)] = Op::join(lo[q], tn[j]);
+          lo[q] = Op::join(lo[q], tp[j]);
+        }
       }
-    }
-    double acc = 0.0;
+      double acc = 0.0;
 #pragma unroll 1
-    for (int hq = 0; hq < (1 << HB); ++hq) {
-      double hs = 0.0;
-      if constexpr (HB > 0) {
-        hs = (hq & 1) ? tn[LB] : tp[LB];
+      for (int hq = 0; hq < (1 << HB); ++hq) {
+        double hs = 0.0;
+        if constexpr (HB > 0) {
+          hs = (hq & 1) ? tn[LB] : tp[LB];
 #pragma unroll
-        for (int j = LB + 1; j < D; ++j) hs = Op::join(hs, ((hq >> (j - LB)) & 1) ? tn[j] : tp[j]);
-      }
+          for (int j = LB + 1; j < D; ++j)
+            hs = Op::join(hs, ((hq >> (j - LB)) & 1) ? tn[j] : tp[j]);
+        }
 #pragma unroll
-      for (int q = 0; q < (1 << LB); ++q) {
-        const double t = HB > 0 ? Op::join(lo[q], hs) : lo[q];
-        if constexpr (kExp)
-          acc += t;
-        else
+        for (int q = 0; q < (1 << LB); ++q) {
+          const double t = HB > 0 ? Op::join(lo[q], hs) : lo[q];
           acc += f.outer(t);
+        }
       }
-    }
-    if constexpr (kExp) acc *= S0;
 #pragma unroll
-    for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+      for (int r = 0; r < PAGANI_NRULES; ++r) rs[r] = __builtin_fma(C.w[8][r], acc, rs[r]);
+    }
+    return true;
   }
-  return true;
 }
 
 // Rule sums and split axis of one region for a cosine integrand (slices ==
@@ -642,24 +696,20 @@ struct RegionRow {
   }
 };
 
-// One wave's share of the evaluation of up to 64 regions, one lane per
-// region t of an n-region SoA list (split form when pdim is not NULL).
-// `wave` indexes the `slices` waves that split the batch's points (0 when
-// slices == 1); with more than one, every wave of the workgroup passes one
-// barrier.  `lows`, `range`, `jac` describe the integration box.
-template <int D, class F, class Real>
-__device__ __forceinline__ void eval_regions_wave(
-    F& f, const double* __restrict__ L, const double* __restrict__ H,
-    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t, bool live,
-    const double* lows, const double* range, double jac, const pagani_eval_consts& C, int slices,
-    int wave, int lane, double* part, double* __restrict__ est, double* __restrict__ err,
-    std::uint8_t* __restrict__ dim)
+// Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
+// width of region t in the integrand's frame (user coordinates, moved to the
+// origin and scaled for a centred integrand); a rule point is x = c - g w.
+template <int D, class F>
+__device__ __forceinline__ void region_frame(F& f, const double* __restrict__ L,
+                                             const double* __restrict__ H,
+                                             const std::uint8_t* __restrict__ pdim, std::size_t n,
+                                             std::size_t t, const double* lows,
+                                             const double* range, double (&cR)[D],
+                                             double (&hR)[D], double& vol, int& maxdim)
 {
-  // Region setup, Phases.cuh:323-349: volume, widest axis, and the centre /
-  // width in user coordinates.
-  double cR[D], hR[D];
-  double vol = 1.0, maxr = 0.0;
-  int maxdim = 0;
+  double maxr = 0.0;
+  vol = 1.0;
+  maxdim = 0;
   const RegionRow rr(pdim, n, t);
 #pragma unroll
   for (int j = 0; j < D; ++j) {
@@ -683,6 +733,40 @@ __device__ __forceinline__ void eval_regions_wave(
       hR[j] *= f.axis_scale(j);
     }
   }
+}
+
+// Generator coordinate j of rule point p (the order of pagani_rule_point):
+// the centre, +-lambda1 and +-lambda2 per axis, then the table.
+__device__ __forceinline__ double point_generator(const pagani_eval_consts& C, int p, int j)
+{
+  if (p == 0) return 0.0;
+  if (p < C.set_begin[3]) {
+    const bool two = p >= C.set_begin[2];
+    const int k = p - (two ? C.set_begin[2] : C.set_begin[1]);
+    if (k / 2 != j) return 0.0;
+    const double lam = two ? C.lambda2 : C.lambda1;
+    return (k & 1) ? -lam : lam;
+  }
+  return C.gtab[static_cast<std::size_t>(p - C.set_begin[3]) * C.ndim + j];
+}
+
+// One wave's share of the evaluation of up to 64 regions, one lane per
+// region t of an n-region SoA list (split form when pdim is not NULL).
+// `wave` indexes the `slices` waves that split the batch's points (0 when
+// slices == 1); with more than one, every wave of the workgroup passes one
+// barrier.  `lows`, `range`, `jac` describe the integration box.
+template <int D, class F, class Real>
+__device__ __forceinline__ void eval_regions_wave(
+    F& f, const double* __restrict__ L, const double* __restrict__ H,
+    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t, bool live,
+    const double* lows, const double* range, double jac, const pagani_eval_consts& C, int slices,
+    int wave, int lane, double* part, double* __restrict__ est, double* __restrict__ err,
+    std::uint8_t* __restrict__ dim)
+{
+  double cR[D], hR[D];
+  double vol;
+  int maxdim;
+  region_frame<D>(f, L, H, pdim, n, t, lows, range, cR, hR, vol, maxdim);
 
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int bisect = maxdim;
@@ -799,9 +883,10 @@ __device__ __forceinline__ void eval_regions_wave(
 
 }  // namespace detail
 
-// Kernel: workgroup b evaluates regions [64 b, 64 b + 64) of an n-region
-// SoA list (flat, or in split form when P is given) with S = slices waves
-// sharing the batch's points.  For a
+// Kernel: workgroup b evaluates a batch of 64 regions of an n-region SoA
+// list (flat: [64 b, 64 b + 64); split form when P is given: the children
+// of parents [32 b, 32 b + 32)) with S = slices waves sharing the batch's
+// points.  For a
 // host-driven launch `it` is NULL and n, the pool (L0, H0) and S = C.slices
 // come from the arguments.  In the device-resident loop they come from *it
 // (nothing at all if !it->run), and the launch is sized for the round's
@@ -844,11 +929,48 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     if (slices > 1) __syncthreads();
     return;
   }
-  const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
-  const bool live = t0 < n;
-  const std::size_t t = live ? t0 : n - 1;
+  // A flat list gives workgroup b the regions 64 b + lane.  A split list
+  // (n = 2m children, child t of parent t mod m) gives it both children of
+  // the parents 32 b .. 32 b + 31: lanes l and l + 32 read the same parent
+  // row in one load instruction, so each row leaves HBM once, and the writes
+  // are two runs of 32 consecutive regions.
+  std::size_t t;
+  bool live;
+  if (P) {
+    const std::size_t m = n / 2;
+    const std::size_t p0 = static_cast<std::size_t>(blockIdx.x) * 32 + (lane & 31);
+    live = p0 < m;
+    t = (live ? p0 : m - 1) + (lane >= 32 ? m : 0);
+  } else {
+    const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
+    live = t0 < n;
+    t = live ? t0 : n - 1;
+  }
   detail::eval_regions_wave<D, F, Real>(f, L, H, P, n, t, live, C.lows, C.range, C.jac, C,
                                         slices, wave, lane, part, est, err, dim);
+}
+
+// The debug recorder's point dump: thread i writes the value at rule point
+// i mod feval of region i / feval, formed as the evaluation kernel forms it.
+template <int D, class F, class Real>
+__global__ void __launch_bounds__(256)
+    point_values_kernel(F f, std::size_t n, const double* __restrict__ L,
+                        const double* __restrict__ H, const std::uint8_t* __restrict__ P,
+                        double* __restrict__ out, const pagani_eval_consts C)
+{
+  const std::size_t i = static_cast<std::size_t>(blockIdx.x) * 256 + threadIdx.x;
+  const std::size_t F_ = static_cast<std::size_t>(C.feval);
+  const std::size_t t = i / F_;
+  if (t >= n) return;
+  const int p = static_cast<int>(i % F_);
+  double cR[D], hR[D], vol;
+  int maxdim;
+  detail::region_frame<D>(f, L, H, P, n, t, C.lows, C.range, cR, hR, vol, maxdim);
+  Real x[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    x[j] = static_cast<Real>(__builtin_fma(-detail::point_generator(C, p, j), hR[j], cR[j]));
+  out[i] = detail::call(f, x);
 }
 
 // Launch the evaluation of args->n regions for integrand f.  Returns 0 or
@@ -875,6 +997,12 @@ int eval_regions(const F& f, const pagani_eval_args* args)
                      args->dev ? args->H2 : args->H, args->pdim,
                      args->dev ? args->pdim2 : args->pdim, args->est, args->err, args->dim,
                      args->consts);
+  if (args->fevals && !args->dev) {
+    const std::size_t threads = args->n * static_cast<std::size_t>(args->consts.feval);
+    hipLaunchKernelGGL((point_values_kernel<D, F, Real>), dim3(static_cast<unsigned>((threads + 255) / 256)),
+                       dim3(256), 0, static_cast<hipStream_t>(args->stream), f, args->n, args->L,
+                       args->H, args->pdim, args->fevals, args->consts);
+  }
   return hipGetLastError() == hipSuccess ? PAGANI_OK : PAGANI_EHIP;
 }
 

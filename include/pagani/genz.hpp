@@ -23,32 +23,38 @@ namespace math {
 
 // e^x for x <= 709 (the exponents of the Gaussian and C0 families are minus
 // sums of non-negative terms, the discontinuous family's are bounded by
-// sum a_i).  x = n ln2 + r with n = rint(x / ln2) and |r| <= ln2/2, r from a
-// two-part ln2 (Cody-Waite); e^r by a degree-11 polynomial; 2^n by ldexp,
-// which also rounds subnormal results, gives 0 below about -745.1 and
-// infinity above about 709.78.  The coefficients interpolate e^r at the 12
-// Chebyshev nodes of [-ln2/2, ln2/2] (computed in extended precision, then
-// rounded to double; the check that recomputes them is in the tests): max
-// polynomial error 0.14 ulp.  The range handling is one clamp, which also
-// maps -inf to 0 (and NaN to 0: the families never produce NaN from finite
-// input).
+// sum a_i).  x = n ln2 + r with n the integer nearest x / ln2 and |r| <=
+// ln2/2, r from a two-part ln2 (Cody-Waite); e^r by a degree-11 polynomial;
+// 2^n by ldexp, which also rounds subnormal results, gives 0 below about
+// -745.1 and infinity above about 709.78.  n comes from one FMA: adding
+// 1.5 * 2^52 to x log2e (|x log2e| < 2^51) leaves a double whose unit is 1,
+// so the sum is rounded to the nearest integer, which its low 32 bits hold
+// in two's complement; subtracting the shift again gives n as a double (no
+// separate multiply, round or conversion instruction).  The coefficients
+// interpolate e^r at the 12 Chebyshev nodes of [-ln2/2, ln2/2] (computed in
+// extended precision, then rounded to double; the check that recomputes
+// them is in the tests): max polynomial error 0.14 ulp.  The range handling
+// is one clamp, which also maps -inf to 0 (and NaN to 0: the families never
+// produce NaN from finite input).
 __device__ __forceinline__ double exp_bounded(double x)
 {
   constexpr double kLog2e = 0x1.71547652b82fep+0;
   constexpr double kLn2Hi = 0x1.62e42fefa39efp-1;
   constexpr double kLn2Lo = 0x1.abc9e3b39803fp-56;
+  constexpr double kShift = 0x1.8p+52;
   constexpr double c[12] = {
       0x1.0000000000000p+0,  0x1.0000000000000p+0,  0x1.0000000000011p-1, 0x1.555555555555fp-3,
       0x1.555555554f0c7p-5,  0x1.111111110e636p-7,  0x1.6c16c187face9p-10, 0x1.a01a01b4a99f9p-13,
       0x1.a01991b378c3ap-16, 0x1.71ddeea4df7bbp-19, 0x1.28b3fce70e98ep-22, 0x1.af6d18013a587p-26};
   x = __builtin_fmax(x, -1.0e4);
-  const double n = __builtin_rint(x * kLog2e);
+  const double t = __builtin_fma(x, kLog2e, kShift);
+  const double n = t - kShift;
   double r = __builtin_fma(-n, kLn2Hi, x);
   r = __builtin_fma(-n, kLn2Lo, r);
   double p = c[11];
 #pragma unroll
   for (int k = 10; k >= 0; --k) p = __builtin_fma(p, r, c[k]);
-  return __builtin_ldexp(p, static_cast<int>(n));
+  return __builtin_ldexp(p, static_cast<int>(__builtin_bit_cast(unsigned long long, t)));
 }
 __device__ __forceinline__ double exp_neg(double x) { return exp_bounded(x); }
 __device__ __forceinline__ float exp_neg(float x) { return ::expf(x); }

@@ -194,9 +194,11 @@ struct Sub_regions {
   }
 };
 
-// Workspace (Workspace.cuh:23-71).  debug / use_custom / collect_mult_runs
-// are accepted for source compatibility; the library always uses its own
-// deterministic reductions.
+// Workspace (Workspace.cuh:23-71).  debug > 0 turns on the reference's
+// debug recorders for this workspace's integrate calls (CSV files in the
+// working directory, pagani_ws_set_debug; collect_mult_runs appends to
+// them); use_custom is accepted for source compatibility -- the library
+// always uses its own deterministic reductions.
 template <typename T, std::size_t ndim, int debug = 0, bool use_custom = false,
           bool collect_mult_runs = false>
 class Workspace {
@@ -219,6 +221,8 @@ class Workspace {
   {
     IntegT f = integrand;
     pagani_result r;
+    if constexpr (debug > 0)
+      pagani_ws_set_debug(ws_, debug < 3 ? debug : 3, ".", nullptr, collect_mult_runs ? 1 : 0);
     // this overload has no caller-owned list to keep a predict_split snapshot in
     pagani_ws_set_snapshot_iter(ws_, ~std::uint64_t(0));
     pagani_ws_set_predict_split(ws_, predict_split ? 1 : 0);
@@ -238,9 +242,13 @@ class Workspace {
                                        bool relerr_classification = true,
                                        const std::string& optional = "default")
   {
-    (void)optional;
     IntegT f = integrand;
     pagani_result r;
+    if constexpr (debug > 0)
+      pagani_ws_set_debug(ws_, debug < 3 ? debug : 3, ".", optional.c_str(),
+                          collect_mult_runs ? 1 : 0);
+    else
+      (void)optional;
     pagani_region_list init{subregions.size, subregions.dLeftCoord, subregions.dLength};
     // predict_split: the list of iteration 15 if no region finished before
     // (Workspace.cuh:244-248)

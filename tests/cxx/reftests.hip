@@ -514,6 +514,45 @@ extern "C" int reftest_integrate_split(const char* name, int k, double epsrel, d
   return -1;
 }
 
+// Workspace<double, D, debug, use_custom, collect_mult_runs> with the debug
+// recorders on: the CSV files go to the working directory.  form 0: the
+// Sub_regions& overload at debug 1 with optional "hdr" (4 per axis); form 1:
+// the plain overload at debug 1 with collect_mult_runs (appends); form 2: the
+// Sub_regions& overload at debug 3 (regions and rule points; 2 per axis).
+template <int debug, bool collect>
+int run_debug_header(int form, double epsrel, pagani_result* out)
+{
+  Workspace<double, 2, debug, false, collect> pagani;
+  quad::Volume<double, 2> vol;
+  numint::integration_result r{};
+  if (form == 1) {
+    r = pagani.integrate(XY{}, epsrel, 1e-12, vol);
+  } else {
+    Sub_regions<double, 2> regions(form == 0 ? 4 : 2);
+    r = pagani.integrate(XY{}, regions, epsrel, 1e-12, vol, true, "hdr");
+  }
+  out->estimate = r.estimate;
+  out->errorest = r.errorest;
+  out->nregions = r.nregions;
+  out->status = r.status;
+  out->iters = r.iters;
+  return 0;
+}
+
+extern "C" int reftest_debug_header(int form, double epsrel, pagani_result* out)
+{
+  if (!out) return -1;
+  std::memset(out, 0, sizeof(*out));
+  try {
+    if (form == 0) return run_debug_header<1, false>(form, epsrel, out);
+    if (form == 1) return run_debug_header<1, true>(form, epsrel, out);
+    if (form == 2) return run_debug_header<3, false>(form, epsrel, out);
+  } catch (const std::exception&) {
+    return -2;
+  }
+  return -1;
+}
+
 extern "C" int reftest_predict_split(const char* name, int k, double epsrel, double epsabs,
                                      int relerr, pagani_result* out, std::size_t* snap_n,
                                      std::uint64_t* trace_n, std::uint64_t* trace_m,

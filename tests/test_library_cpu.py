@@ -54,6 +54,26 @@ def test_rule_tables_bit_identical_to_oracle(pg, d):
     assert sb == [int(v) for v in np.concatenate([[0], np.cumsum(cnt)])]
 
 
+@pytest.mark.parametrize("d", [2, 3, 5, 8, 10, 16])
+def test_rule_point_order_is_the_oracles(pg, d):
+    """pagani_rule_point (the debug recorder's point order) lists the
+    oracle's generators: the centre, +-lambda1 and +-lambda2 per axis in the
+    fourth difference's order (Sample.cuh:194-218), then the table."""
+    og, _ = O.generators(d)
+    got = np.array([[pg.rule_point(d, p, j) for j in range(d)] for p in range(len(og))])
+    assert np.array_equal(got, og)
+
+
+def test_rule_point_rejects_bad_arguments(pg):
+    lib = pg.load()
+    g = ctypes.c_double()
+    F = lib.pagani_rule_tables(4, None, None, None, None, None, None)
+    for args in ((1, 0, 0), (17, 0, 0), (4, -1, 0), (4, F, 0), (4, 0, -1), (4, 0, 4)):
+        assert lib.pagani_rule_point(*args, ctypes.byref(g)) == pg.EBADARG, args
+    assert lib.pagani_rule_point(4, 0, 0, None) == pg.EBADARG
+    assert lib.pagani_rule_point(4, F - 1, 3, ctypes.byref(g)) == 0
+
+
 def test_rule_tables_reject_bad_dims(pg):
     lib = pg.load()
     for d in (0, 1, 17, 18):
