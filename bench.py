@@ -88,10 +88,10 @@ def noneval_bytes(trace, d):
     each region's estimate and error (16), its share of the parents'
     estimates (4), writes the refined error (8) and the flag (1); the
     compaction reads every flag (1) and, per kept region, its parent's row
-    of bounds (16d), the parent's halved axis, its own axis (1 + 1) and
-    estimate (8), and writes it as a parent of the next list (16d bounds +
-    1 axis + 8 estimate): 30 n + (32d + 19) m."""
-    return sum(int(n) * 30 + int(m) * (32 * d + 19) for n, m in zip(trace["n"], trace["m"]))
+    (8d lower bounds + 2d width levels), the parent's halved axis, its own
+    axis (1 + 1) and estimate (8), and writes it as a parent of the next
+    list (10d row + 1 axis + 8 estimate): 30 n + (20d + 19) m."""
+    return sum(int(n) * 30 + int(m) * (20 * d + 19) for n, m in zip(trace["n"], trace["m"]))
 
 
 def reference_noneval_bytes(trace, d):
@@ -327,15 +327,15 @@ def config2(args, pg, W):
             "eval_launches": host_launches,
             "avg_launch_ms": st["eval_ms"] / max(host_launches, 1),
             "avg_regions_per_launch": host_regions / max(host_launches, 1),
-            # the host-driven launches evaluate split-form lists: a parent's
-            # row of bounds (16d) serves its two children, plus the halved
-            # axis (1 per parent), and each child writes estimate, error and
-            # axis (17)
+            # the host-driven launches evaluate split-form lists with widths
+            # as levels: a parent's row (8d lower bounds + 2d levels) serves
+            # its two children, plus the halved axis (1 per parent), and each
+            # child writes estimate, error and axis (17)
             "traffic": fc["split_hbm_bytes_per_region"] * host_regions / max(host_launches, 1),
             "traffic_unit": "bytes per launch (counter-measured per-region HBM bytes of the "
                             "headline's split-form launches x average regions per launch)",
-            "algorithmic_bytes_per_launch": (8 * d + 17.5) * host_regions / max(host_launches, 1),
-            "traffic_vs_algorithmic": fc["split_hbm_bytes_per_region"] / (8 * d + 17.5),
+            "algorithmic_bytes_per_launch": (5 * d + 17.5) * host_regions / max(host_launches, 1),
+            "traffic_vs_algorithmic": fc["split_hbm_bytes_per_region"] / (5 * d + 17.5),
         },
     }
     if not args.no_generic:

@@ -235,6 +235,8 @@ _SIGS = {
     "pagani_ws_write_regions_csv": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "pagani_ws_set_snapshot_iter": (ctypes.c_int, [_vp, ctypes.c_uint64]),
     "pagani_ws_set_predict_split": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_device_search": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_ws_set_level_form": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_ws_set_debug": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p,
                                            ctypes.c_int]),
     "pagani_rule_point": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
@@ -540,6 +542,16 @@ class Workspace:
         (off before iteration 15 while nothing finished and the list is at
         most 15M regions, on after), instead of the relerr argument."""
         self._check(self.lib.pagani_ws_set_predict_split(self._ws, 1 if on else 0))
+
+    def set_device_search(self, on=True):
+        """Run the classifier's threshold search on the device (default) or,
+        with on=False, on the host (one probe pass per round trip)."""
+        self._check(self.lib.pagani_ws_set_device_search(self._ws, 1 if on else 0))
+
+    def set_level_form(self, on=True):
+        """Keep the uniform start's widths as levels (two bytes per axis, the
+        default) or, with on=False, as doubles; the results are the same."""
+        self._check(self.lib.pagani_ws_set_level_form(self._ws, 1 if on else 0))
 
     def set_debug(self, level, directory=".", id="default", append=False):
         """The reference's debug recorders for the following integrate calls:

@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cinttypes>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -145,6 +146,27 @@ int write_device_array(pagani_ws* ws, Writer& w, const T* d, std::size_t count, 
   return w.ok ? PAGANI_OK : fail(ws, PAGANI_EBADARG, "checkpoint write failed");
 }
 
+// The widths of a list held as levels (pagani_eval_args), written as the
+// doubles a list with double widths holds: width = h0[j] * 2^-level, row-major
+// per axis like the levels.
+int write_level_widths(pagani_ws* ws, Writer& w, const std::uint16_t* d, std::size_t rows,
+                       std::size_t ndim, const pagani::phases::Levels& lv, Stage& st)
+{
+  const std::size_t per = kStageBytes / sizeof(double);
+  std::vector<double> wd(per);
+  for (std::size_t j = 0; j < ndim && w.ok; ++j)
+    for (std::size_t i = 0; i < rows && w.ok; i += per) {
+      const std::size_t c = std::min(per, rows - i);
+      HIP_TRY(ws, hipMemcpyAsync(st.p, d + j * rows + i, c * sizeof(std::uint16_t),
+                                 hipMemcpyDeviceToHost, ws->stream));
+      HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+      const std::uint16_t* lev = static_cast<const std::uint16_t*>(st.p);
+      for (std::size_t k = 0; k < c; ++k) wd[k] = std::ldexp(lv.h0[j], -static_cast<int>(lev[k]));
+      w.raw(wd.data(), c * sizeof(double));
+    }
+  return w.ok ? PAGANI_OK : fail(ws, PAGANI_EBADARG, "checkpoint write failed");
+}
+
 template <class T>
 int read_device_array(pagani_ws* ws, Reader& r, T* d, std::size_t count, Stage& st)
 {
@@ -228,8 +250,13 @@ int pagani_ws_save_state(pagani_ws* ws, const char* path)
     return fail(ws, PAGANI_ENOMEM, "pinned staging buffer");
   }
   int rc;
+  // widths held as levels go to the file as doubles: a loaded run uses
+  // double widths
   if ((rc = write_device_array(ws, w, ws->L[ws->cur].p, d * rows, st)) ||
-      (rc = write_device_array(ws, w, ws->H[ws->cur].p, d * rows, st)) ||
+      (rc = s.lv.on ? write_level_widths(ws, w, reinterpret_cast<const std::uint16_t*>(
+                                                    ws->H[ws->cur].p),
+                                         rows, d, s.lv, st)
+                    : write_device_array(ws, w, ws->H[ws->cur].p, d * rows, st)) ||
       (s.split && (rc = write_device_array(ws, w, ws->pdim[ws->cur].p, rows, st))) ||
       (rc = write_device_array(ws, w, ws->pest.p, np, st)))
     return rc;
@@ -351,6 +378,7 @@ int pagani_ws_load_state(pagani_ws* ws, const char* path)
   ws->live_n = s.n;
   ws->live_evaluated = false;
   ws->split = s.split != 0;
+  ws->lv = pagani::phases::Levels{};  // the file holds double widths
   return PAGANI_OK;
 }
 
@@ -372,8 +400,17 @@ int pagani_ws_write_regions_csv(pagani_ws* ws, const char* path)
   if (n) {
     HIP_TRY(ws, hipMemcpy(L.data(), ws->L[ws->cur].p, rows * d * sizeof(double),
                           hipMemcpyDeviceToHost));
-    HIP_TRY(ws, hipMemcpy(H.data(), ws->H[ws->cur].p, rows * d * sizeof(double),
-                          hipMemcpyDeviceToHost));
+    if (ws->lv.on) {
+      std::vector<std::uint16_t> lev(rows * d);
+      HIP_TRY(ws, hipMemcpy(lev.data(), ws->H[ws->cur].p, rows * d * sizeof(std::uint16_t),
+                            hipMemcpyDeviceToHost));
+      for (std::size_t j = 0; j < d; ++j)
+        for (std::size_t i = 0; i < rows; ++i)
+          H[j * rows + i] = std::ldexp(ws->lv.h0[j], -static_cast<int>(lev[j * rows + i]));
+    } else {
+      HIP_TRY(ws, hipMemcpy(H.data(), ws->H[ws->cur].p, rows * d * sizeof(double),
+                            hipMemcpyDeviceToHost));
+    }
     if (ws->split)
       HIP_TRY(ws, hipMemcpy(pdim.data(), ws->pdim[ws->cur].p, rows, hipMemcpyDeviceToHost));
     if (ws->live_evaluated) {

@@ -23,6 +23,7 @@
 #include "pagani/eval_kernel.hpp"
 #include "pagani_ext.h"
 #include "phases.hpp"
+#include "classify_search.hpp"
 #include "rule.hpp"
 
 #include "ws_internal.hpp"
@@ -93,10 +94,14 @@ int choose_slices(const pagani_ws* ws, std::size_t n)
 void set_slices(pagani_eval_consts& C, int S);
 
 void make_consts(const pagani_ws* ws, const double* lows, const double* highs, std::size_t n,
-                 pagani_eval_consts& C)
+                 pagani_eval_consts& C, const ph::Levels* lv = nullptr)
 {
   std::memset(&C, 0, sizeof(C));
   C.abi_version = PAGANI_EVAL_ABI_VERSION;
+  if (lv && lv->on) {
+    C.levels = 1;
+    for (int j = 0; j < PAGANI_MAX_DIM; ++j) C.h0[j] = lv->h0[j];
+  }
   const int d = ws->ndim;
   const pagani::RuleTables& R = ws->rule;
   std::memcpy(C.w, R.w, sizeof(C.w));
@@ -138,179 +143,106 @@ void set_slices(pagani_eval_consts& C, int S)
 }
 
 // ---------------------------------------------------------------------------
-// Heuristic classifier, heuristic_classifier.cuh:89-452 (host logic; the
-// threshold passes run on the device).
+// Heuristic classifier, heuristic_classifier.cuh:89-452.  The threshold
+// search (classify_search.hpp) runs on the device: the host queues rounds of
+// probe passes and replays and reads the outcome once; a search too long for
+// the device's cache is finished here with the same replay.
 
-
-// The classifier's threshold search (heuristic_classifier.cuh:282-451)
-// needs, per candidate threshold t, the number of errors above t and their
-// sum.  The search is a walk over states (t, low, high) with two moves:
-//   - raise, when t fails the memory test: low = t, t += |high - t| / 2
-//     (get_larger_threshold_results, :327-346);
-//   - lower, when t passes it but not the error budget: high = t,
-//     t -= |low - t| / 2 (:404-410).
-// When a candidate has not been measured yet, one pass measures it together
-// with every state reachable from it in kTreeDepth moves (a full binary
-// tree, breadth first).  The search is then replayed on the host with the
-// reference's arithmetic, reading measured values from the cache; a
-// candidate outside it costs another pass.  Decisions are those of the
-// sequential search.
-constexpr int kRaiseLimit = 20;  // :335
-constexpr int kTreeDepth = 3;
-constexpr int kTreeNodes = (2 << kTreeDepth) - 1;
-static_assert(kTreeNodes < ph::kMaxThresholds, "one pass measures the whole tree");
-
-struct ThresholdCache {
+// A state measured on the host: one probe pass over the tree around it.
+struct HostCache {
   std::vector<double> t, dot;
   std::vector<std::uint64_t> cnt;
-  bool find(double x, std::uint64_t& c, double& d) const
-  {
-    for (std::size_t i = 0; i < t.size(); ++i)
-      if (t[i] == x) {
-        c = cnt[i];
-        d = dot[i];
-        return true;
-      }
-    return false;
-  }
 };
 
-int measure_threshold(pagani_ws* ws, const double* err, std::size_t n, double t, double low,
-                      double high, ThresholdCache& cache, std::uint64_t& c, double& d)
+int host_probe(pagani_ws* ws, const double* err, std::size_t n, double t, double low,
+               double high, HostCache& cache)
 {
-  if (cache.find(t, c, d)) return PAGANI_OK;
-  ph::ThresholdSet T;
-  double lo[kTreeNodes], hi[kTreeNodes];
-  int k = 0;
-  T.t[k] = t;
-  lo[k] = low;
-  hi[k] = high;
-  ++k;
-  for (int i = 0; k < kTreeNodes; ++i) {
-    const double x = T.t[i];
-    T.t[k] = x + std::fabs(hi[i] - x) * .5;  // raise
-    lo[k] = x;
-    hi[k] = hi[i];
-    ++k;
-    T.t[k] = x - std::fabs(lo[i] - x) * .5;  // lower
-    lo[k] = lo[i];
-    hi[k] = x;
-    ++k;
-  }
-  T.k = k;
-  // the probe takes its candidates in ascending order; a NaN candidate
-  // (err > NaN holds for no error) is measured as +inf, which counts the same
-  int perm[ph::kMaxThresholds];
-  for (int i = 0; i < k; ++i) perm[i] = i;
-  auto key = [&](int i) { return std::isnan(T.t[i]) ? HUGE_VAL : T.t[i]; };
-  std::sort(perm, perm + k, [&](int a, int b) { return key(a) < key(b) || (key(a) == key(b) && a < b); });
+  double node[kTreeNodes];
+  int perm[kTreeNodes];
   ph::ThresholdSet S;
-  S.k = k;
-  for (int i = 0; i < k; ++i) S.t[i] = key(perm[i]);
+  search_tree(t, low, high, node, S, perm);
   HIP_TRY(ws, ph::multi_threshold(ws->stream, n, err, S, ws->d_mpart, ws->d_mout));
-  HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * k * sizeof(double),
+  HIP_TRY(ws, hipMemcpyAsync(ws->h_mout, ws->d_mout, 2 * S.k * sizeof(double),
                              hipMemcpyDeviceToHost, ws->stream));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));
   ws->stats.classify_passes++;
-  for (int i = 0; i < k; ++i) {
-    const int o = perm[i];  // sorted slot i holds candidate o
-    cache.t.push_back(T.t[o]);
+  for (int i = 0; i < S.k; ++i) {
+    cache.t.push_back(node[perm[i]]);
     cache.cnt.push_back(static_cast<std::uint64_t>(ws->h_mout[i]));
-    cache.dot.push_back(ws->h_mout[k + i]);
-    if (o == 0) {
-      c = static_cast<std::uint64_t>(ws->h_mout[i]);
-      d = ws->h_mout[k + i];
-    }
+    cache.dot.push_back(ws->h_mout[S.k + i]);
   }
   return PAGANI_OK;
 }
 
-// get_larger_threshold_results (:327-346) with apply_threshold (:282-303).
-// `last_dot` / `last_thr` are the sum of err > t and t of the last applied
-// threshold.
-int larger_threshold(pagani_ws* ws, const Classifier& c, const double* err, std::size_t n,
-                     ThresholdCache& cache, pagani_class_res& res, double& last_dot,
-                     double& last_thr)
-{
-  res.pass_mem = 0;
-  for (int k = 0; k < kRaiseLimit && !res.pass_mem; ++k) {
-    std::uint64_t cnt = 0;
-    double dot = 0.0;
-    const int rc = measure_threshold(ws, err, n, res.threshold, res.range_low, res.range_high,
-                                     cache, cnt, dot);
-    if (rc) return rc;
-    last_dot = dot;
-    last_thr = res.threshold;
-    res.threshold_passes++;
-    res.num_active = cnt;
-    res.percent_mem_active = static_cast<double>(static_cast<int>(cnt)) /
-                             static_cast<double>(static_cast<int>(n));
-    res.pass_mem = res.percent_mem_active <= c.max_active;
-    if (!res.pass_mem) {
-      res.range_low = res.threshold;
-      const double diff = std::fabs(res.range_high - res.threshold);
-      res.threshold += diff * .5;
-    }
-  }
-  return PAGANI_OK;
-}
-
-// classify, :362-451 (with evaluate_error_budget :305-325).  The flags of
-// the last applied threshold (*last_thr) are written by the caller with
-// k_threshold_flags when it needs them.
+// classify, :362-451.  The flags of the accepted threshold (*last_thr) are
+// written by the caller.
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
              double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr)
 {
   std::memset(&res, 0, sizeof(res));
   int rc = ensure_chunks(ws, n);
   if (rc) return rc;
+  SearchInputs in{};
+  in.n = n;
+  in.iter_err = iter_err;
+  in.iter_fin_err = iter_fin_err;
+  in.tot_fin_err = tot_fin_err;
+  in.target_error = std::fabs(c.hist[2]) * c.epsrel;
+  in.max_budget = c.max_budget;
+  in.max_active = c.max_active;
+  // min and max of the errors
   HIP_TRY(ws, ph::minmax(ws->stream, n, err, ws->part.p));
-  rc = finalize_read(ws, n, false, true);
-  if (rc) return rc;
-  const double mn = ws->h_scal->mn, mx = ws->h_scal->mx;
-  res.range_low = mn;
-  res.range_high = mx;
-  res.threshold = iter_err / static_cast<double>(n);
-  const double target_error = std::fabs(c.hist[2]) * c.epsrel;
-  std::size_t n_inc = 0, n_dec = 0;
-  double last_dot = 0.;
-  *last_thr = res.threshold;
-  ThresholdCache cache;
-  do {
-    if (!res.pass_mem && n_inc <= 20) {
-      rc = larger_threshold(ws, c, err, n, cache, res, last_dot, *last_thr);
-      if (rc) return rc;
-      n_inc++;
+  HIP_TRY(ws, launch_finalize(ws, n, false, true));
+  SearchStep step{};
+  HostCache cache;
+  if (ws->device_search) {
+    HIP_TRY(ws, ph::search_start(ws->stream, ws->d_search, ws->d_scal, in));
+    for (;;) {
+      for (int r = 0; r < kSearchRounds; ++r)
+        HIP_TRY(ws, ph::search_round(ws->stream, n, err, ws->d_search, ws->d_mpart, ws->d_mout));
+      HIP_TRY(ws, hipMemcpyAsync(ws->h_search, ws->d_search, sizeof(SearchState),
+                                 hipMemcpyDeviceToHost, ws->stream));
+      HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+      step = ws->h_search->step;
+      if (step.done || step.overflow) break;
     }
-    if (res.pass_mem) {
-      const double extra = iter_err - last_dot - iter_fin_err;
-      const double budget = target_error - tot_fin_err;
-      res.pass_errorest_budget = extra <= c.max_budget * budget;
-      res.finished_errorest = extra;
-      if (!res.pass_errorest_budget && n_dec <= 20) {
-        res.range_high = res.threshold;
-        const double diff = std::fabs(res.range_low - res.threshold);
-        res.threshold -= diff * .5;
-        res.pass_mem = 0;
-        n_dec++;
+    ws->stats.classify_passes += ws->h_search->passes;
+    if (!step.done) {
+      // finish on the host, from the start, with the device's measurements
+      in = ws->h_search->in;
+      const SearchState& hs = *ws->h_search;
+      for (int i = 0; i < hs.len; ++i) {
+        cache.t.push_back(hs.ct[i]);
+        cache.cnt.push_back(hs.ccnt[i]);
+        cache.dot.push_back(hs.cdot[i]);
       }
     }
-    const bool exhausted = n_dec >= 20 || n_inc >= 20;
-    if (exhausted && c.max_budget < .7) {
-      c.max_budget += 0.1;
-      n_dec = 0;
-      n_inc = 0;
-      res.range_low = mn;
-      res.range_high = mx;
-      res.threshold = iter_err / static_cast<double>(n);
-    } else if (exhausted && c.max_budget >= .7 && c.max_active <= .7) {
-      c.max_active += .1;
-    } else if (exhausted) {
-      break;
+  } else {
+    HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
+                               hipMemcpyDeviceToHost, ws->stream));
+    HIP_TRY(ws, hipStreamSynchronize(ws->stream));
+    in.mn = ws->h_scal->mn;
+    in.mx = ws->h_scal->mx;
+  }
+  if (!step.done) {
+    for (;;) {
+      auto find = [&](double x, std::uint64_t& cn, double& d) {
+        for (std::size_t i = 0; i < cache.t.size(); ++i)
+          if (cache.t[i] == x) {
+            cn = cache.cnt[i];
+            d = cache.dot[i];
+            return true;
+          }
+        return false;
+      };
+      search_replay(in, find, step);
+      if (step.done) break;
+      if (step.overflow) return fail(ws, PAGANI_EHIP, "classifier search did not end");
+      if ((rc = host_probe(ws, err, n, step.t, step.low, step.high, cache))) return rc;
     }
-  } while (!res.pass_mem || !res.pass_errorest_budget);
-  res.max_budget_perc_to_cover = c.max_budget;
-  res.max_active_perc = c.max_active;
+  }
+  res = step.res;
+  *last_thr = step.last_thr;
   c.max_active = .5;
   c.max_budget = .25;
   return PAGANI_OK;
@@ -499,7 +431,7 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
   args.dim = ws->dim.p;
   args.stream = ws->stream;
   args.dev = &ws->d_loop->it;
-  make_consts(ws, lows, highs, cap, args.consts);
+  make_consts(ws, lows, highs, cap, args.consts, &ws->lv);
   RoundGraphs graphs;
 
   for (int queued = 2;; queued = std::min(queued * 2, kDevQueueMax)) {
@@ -533,7 +465,7 @@ int run_device_segment(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double eps
                                     ws->H[0].p, ws->L[1].p, ws->H[1].p, ws->pdim[0].p,
                                     ws->pdim[1].p, ws->active.p, ws->est.p, ws->err0.p,
                                     ws->err1.p, ws->dim.p, ws->off.p, ws->foff.p, ws->pest.p,
-                                    nullptr));
+                                    nullptr, ws->lv));
       }
       return PAGANI_OK;
     };
@@ -658,8 +590,10 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
   if (resume) {
     n = st.n;
     ws->split = st.split != 0;
+    ws->lv = st.lv;
   } else if (init) {
     ws->split = false;
+    ws->lv = ph::Levels{};
     n = init->n;
     if (n == 0 || !init->L || !init->H) return fail(ws, PAGANI_EBADARG, "empty region list");
     if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
@@ -675,6 +609,11 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     ws->cur = 0;
     ws->split = false;
     std::size_t k = init_k ? init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
+    // every width of the run is 1/k halved some number of times: H holds
+    // that number, one uint16_t per axis and region (pagani_eval_args)
+    ws->lv = ph::Levels{};
+    ws->lv.on = ws->level_form ? 1 : 0;
+    for (int j = 0; j < d; ++j) ws->lv.h0[j] = 1. / static_cast<double>(k);
     n = 1;
     for (int j = 0; j < d; ++j) {
       if (n > (std::size_t(1) << 40) / k) return fail(ws, PAGANI_EBADARG, "initial split too large");
@@ -682,7 +621,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
     }
     if ((rc = ensure(ws, ws->L[ws->cur], n * d, "region bounds"))) return rc;
     if ((rc = ensure(ws, ws->H[ws->cur], n * d, "region bounds"))) return rc;
-    HIP_TRY(ws, ph::uniform_split(ws->stream, d, k, n, ws->L[ws->cur].p, ws->H[ws->cur].p));
+    HIP_TRY(ws, ph::uniform_split(ws->stream, d, k, n, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                                  ws->lv.on != 0));
   }
 
   const std::size_t total = st.mem_total;
@@ -714,6 +654,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       st.n = n;
       st.has_parents = has_parents ? 1 : 0;
       st.split = ws->split ? 1 : 0;
+      st.lv = ws->lv;
       st.cum = cum;
       st.hist[0] = clf.hist[0];
       st.hist[1] = clf.hist[1];
@@ -767,7 +708,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       args.err = ws->err0.p;
       args.dim = ws->dim.p;
       args.stream = ws->stream;
-      make_consts(ws, lows, highs, n, args.consts);
+      make_consts(ws, lows, highs, n, args.consts, &ws->lv);
       if (rec.points(n)) {
         if ((rc = ensure(ws, ws->dfe, n * static_cast<std::size_t>(ws->rule.feval),
                          "point values")))
@@ -782,7 +723,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       if (rec.level() > 0) {
         rec.phase(it, "apply_cubature_rules");
         HIP_TRY(ws, rec.regions(it, n, ws->est.p, ws->err0.p));
-        HIP_TRY(ws, rec.points(n, args.L, args.H, args.pdim, ws->est.p, ws->err0.p, args.fevals));
+        HIP_TRY(ws, rec.points(n, args.L, args.H, args.pdim, ws->lv, ws->est.p, ws->err0.p,
+                               args.fevals));
         rec.mark();
       }
       ws->stats.regions_evaluated += n;
@@ -802,7 +744,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
                              has_parents ? ws->pest.p : nullptr, epsrel, relerr_it, ws->active.p,
                              ws->part.p, ws->cnt.p));
-      HIP_TRY(ws, launch_finalize(ws, n, true, false));
+      HIP_TRY(ws, launch_finalize(ws, n, true, false, has_parents));
       record(ws, EV_REF1);
     }
     errs = has_parents ? ws->err1.p : ws->err0.p;
@@ -833,9 +775,10 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       if ((rc = ensure(ws, ws->snapL, n * d, "snapshot")) ||
           (rc = ensure(ws, ws->snapH, n * d, "snapshot")))
         return rc;
-      if (ws->split) {
-        HIP_TRY(ws, ph::split_all(ws->stream, d, n / 2, ws->L[ws->cur].p, ws->H[ws->cur].p,
-                                  ws->pdim[ws->cur].p, ws->snapL.p, ws->snapH.p));
+      if (ws->split || ws->lv.on) {
+        HIP_TRY(ws, ph::expand(ws->stream, d, n, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                               ws->split ? ws->pdim[ws->cur].p : nullptr, ws->lv, ws->snapL.p,
+                               ws->snapH.p));
       } else {
         HIP_TRY(ws, hipMemcpyAsync(ws->snapL.p, ws->L[ws->cur].p, n * d * sizeof(double),
                                    hipMemcpyDeviceToDevice, ws->stream));
@@ -889,8 +832,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
           // the accepted threshold's flags replace the active flags
           // (Workspace.cuh:104-111), with their counts for the compaction
           HIP_TRY(ws, ph::threshold_flags(ws->stream, n, ws->est.p, errs, thr, ws->flags.p,
-                                          ws->part.p, ws->cnt.p));
-          if ((rc = finalize_read(ws, n, true, false))) return rc;
+                                          ws->part.p, ws->cnt.p, has_parents));
+          if ((rc = finalize_read(ws, n, true, false, has_parents))) return rc;
           if (ws->h_scal->count != res.num_active)
             return fail(ws, PAGANI_EHIP, "classifier flag count disagrees with its pass");
           std::swap(ws->active, ws->flags);
@@ -909,8 +852,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       }
       if (recount) {
         HIP_TRY(ws, ph::count_flags(ws->stream, n, ws->active.p, ws->est.p, errs, ws->part.p,
-                                    ws->cnt.p));
-        if ((rc = finalize_read(ws, n, true, false))) return rc;
+                                    ws->cnt.p, has_parents));
+        if ((rc = finalize_read(ws, n, true, false, has_parents))) return rc;
         m = ws->h_scal->count;
       }
       rec.phase(it, "heuristic_classify");
@@ -953,7 +896,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
                                       ws->split ? ws->pdim[ws->cur].p : nullptr, ws->active.p,
                                       ws->est.p, errs, ws->dim.p, ws->off.p, ws->foff.p,
                                       ws->L[nxt].p, ws->H[nxt].p, ws->pdim[nxt].p, ws->pest.p,
-                                      nullptr));
+                                      nullptr, has_parents, ws->lv));
       record(ws, EV_CMP1);
       ws->compact_pending = ws->timing;
       ws->cur = nxt;
@@ -1103,6 +1046,10 @@ pagani_ws* pagani_ws_create(int ndim, int device)
   if ((e = hipHostMalloc(&ws->h_mout, 2 * ph::kMaxThresholds * sizeof(double),
                          hipHostMallocDefault)) != hipSuccess)
     return bail(ws, "hipHostMalloc(threshold results)", e);
+  if ((e = hipMalloc(&ws->d_search, sizeof(SearchState))) != hipSuccess)
+    return bail(ws, "hipMalloc(classifier search)", e);
+  if ((e = hipHostMalloc(&ws->h_search, sizeof(SearchState), hipHostMallocDefault)) != hipSuccess)
+    return bail(ws, "hipHostMalloc(classifier search)", e);
   if ((e = hipMalloc(&ws->d_loop, sizeof(DevLoop))) != hipSuccess)
     return bail(ws, "hipMalloc(device loop state)", e);
   if ((e = hipHostMalloc(&ws->h_loop, sizeof(DevLoop), hipHostMallocDefault)) != hipSuccess)
@@ -1120,6 +1067,8 @@ void pagani_ws_destroy(pagani_ws* ws)
   if (ws->d_mpart) (void)hipFree(ws->d_mpart);
   if (ws->d_mout) (void)hipFree(ws->d_mout);
   if (ws->h_mout) (void)hipHostFree(ws->h_mout);
+  if (ws->d_search) (void)hipFree(ws->d_search);
+  if (ws->h_search) (void)hipHostFree(ws->h_search);
   if (ws->d_loop) (void)hipFree(ws->d_loop);
   if (ws->h_loop) (void)hipHostFree(ws->h_loop);
   if (ws->d_gtab) (void)hipFree(ws->d_gtab);
@@ -1222,20 +1171,26 @@ size_t pagani_ws_trace(const pagani_ws* ws, size_t cap, uint64_t* n, uint64_t* m
 int pagani_ws_live_regions(pagani_ws* ws, size_t* n, const double** L, const double** H)
 {
   if (!ws || !n || !L || !H) return PAGANI_EBADARG;
-  if (ws->split && ws->live_n > 0) {
-    // write the list out flat into the other pair, which then holds it
+  if ((ws->split || ws->lv.on) && ws->live_n > 0) {
+    // write the list out flat, widths as doubles, into the other pair, which
+    // then holds it
     if (hipSetDevice(ws->device) != hipSuccess) return fail(ws, PAGANI_EHIP, "hipSetDevice");
     const int d = ws->ndim, nxt = 1 - ws->cur;
     int rc;
     if ((rc = ensure(ws, ws->L[nxt], ws->live_n * d, "region bounds")) ||
         (rc = ensure(ws, ws->H[nxt], ws->live_n * d, "region bounds")))
       return rc;
-    HIP_TRY(ws, ph::split_all(ws->stream, d, ws->live_n / 2, ws->L[ws->cur].p, ws->H[ws->cur].p,
-                              ws->pdim[ws->cur].p, ws->L[nxt].p, ws->H[nxt].p));
+    HIP_TRY(ws, ph::expand(ws->stream, d, ws->live_n, ws->L[ws->cur].p, ws->H[ws->cur].p,
+                           ws->split ? ws->pdim[ws->cur].p : nullptr, ws->lv, ws->L[nxt].p,
+                           ws->H[nxt].p));
     HIP_TRY(ws, hipStreamSynchronize(ws->stream));
     ws->cur = nxt;
     ws->split = false;
-    if (ws->susp.held) ws->susp.split = 0;
+    ws->lv = ph::Levels{};
+    if (ws->susp.held) {
+      ws->susp.split = 0;
+      ws->susp.lv = ph::Levels{};
+    }
   }
   *n = ws->live_n;
   *L = ws->L[ws->cur].p;
@@ -1348,6 +1303,20 @@ int pagani_ws_set_predict_split(pagani_ws* ws, int enable)
   return PAGANI_OK;
 }
 
+int pagani_ws_set_level_form(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->level_form = enable != 0;
+  return PAGANI_OK;
+}
+
+int pagani_ws_set_device_search(pagani_ws* ws, int enable)
+{
+  if (!ws) return PAGANI_EBADARG;
+  ws->device_search = enable != 0;
+  return PAGANI_OK;
+}
+
 int pagani_ws_set_debug(pagani_ws* ws, int level, const char* dir, const char* id, int append)
 {
   if (!ws || level < 0 || level > 3) return PAGANI_EBADARG;
@@ -1434,7 +1403,7 @@ int pagani_refine_dev(pagani_ws* ws, size_t n, const double* est, const double* 
   if (rc) return rc;
   HIP_TRY(ws, ph::refine(ws->stream, n, est, err_in, err_out, parent_est, epsrel,
                          relerr_classification, active, ws->part.p, ws->cnt.p));
-  if ((rc = finalize_read(ws, n, true, false))) return rc;
+  if ((rc = finalize_read(ws, n, true, false, parent_est != nullptr))) return rc;
   if (sums)
     for (int i = 0; i < 4; ++i) sums[i] = ws->h_scal->sum[i];
   if (num_active) *num_active = ws->h_scal->count;
@@ -1452,7 +1421,7 @@ int pagani_filter_dev(pagani_ws* ws, size_t n, const double* L, const double* H,
   if (n == 0) return PAGANI_OK;
   int rc = ensure_chunks(ws, n);
   if (rc) return rc;
-  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p));
+  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p, false));
   if ((rc = finalize_read(ws, n, true, false))) return rc;
   const std::size_t m = ws->h_scal->count;
   *m_out = m;
@@ -1483,7 +1452,7 @@ int pagani_filter_split_dev(pagani_ws* ws, size_t n, const double* L, const doub
   if (n == 0) return PAGANI_OK;
   int rc = ensure_chunks(ws, n);
   if (rc) return rc;
-  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p));
+  HIP_TRY(ws, ph::count_flags(ws->stream, n, active, est, err, ws->part.p, ws->cnt.p, false));
   if ((rc = finalize_read(ws, n, true, false))) return rc;
   const std::size_t m = ws->h_scal->count;
   *m_out = m;

@@ -29,7 +29,9 @@ namespace {
 
 namespace ph = pagani::phases;
 
-constexpr std::size_t kMaxChunks = PAGANI_NOALLOC_MAX_REGIONS / ph::kChunk;
+// one chunk more than a flat list of the maximum size: a list with parents
+// is cut at its half (ph::num_chunks(n, true))
+constexpr std::size_t kMaxChunks = PAGANI_NOALLOC_MAX_REGIONS / ph::kChunk + 1;
 constexpr std::size_t kMaxFolds = (kMaxChunks + ph::kFold - 1) / ph::kFold;
 static_assert(PAGANI_NOALLOC_MAX_REGIONS % ph::kChunk == 0, "whole chunks");
 
@@ -108,8 +110,8 @@ extern "C" int pagani_refine_dev_noalloc(pagani_ws* ws, size_t n, const double* 
   ph::Scalars* scal = reinterpret_cast<ph::Scalars*>(base + kOffScal);
   HIP_TRY(ws, ph::refine(ws->stream, n, est, err_in, err_out, parent_est, epsrel,
                          relerr_classification, active, part, cnt));
-  HIP_TRY(ws, ph::finalize(ws->stream, ph::num_chunks(n), part, cnt, off, fpart, fcnt, foff, scal,
-                           0));
+  HIP_TRY(ws, ph::finalize(ws->stream, ph::num_chunks(n, parent_est != nullptr), part, cnt, off,
+                           fpart, fcnt, foff, scal, 0));
   if (sums_dev)
     HIP_TRY(ws, hipMemcpyAsync(sums_dev, scal->sum, 4 * sizeof(double), hipMemcpyDeviceToDevice,
                                ws->stream));

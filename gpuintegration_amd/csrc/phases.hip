@@ -8,6 +8,7 @@
 #include <cmath>
 #include <type_traits>
 
+#include "classify_search.hpp"
 #include "devloop_decide.hpp"
 
 namespace pagani {
@@ -102,7 +103,8 @@ __device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::ui
 }
 
 __global__ void __launch_bounds__(256) k_uniform_split(int d, std::size_t k, std::size_t n,
-                                                       double len, double* L, double* H)
+                                                       double len, double* L, double* H,
+                                                       int levels)
 {
   const std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -111,7 +113,26 @@ __global__ void __launch_bounds__(256) k_uniform_split(int d, std::size_t k, std
     const std::size_t id = q % k;
     q /= k;
     L[static_cast<std::size_t>(j) * n + t] = static_cast<double>(id) * len;
-    H[static_cast<std::size_t>(j) * n + t] = len;
+    if (levels)
+      reinterpret_cast<std::uint16_t*>(H)[static_cast<std::size_t>(j) * n + t] = 0;
+    else
+      H[static_cast<std::size_t>(j) * n + t] = len;
+  }
+}
+
+// Row `row` of a list's bounds along axis j at stride `stride`: lower bound
+// and width (from the level when lv.on), and the width's level.
+__device__ __forceinline__ void read_row(const double* __restrict__ L,
+                                         const double* __restrict__ H, const Levels& lv,
+                                         std::size_t o, int j, double& l, double& w, int& lev)
+{
+  l = L[o];
+  if (lv.on) {
+    lev = reinterpret_cast<const std::uint16_t*>(H)[o];
+    w = __builtin_ldexp(lv.h0[j], -lev);
+  } else {
+    lev = 0;
+    w = H[o];
   }
 }
 
@@ -125,14 +146,15 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
                                              std::uint32_t* __restrict__ cnt,
                                              std::size_t chunk = blockIdx.x)
 {
-  const std::size_t base = chunk * kChunk;
+  std::size_t base, end;
+  chunk_range(chunk, n, pest != nullptr, base, end);
   const std::size_t h = n / 2;
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
-    if (t < n) {
+    if (t < end) {
       const double e = est[t];
       double r = err_in[t];
       bool a = true;
@@ -166,7 +188,8 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
   write_record(se, sr, sae, sar, c, part, cnt, chunk);
 }
 
-// A region's sibling lies n/2 away: workgroups are paired so that a chunk
+// A region's sibling lies n/2 away, in chunk c + L of a paired list's
+// chunks (num_chunks(n, true) = 2L): workgroups are paired so that a chunk
 // and its sibling chunk are read by workgroups running at the same time on
 // the same XCD (workgroups are dealt to the 8 XCDs round robin, so b and
 // b + 8 share one), and the second read of the siblings and their parent
@@ -197,15 +220,16 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_count_flags(std::size_t n, const std::uint8_t* __restrict__ flags,
                   const double* __restrict__ est, const double* __restrict__ err,
-                  double* __restrict__ part, std::uint32_t* __restrict__ cnt)
+                  double* __restrict__ part, std::uint32_t* __restrict__ cnt, int paired)
 {
-  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  std::size_t base, end;
+  chunk_range(blockIdx.x, n, paired != 0, base, end);
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
-    if (t < n) {
+    if (t < end) {
       const double e = est[t], r = err[t];
       se = dd_add(se, e);
       sr = dd_add(sr, r);
@@ -226,15 +250,16 @@ __global__ void __launch_bounds__(kThreads)
     k_threshold_flags(std::size_t n, const double* __restrict__ est,
                       const double* __restrict__ err, double thr,
                       std::uint8_t* __restrict__ flags, double* __restrict__ part,
-                      std::uint32_t* __restrict__ cnt)
+                      std::uint32_t* __restrict__ cnt, int paired)
 {
-  const std::size_t base = static_cast<std::size_t>(blockIdx.x) * kChunk;
+  std::size_t base, end;
+  chunk_range(blockIdx.x, n, paired != 0, base, end);
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
 #pragma unroll 4
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
-    if (t < n) {
+    if (t < end) {
       const double e = est[t], r = err[t];
       const bool a = r > thr;
       flags[t] = a ? 1 : 0;
@@ -262,9 +287,9 @@ __global__ void __launch_bounds__(kThreads)
 // k..K-2.  Fixed grid, grid-stride order, per-thread sums in element order,
 // fixed reduction trees: deterministic.
 template <int K>
-__global__ void __launch_bounds__(kThreads)
-    k_multi_threshold(std::size_t n, const double* __restrict__ err, const ThresholdSet T,
-                      double* __restrict__ bpart)
+__device__ __forceinline__ void multi_threshold_block(std::size_t n, const double* __restrict__ err,
+                                                      const ThresholdSet& T,
+                                                      double* __restrict__ bpart)
 {
   static_assert((K & (K - 1)) == 0, "binary search over a power of two");
   __shared__ double ts[K];
@@ -324,12 +349,32 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+template <int K>
+__global__ void __launch_bounds__(kThreads)
+    k_multi_threshold(std::size_t n, const double* __restrict__ err, const ThresholdSet T,
+                      double* __restrict__ bpart)
+{
+  multi_threshold_block<K>(n, err, T, bpart);
+}
+
+// The pass the device search queued (nothing when it queued none).
+template <int K>
+__global__ void __launch_bounds__(kThreads)
+    k_multi_threshold_dev(std::size_t n, const double* __restrict__ err,
+                          const internal::SearchState* __restrict__ st, double* __restrict__ bpart)
+{
+  if (!st->pending) return;
+  // the candidates are read where they lie (a local copy of the set would
+  // go to scratch memory, thread by thread)
+  multi_threshold_block<K>(n, err, st->S, bpart);
+}
+
 // Fold the kMultiBlocks partials of k_multi_threshold: one wave per
 // threshold (workgroup k), lane j folds blocks j, j+64, j+128, ... in order,
 // then a fixed butterfly over the wave.
-__global__ void __launch_bounds__(64)
-    k_multi_reduce(int nblocks, int nthr, const double* __restrict__ bpart,
-                   double* __restrict__ out)
+__device__ __forceinline__ void multi_reduce_wave(int nblocks, int nthr,
+                                                  const double* __restrict__ bpart,
+                                                  double* __restrict__ out)
 {
   const int k = blockIdx.x, j = threadIdx.x;
   DD s{0.0, 0.0};
@@ -350,6 +395,109 @@ __global__ void __launch_bounds__(64)
   if (j == 0) {
     out[k] = c;
     out[nthr + k] = s.hi + s.lo;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_multi_reduce(int nblocks, int nthr, const double* __restrict__ bpart,
+                   double* __restrict__ out)
+{
+  multi_reduce_wave(nblocks, nthr, bpart, out);
+}
+
+__global__ void __launch_bounds__(64)
+    k_multi_reduce_dev(int nblocks, const internal::SearchState* __restrict__ st,
+                       const double* __restrict__ bpart, double* __restrict__ out)
+{
+  if (!st->pending || static_cast<int>(blockIdx.x) >= st->S.k) return;
+  multi_reduce_wave(nblocks, st->S.k, bpart, out);
+}
+
+// The heuristic classifier's search on the device, one wave: first != 0
+// starts a search from `in` with the errors' min and max from the minmax
+// finalize (sc); otherwise the last queued pass's results (mout) join the
+// measured states.  Then search_replay runs the reference's loop over them
+// (every lane the same scalar work, the lookups spread over the lanes) and
+// either records the result or queues the next pass.  A search that would
+// outgrow the cache or the step limit stops with step.overflow set, for the
+// host to finish.
+__global__ void __launch_bounds__(64)
+    k_search(internal::SearchState* __restrict__ st, const Scalars* __restrict__ sc,
+             const double* __restrict__ mout, int first, internal::SearchInputs in)
+{
+  namespace I = internal;
+  const int lane = threadIdx.x;
+  int len = 0;
+  bool fresh = false;  // the last pass's results, read from mout
+  if (first) {
+    in.mn = sc->mn;
+    in.mx = sc->mx;
+  } else {
+    if (!st->pending) return;
+    in = st->in;
+    len = st->len;
+    fresh = true;
+  }
+  // measured states: the cache of earlier launches, then the last pass
+  // (slot i of the pass holds tree node perm[i]).  Each lane holds its share
+  // of the states in registers (entry q*64 + lane, NaN where there is none,
+  // which equals nothing, as a NaN state equals nothing), so a lookup is a
+  // few compares and ballots; only a hit reads memory.
+  static_assert(I::kSearchCache <= 4 * 64, "four cache entries per lane");
+  double cv[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int i = q * 64 + lane;
+    cv[q] = i < len ? st->ct[i] : __builtin_nan("");
+  }
+  const double fv =
+      fresh && lane < I::kTreeNodes ? st->node[st->perm[lane]] : __builtin_nan("");
+  auto find = [&](double x, std::uint64_t& c, double& d) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const unsigned long long hit = __ballot(cv[q] == x);
+      if (hit) {
+        const int j = q * 64 + __ffsll(static_cast<long long>(hit)) - 1;
+        c = st->ccnt[j];
+        d = st->cdot[j];
+        return true;
+      }
+    }
+    const unsigned long long hit = __ballot(fv == x);
+    if (hit) {
+      const int slot = __ffsll(static_cast<long long>(hit)) - 1;
+      c = static_cast<std::uint64_t>(mout[slot]);
+      d = mout[st->S.k + slot];
+      return true;
+    }
+    return false;
+  };
+  I::SearchStep step;
+  I::search_replay(in, find, step);
+  // every lane has read what it needs: the state may change now
+  __syncthreads();
+  if (lane == 0) {
+    if (first) {
+      st->in = in;
+      st->passes = 0;
+    }
+    if (fresh) {
+      for (int i = 0; i < I::kTreeNodes; ++i) {
+        st->ct[len + i] = st->node[st->perm[i]];
+        st->ccnt[len + i] = static_cast<std::uint64_t>(mout[i]);
+        st->cdot[len + i] = mout[st->S.k + i];
+      }
+      len += I::kTreeNodes;
+    }
+    st->len = len;
+    st->pending = 0;
+    if (!step.done && !step.overflow && len + I::kTreeNodes > I::kSearchCache) step.overflow = 1;
+    st->step = step;
+    if (!step.done && !step.overflow) {
+      I::search_tree(step.t, step.low, step.high, st->node, st->S, st->perm);
+      st->pending = 1;
+      st->passes += 1;
+    }
   }
 }
 
@@ -646,12 +794,11 @@ __global__ void __launch_bounds__(kThreads)
     return;
   }
   const std::size_t n = it.n;
-  const std::size_t nch = num_chunks(n);
-  if (blockIdx.x < nch) {
-    const bool parents = it.has_parents != 0;
+  const bool parents = it.has_parents != 0;
+  const std::size_t nch = num_chunks(n, parents);
+  if (blockIdx.x < nch)
     refine_chunk(n, est, err0, parents ? err1 : err0, parents ? pest : nullptr, epsrel, relerr,
-                 active, part, cnt);
-  }
+                 active, part, cnt, paired_chunk(blockIdx.x, nch));
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) last = atomicAdd(ticket, 1u) == gridDim.x - 1 ? 1 : 0;
@@ -816,62 +963,90 @@ __device__ __forceinline__ void compact_parents_chunk(
     const double* __restrict__ err, const std::uint8_t* __restrict__ dim,
     const std::uint64_t* __restrict__ off, const std::uint64_t* __restrict__ foff,
     double* __restrict__ L2, double* __restrict__ H2, std::uint8_t* __restrict__ pdim2,
-    double* __restrict__ pest, double* __restrict__ perr, std::size_t chunk)
+    double* __restrict__ pest, double* __restrict__ perr, std::size_t chunk, bool paired,
+    const Levels& lv)
 {
-  __shared__ std::uint32_t wsum[kCompactThreads / 64];
-  const std::size_t base = chunk * kChunk;
-  std::uint64_t run = off[chunk] + foff[chunk / kFold];
+  // Every step's positions first (ballots per step and wave, one barrier),
+  // then the moves, with no barrier between the steps, so the row reads of
+  // all steps can be in flight together.  Positions follow the steps in
+  // order, so the list is the one a step-by-step scan gives.
+  constexpr int kSteps = static_cast<int>(kChunk / kCompactThreads);
+  constexpr int kWaves = kCompactThreads / 64;
+  __shared__ std::uint32_t wsum[kSteps][kWaves];
+  std::size_t base, end;
+  chunk_range(chunk, n, paired, base, end);
   const std::size_t h = n / 2;
-  for (int k = 0; k < static_cast<int>(kChunk / kCompactThreads); ++k) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  bool a[kSteps];
+  std::uint32_t pre[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
-    const bool a = t < n && active[t] == 1;
-    double lv[D], hv[D];
-    double ev = 0.0, rv = 0.0;
-    int b = 0;
-    if (a) {
-      ev = est[t];
-      if (perr) rv = err[t];
-      b = dim[t];
-      // the row this region's bounds come from, and the axis its parent
-      // halves (none for a flat list)
-      std::size_t row = t, stride = n;
-      int axis = -1;
-      bool right = false;
-      if (pdim) {
-        right = t >= h;
-        row = right ? t - h : t;
-        stride = h;
-        axis = pdim[row];
-      }
+    a[k] = t < end && active[t] == 1;
+    const unsigned long long ballot = __ballot(a[k]);
+    const unsigned long long below = lane ? (ballot & ((1ull << lane) - 1ull)) : 0ull;
+    pre[k] = static_cast<std::uint32_t>(__popcll(below));
+    if (lane == 0) wsum[k][wave] = static_cast<std::uint32_t>(__popcll(ballot));
+  }
+  __syncthreads();
+  std::uint64_t idx[kSteps];
+  {
+    std::uint64_t run = off[chunk] + foff[chunk / kFold];
 #pragma unroll
-      for (int j = 0; j < D; ++j) {
-        double l = L[static_cast<std::size_t>(j) * stride + row];
-        double w = H[static_cast<std::size_t>(j) * stride + row];
-        if (j == axis) {
-          const double half = w / 2;
-          if (right) l = l + half;
-          w = half;
-        }
-        lv[j] = l;
-        hv[j] = w;
-      }
-    }
-    std::uint32_t rank;
-    const std::uint32_t tot = chunk_rank<kCompactThreads>(a, rank, wsum);
-    if (a) {
-      const std::size_t idx = run + rank;
-      if (idx < m) {
+    for (int k = 0; k < kSteps; ++k) {
+      std::uint32_t woff = 0, tot = 0;
 #pragma unroll
-        for (int j = 0; j < D; ++j) {
-          L2[static_cast<std::size_t>(j) * m + idx] = lv[j];
-          H2[static_cast<std::size_t>(j) * m + idx] = hv[j];
-        }
-        pdim2[idx] = static_cast<std::uint8_t>(b);
-        pest[idx] = ev;
-        if (perr) perr[idx] = rv;
+      for (int w = 0; w < kWaves; ++w) {
+        const std::uint32_t c = wsum[k][w];
+        if (w < wave) woff += c;
+        tot += c;
       }
+      idx[k] = run + woff + pre[k];
+      run += tot;
     }
-    run += tot;
+  }
+  // one step's moves at a time: 46 VGPRs, two workgroups per CU (unrolled,
+  // 101 VGPRs and one workgroup per CU measured 3 % slower)
+#pragma unroll 1
+  for (int k = 0; k < kSteps; ++k) {
+    if (!a[k] || idx[k] >= m) continue;
+    const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
+    const double ev = est[t];
+    const double rv = perr ? err[t] : 0.0;
+    const int b = dim[t];
+    // the row this region's bounds come from, and the axis its parent
+    // halves (none for a flat list)
+    std::size_t row = t, stride = n;
+    int axis = -1;
+    bool right = false;
+    if (pdim) {
+      right = t >= h;
+      row = right ? t - h : t;
+      stride = h;
+      axis = pdim[row];
+    }
+    const std::size_t i = idx[k];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double l, w;
+      int g;
+      read_row(L, H, lv, static_cast<std::size_t>(j) * stride + row, j, l, w, g);
+      if (j == axis) {
+        const double half = w / 2;
+        if (right) l = l + half;
+        w = half;
+        ++g;
+      }
+      const std::size_t o = static_cast<std::size_t>(j) * m + i;
+      L2[o] = l;
+      if (lv.on)
+        reinterpret_cast<std::uint16_t*>(H2)[o] = static_cast<std::uint16_t>(g);
+      else
+        H2[o] = w;
+    }
+    pdim2[i] = static_cast<std::uint8_t>(b);
+    pest[i] = ev;
+    if (perr) perr[i] = rv;
   }
 }
 
@@ -884,10 +1059,11 @@ __global__ void __launch_bounds__(kCompactThreads)
                       const std::uint64_t* __restrict__ off,
                       const std::uint64_t* __restrict__ foff, double* __restrict__ L2,
                       double* __restrict__ H2, std::uint8_t* __restrict__ pdim2,
-                      double* __restrict__ pest, double* __restrict__ perr)
+                      double* __restrict__ pest, double* __restrict__ perr, int paired,
+                      const Levels lv)
 {
   compact_parents_chunk<D>(n, m, L, H, pdim, active, est, err, dim, off, foff, L2, H2, pdim2,
-                           pest, perr, paired_chunk(blockIdx.x, gridDim.x));
+                           pest, perr, paired_chunk(blockIdx.x, gridDim.x), paired != 0, lv);
 }
 
 // The device-resident loop's filter + split: the descriptor says whether it
@@ -901,18 +1077,18 @@ __global__ void __launch_bounds__(kCompactThreads)
                   const std::uint8_t* __restrict__ active, const double* __restrict__ est,
                   const double* err0, const double* err1, const std::uint8_t* __restrict__ dim,
                   const std::uint64_t* __restrict__ off, const std::uint64_t* __restrict__ foff,
-                  double* __restrict__ pest, double* __restrict__ perr)
+                  double* __restrict__ pest, double* __restrict__ perr, const Levels lv)
 {
   if (!cp->run) return;
   const std::size_t n = cp->n, m = cp->m;
-  const std::size_t nch = num_chunks(n);
+  const bool paired = cp->has_parents != 0;
+  const std::size_t nch = num_chunks(n, paired);
   if (blockIdx.x >= nch) return;
   const bool c1 = cp->cur != 0;
   compact_parents_chunk<D>(n, m, c1 ? L1 : L0, c1 ? H1 : H0,
-                        cp->split ? (c1 ? P1 : P0) : nullptr, active, est,
-                        cp->has_parents ? err1 : err0, dim, off, foff, c1 ? L0 : L1,
-                        c1 ? H0 : H1, c1 ? P0 : P1, pest, perr,
-                        paired_chunk(blockIdx.x, nch));
+                           cp->split ? (c1 ? P1 : P0) : nullptr, active, est,
+                           paired ? err1 : err0, dim, off, foff, c1 ? L0 : L1, c1 ? H0 : H1,
+                           c1 ? P0 : P1, pest, perr, paired_chunk(blockIdx.x, nch), paired, lv);
 }
 
 __global__ void __launch_bounds__(256)
@@ -940,6 +1116,32 @@ __global__ void __launch_bounds__(256)
       H2[o] = h;
       H2[o + m] = h;
     }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_expand(int d, std::size_t n, const double* __restrict__ L, const double* __restrict__ H,
+             const std::uint8_t* __restrict__ pdim, const Levels lv, double* __restrict__ L2,
+             double* __restrict__ H2)
+{
+  const std::size_t t = static_cast<std::size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const std::size_t h = n / 2;
+  const bool right = pdim && t >= h;
+  const std::size_t row = pdim ? (right ? t - h : t) : t;
+  const std::size_t stride = pdim ? h : n;
+  const int axis = pdim ? pdim[row] : -1;
+  for (int j = 0; j < d; ++j) {
+    double l, w;
+    int g;
+    read_row(L, H, lv, static_cast<std::size_t>(j) * stride + row, j, l, w, g);
+    if (j == axis) {
+      const double half = w / 2;
+      if (right) l = l + half;
+      w = half;
+    }
+    L2[static_cast<std::size_t>(j) * n + t] = l;
+    H2[static_cast<std::size_t>(j) * n + t] = w;
   }
 }
 
@@ -987,11 +1189,11 @@ hipError_t with_dim(int d, Fn&& fn)
 }  // namespace
 
 hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, double* L,
-                         double* H)
+                         double* H, bool levels)
 {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_uniform_split, grid_for(n, 256), dim3(256), 0, s, d, k, n,
-                     1. / static_cast<double>(k), L, H);
+                     1. / static_cast<double>(k), L, H, levels ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1000,17 +1202,19 @@ hipError_t refine(hipStream_t s, std::size_t n, const double* est, const double*
                   std::uint8_t* active, double* part, std::uint32_t* cnt)
 {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_refine, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads), 0, s,
-                     n, est, err_in, err_out, parent_est, epsrel, relerr, active, part, cnt);
+  hipLaunchKernelGGL(k_refine, dim3(static_cast<unsigned>(num_chunks(n, parent_est != nullptr))),
+                     dim3(kThreads), 0, s, n, est, err_in, err_out, parent_est, epsrel, relerr,
+                     active, part, cnt);
   return hipGetLastError();
 }
 
 hipError_t count_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
-                       const double* est, const double* err, double* part, std::uint32_t* cnt)
+                       const double* est, const double* err, double* part, std::uint32_t* cnt,
+                       bool paired)
 {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_count_flags, dim3(static_cast<unsigned>(num_chunks(n))), dim3(kThreads),
-                     0, s, n, flags, est, err, part, cnt);
+  hipLaunchKernelGGL(k_count_flags, dim3(static_cast<unsigned>(num_chunks(n, paired))),
+                     dim3(kThreads), 0, s, n, flags, est, err, part, cnt, paired ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1041,11 +1245,34 @@ hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x, const std::uin
 }
 
 hipError_t threshold_flags(hipStream_t s, std::size_t n, const double* est, const double* err,
-                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt)
+                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt,
+                           bool paired)
 {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_threshold_flags, dim3(static_cast<unsigned>(num_chunks(n))),
-                     dim3(kThreads), 0, s, n, est, err, thr, flags, part, cnt);
+  hipLaunchKernelGGL(k_threshold_flags, dim3(static_cast<unsigned>(num_chunks(n, paired))),
+                     dim3(kThreads), 0, s, n, est, err, thr, flags, part, cnt, paired ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t search_start(hipStream_t s, internal::SearchState* st, const Scalars* sc,
+                        const internal::SearchInputs& in)
+{
+  hipLaunchKernelGGL(k_search, dim3(1), dim3(64), 0, s, st, sc, nullptr, 1, in);
+  return hipGetLastError();
+}
+
+hipError_t search_round(hipStream_t s, std::size_t n, const double* err,
+                        internal::SearchState* st, double* bpart, double* mout)
+{
+  hipLaunchKernelGGL(k_multi_threshold_dev<kMaxThresholds>, dim3(kMultiBlocks), dim3(kThreads), 0,
+                     s, n, err, st, bpart);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_multi_reduce_dev, dim3(kMaxThresholds - 1), dim3(64), 0, s, kMultiBlocks,
+                     st, bpart, mout);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_search, dim3(1), dim3(64), 0, s, st, nullptr, mout, 0,
+                     internal::SearchInputs{});
   return hipGetLastError();
 }
 
@@ -1102,7 +1329,9 @@ hipError_t step_dev(hipStream_t s, std::size_t n_cap, internal::DevLoop* loop, c
                     double* fpart, std::uint32_t* fcnt, std::uint64_t* foff, Scalars* sc,
                     std::uint32_t* ticket)
 {
-  hipLaunchKernelGGL(k_step_dev, dim3(static_cast<unsigned>(num_chunks(n_cap))), dim3(kThreads),
+  // one workgroup more than num_chunks(n_cap): a paired list may have one
+  // chunk more than a flat one
+  hipLaunchKernelGGL(k_step_dev, dim3(static_cast<unsigned>(num_chunks(n_cap) + 1)), dim3(kThreads),
                      0, s, loop, est, err0, err1, pest, epsrel, relerr, active, part, cnt, off,
                      fpart, fcnt, foff, sc, ticket);
   return hipGetLastError();
@@ -1113,13 +1342,13 @@ hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, i
                        std::uint8_t* P1, const std::uint8_t* active, const double* est,
                        const double* err0, const double* err1, const std::uint8_t* dim,
                        const std::uint64_t* off, const std::uint64_t* foff, double* pest,
-                       double* perr)
+                       double* perr, const Levels& lv)
 {
   return with_dim(d, [&](auto D) {
     hipLaunchKernelGGL((k_compact_dev<decltype(D)::value>),
-                       dim3(static_cast<unsigned>(num_chunks(n_cap))), dim3(kCompactThreads), 0, s,
+                       dim3(static_cast<unsigned>(num_chunks(n_cap) + 1)), dim3(kCompactThreads), 0, s,
                        cp, L0, H0, L1, H1, P0, P1, active, est, err0, err1, dim, off, foff, pest,
-                       perr);
+                       perr, lv);
     return hipGetLastError();
   });
 }
@@ -1129,15 +1358,25 @@ hipError_t compact_parents(hipStream_t s, int d, std::size_t n, std::size_t m, c
                            const std::uint8_t* active, const double* est, const double* err,
                            const std::uint8_t* dim, const std::uint64_t* off,
                            const std::uint64_t* foff, double* L2, double* H2,
-                           std::uint8_t* pdim2, double* pest, double* perr)
+                           std::uint8_t* pdim2, double* pest, double* perr, bool paired,
+                           const Levels& lv)
 {
   if (n == 0 || m == 0) return hipSuccess;
   return with_dim(d, [&](auto D) {
     hipLaunchKernelGGL((k_compact_parents<decltype(D)::value>),
-                       dim3(static_cast<unsigned>(num_chunks(n))), dim3(kCompactThreads), 0, s, n,
-                       m, L, H, pdim, active, est, err, dim, off, foff, L2, H2, pdim2, pest, perr);
+                       dim3(static_cast<unsigned>(num_chunks(n, paired))), dim3(kCompactThreads),
+                       0, s, n, m, L, H, pdim, active, est, err, dim, off, foff, L2, H2, pdim2,
+                       pest, perr, paired ? 1 : 0, lv);
     return hipGetLastError();
   });
+}
+
+hipError_t expand(hipStream_t s, int d, std::size_t n, const double* L, const double* H,
+                  const std::uint8_t* pdim, const Levels& lv, double* L2, double* H2)
+{
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_expand, grid_for(n, 256), dim3(256), 0, s, d, n, L, H, pdim, lv, L2, H2);
+  return hipGetLastError();
 }
 
 hipError_t split_all(hipStream_t s, int d, std::size_t m, const double* L, const double* H,

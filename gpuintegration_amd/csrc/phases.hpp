@@ -18,6 +18,8 @@
 namespace pagani {
 namespace internal {
 struct DevLoop;
+struct SearchState;
+struct SearchInputs;
 }
 namespace phases {
 
@@ -30,6 +32,28 @@ constexpr std::size_t kFold = 256;
 __host__ __device__ inline std::size_t num_chunks(std::size_t n)
 {
   return (n + kChunk - 1) / kChunk;
+}
+
+// Chunks of a list whose regions have parents ("paired": n = 2h children,
+// t and t + h the two children of parent t mod h) are cut at h as well:
+// ceil(h / kChunk) chunks of each half, so chunk c and chunk L + c hold
+// exactly each other's siblings, and a kernel that reads a region's sibling
+// or parent finds them in the paired chunk, which runs beside it on the same
+// XCD (paired_chunk).  The refinement, the flag counts of the classifier and
+// the error-budget fix, their finalize and the compaction of one iteration
+// all cut the list the same way.  At most one chunk more than num_chunks(n).
+__host__ __device__ inline std::size_t num_chunks(std::size_t n, bool paired)
+{
+  return paired ? 2 * ((n / 2 + kChunk - 1) / kChunk) : num_chunks(n);
+}
+__host__ __device__ inline void chunk_range(std::size_t c, std::size_t n, bool paired,
+                                            std::size_t& begin, std::size_t& end)
+{
+  const std::size_t h = paired ? n / 2 : n;
+  const std::size_t L = (h + kChunk - 1) / kChunk;
+  begin = c < L ? c * kChunk : h + (c - L) * kChunk;
+  const std::size_t lim = c < L ? h : n;
+  end = begin + kChunk < lim ? begin + kChunk : lim;
 }
 __host__ __device__ inline std::size_t num_folds(std::size_t nchunks)
 {
@@ -56,8 +80,10 @@ struct ThresholdSet {
   double t[kMaxThresholds];
   int k;
 };
-// fixed grid of the multi-threshold pass (deterministic partial order)
-constexpr int kMultiBlocks = 1024;
+// fixed grid of the multi-threshold pass (deterministic partial order):
+// one workgroup per CU slot -- its 45 KB of LDS buckets leave room for
+// three per CU on the 256 CUs, so the whole grid runs in one wave
+constexpr int kMultiBlocks = 768;
 
 // Results of finalize(), read back by the host driver.
 struct Scalars {
@@ -70,8 +96,18 @@ struct Scalars {
 // per-chunk partial record: 4 (hi, lo) sums + min + max
 constexpr int kPartDoubles = 10;
 
+// Widths of a list in level form (pagani_eval_args): H holds one uint16_t
+// per axis and row, the number of halvings from h0[j]; on = 0: H holds
+// doubles.
+struct Levels {
+  double h0[PAGANI_MAX_DIM];
+  int on;
+};
+
+// k^d regions of the unit cube, widths as doubles, or, with levels, as
+// level 0 (the width 1/k is the caller's h0).
 hipError_t uniform_split(hipStream_t s, int d, std::size_t k, std::size_t n, double* L,
-                         double* H);
+                         double* H, bool levels = false);
 
 // Device-resident loop variants (lists of at most kDevLoopMax regions):
 // fixed launch shapes for n_cap regions, the actual count and selections
@@ -86,18 +122,20 @@ hipError_t compact_dev(hipStream_t s, std::size_t n_cap, const DevCompact* cp, i
                        std::uint8_t* P1, const std::uint8_t* active, const double* est,
                        const double* err0, const double* err1, const std::uint8_t* dim,
                        const std::uint64_t* off, const std::uint64_t* foff, double* pest,
-                       double* perr);
+                       double* perr, const Levels& lv);
 
 // RefineError + relerr classification (parent_est == nullptr: iteration 0,
-// err_out may equal err_in).  Partial sums: {est, err_out, act*est, act*err_out}.
+// err_out may equal err_in).  Partial sums: {est, err_out, act*est, act*err_out},
+// one record per chunk of num_chunks(n, parent_est != nullptr).
 hipError_t refine(hipStream_t s, std::size_t n, const double* est, const double* err_in,
                   double* err_out, const double* parent_est, double epsrel, int relerr,
                   std::uint8_t* active, double* part, std::uint32_t* cnt);
 
-// Same partial sums for given flags (after the budget fix or a classification).
+// Same partial sums for given flags (after the budget fix or a classification),
+// in the chunks of num_chunks(n, paired).
 hipError_t count_flags(hipStream_t s, std::size_t n, const std::uint8_t* flags,
                        const double* est, const double* err, double* part,
-                       std::uint32_t* cnt);
+                       std::uint32_t* cnt, bool paired);
 
 // flags = err > thr; partial sums {0, err, 0, flag*err}, count.
 hipError_t threshold(hipStream_t s, std::size_t n, const double* err, double thr,
@@ -114,11 +152,23 @@ hipError_t sum_dot(hipStream_t s, std::size_t n, const double* x,
 // flags = err > thr and the refine-style record {est, err, flag*est,
 // flag*err} with the flag count (the classifier's accepted threshold).
 hipError_t threshold_flags(hipStream_t s, std::size_t n, const double* est, const double* err,
-                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt);
+                           double thr, std::uint8_t* flags, double* part, std::uint32_t* cnt,
+                           bool paired);
 
 // For every threshold t_k of T: count of err > t_k and sum of those err.
 // Writes T.k counts then T.k sums (doubles) to out (device).  bpart holds
 // kMultiBlocks * 3 * kMaxThresholds doubles.
+// The heuristic classifier's threshold search on the device
+// (classify_search.hpp): search_start begins a search from `in` with the
+// errors' min and max from the last minmax finalize (sc) and queues its first
+// probe pass; each search_round runs the queued pass over err (bpart,
+// mout: the multi-threshold scratch) and the replay that queues the next
+// one.  Rounds after the search has finished do nothing.
+hipError_t search_start(hipStream_t s, internal::SearchState* st, const Scalars* sc,
+                        const internal::SearchInputs& in);
+hipError_t search_round(hipStream_t s, std::size_t n, const double* err,
+                        internal::SearchState* st, double* bpart, double* mout);
+
 hipError_t multi_threshold(hipStream_t s, std::size_t n, const double* err,
                            const ThresholdSet& T, double* bpart, double* out);
 
@@ -149,13 +199,20 @@ hipError_t compact(hipStream_t s, int split, int d, std::size_t n, std::size_t m
 // each is to be halved along (its own fourth-difference choice, dim) into
 // pdim2, their estimates and errors into pest/perr -- and the next list is
 // their 2m children in split form.  The reference writes both children's
-// full bounds (32d bytes per kept region) where this writes 16d + 1.
+// full bounds (32d bytes per kept region) where this writes 16d + 1.  off /
+// foff: the offsets of the chunks of num_chunks(n, paired).
 hipError_t compact_parents(hipStream_t s, int d, std::size_t n, std::size_t m, const double* L,
                            const double* H, const std::uint8_t* pdim,
                            const std::uint8_t* active, const double* est, const double* err,
                            const std::uint8_t* dim, const std::uint64_t* off,
                            const std::uint64_t* foff, double* L2, double* H2,
-                           std::uint8_t* pdim2, double* pest, double* perr);
+                           std::uint8_t* pdim2, double* pest, double* perr, bool paired,
+                           const Levels& lv);
+
+// Any of the loop's lists (flat or split form, widths as doubles or levels)
+// written out flat with double widths: n regions at [j*n + t] of L2, H2.
+hipError_t expand(hipStream_t s, int d, std::size_t n, const double* L, const double* H,
+                  const std::uint8_t* pdim, const Levels& lv, double* L2, double* H2);
 
 // Children of all m regions (Sub_region_splitter::split without a filter);
 // also writes a list held in split form out flat.

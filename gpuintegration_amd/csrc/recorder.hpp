@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "pagani.h"
+#include "phases.hpp"
 #include "rule.hpp"
 
 namespace pagani {
@@ -173,11 +174,12 @@ class DebugRecorder {
 
   // Print_func_evals (PaganiUtils.cuh:130-173): every rule point of every
   // region -- the region's bounds in the unit cube, the integration box, the
-  // point, its value, the region's estimate and error.  L, H, pdim: the list
-  // (split form when pdim is not NULL), fev: the point values
-  // (pagani_eval_args::fevals).
+  // point, its value, the region's estimate and error.  L, H, pdim, lv: the
+  // list (split form when pdim is not NULL, widths as levels when lv.on),
+  // fev: the point values (pagani_eval_args::fevals).
   hipError_t points(std::size_t n, const double* L, const double* H, const std::uint8_t* pdim,
-                    const double* est, const double* err, const double* fev)
+                    const phases::Levels& lv, const double* est, const double* err,
+                    const double* fev)
   {
     if (!points(n)) return hipSuccess;
     const int F = rule_->feval;
@@ -187,8 +189,14 @@ class DebugRecorder {
     hipError_t e = copy(n, est, err);
     if (e == hipSuccess)
       e = hipMemcpy(hL.data(), L, hL.size() * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess)
+    if (e == hipSuccess && lv.on) {
+      std::vector<std::uint16_t> lev(hH.size());
+      e = hipMemcpy(lev.data(), H, lev.size() * sizeof(std::uint16_t), hipMemcpyDeviceToHost);
+      for (std::size_t i = 0; i < lev.size(); ++i)
+        hH[i] = std::ldexp(lv.h0[i / rows], -static_cast<int>(lev[i]));
+    } else if (e == hipSuccess) {
       e = hipMemcpy(hH.data(), H, hH.size() * sizeof(double), hipMemcpyDeviceToHost);
+    }
     if (e == hipSuccess && pdim) e = hipMemcpy(hP.data(), pdim, rows, hipMemcpyDeviceToHost);
     if (e == hipSuccess)
       e = hipMemcpy(hF.data(), fev, hF.size() * sizeof(double), hipMemcpyDeviceToHost);

@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "classify_search.hpp"
 #include "pagani.h"
 #include "phases.hpp"
 #include "recorder.hpp"
@@ -97,6 +98,7 @@ struct LoopState {
   std::uint64_t n = 0;   // regions to evaluate next
   int has_parents = 0;
   int split = 0;         // the list is held in split form
+  phases::Levels lv{};   // the list's widths are levels (the library's uniform start)
   int user_regions = 0;  // Sub_regions& overload bookkeeping
   int relerr = 1;
   double epsrel = 0, epsabs = 0;
@@ -133,6 +135,10 @@ struct pagani_ws {
   pagani::internal::DevBuf<std::uint8_t> pdim[2];  // halved axes of a list in split form
   int cur = 0;
   bool split = false;  // the list in L[cur], H[cur] is in split form (pagani_eval_args)
+  // H[cur] holds levels, not widths (pagani_eval_args); level_form: the
+  // uniform start uses them (pagani_ws_set_level_form)
+  pagani::phases::Levels lv{};
+  bool level_form = true;
   pagani::internal::DevBuf<double> est, err0, err1, pest, perr;
   pagani::internal::DevBuf<std::uint8_t> dim, active, flags;
   pagani::internal::DevBuf<double> part, fpart;
@@ -144,6 +150,10 @@ struct pagani_ws {
   double* d_mpart = nullptr;
   double* d_mout = nullptr;
   double* h_mout = nullptr;
+  // the classifier's device search (classify_search.hpp) and its host copy
+  pagani::internal::SearchState* d_search = nullptr;
+  pagani::internal::SearchState* h_search = nullptr;
+  bool device_search = true;
   hipEvent_t ev[10] = {};
   // device-resident loop, and its full-length rounds as HIP graphs
   bool device_loop = true;
@@ -314,7 +324,8 @@ inline void for_each_pool(pagani_ws* ws, F&& f)
 
 inline int ensure_chunks(pagani_ws* ws, std::size_t n)
 {
-  const std::size_t nch = std::max<std::size_t>(ph::num_chunks(n), 1);
+  // a paired list (ph::num_chunks(n, true)) may have one chunk more
+  const std::size_t nch = ph::num_chunks(n) + 1;
   const std::size_t nfold = std::max<std::size_t>(ph::num_folds(nch), 1);
   int rc;
   if ((rc = ensure(ws, ws->part, nch * ph::kPartDoubles, "chunk sums")) ||
@@ -326,17 +337,21 @@ inline int ensure_chunks(pagani_ws* ws, std::size_t n)
   return ensure(ws, ws->foff, nfold, "fold offsets");
 }
 
-inline hipError_t launch_finalize(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+inline hipError_t launch_finalize(pagani_ws* ws, std::size_t n, bool counts, bool minmax,
+                                  bool paired = false)
 {
-  return ph::finalize(ws->stream, ph::num_chunks(n), ws->part.p, counts ? ws->cnt.p : nullptr,
+  return ph::finalize(ws->stream, ph::num_chunks(n, paired), ws->part.p,
+                      counts ? ws->cnt.p : nullptr,
                       counts ? ws->off.p : nullptr, ws->fpart.p, ws->fcnt.p, ws->foff.p,
                       ws->d_scal, minmax ? 1 : 0);
 }
 
-// Finalize the chunk records of an n-region pass and read the scalars back.
-inline int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax)
+// Finalize the chunk records of an n-region pass (in the chunks of
+// ph::num_chunks(n, paired)) and read the scalars back.
+inline int finalize_read(pagani_ws* ws, std::size_t n, bool counts, bool minmax,
+                         bool paired = false)
 {
-  HIP_TRY(ws, launch_finalize(ws, n, counts, minmax));
+  HIP_TRY(ws, launch_finalize(ws, n, counts, minmax, paired));
   HIP_TRY(ws, hipMemcpyAsync(ws->h_scal, ws->d_scal, sizeof(ph::Scalars),
                              hipMemcpyDeviceToHost, ws->stream));
   HIP_TRY(ws, hipStreamSynchronize(ws->stream));

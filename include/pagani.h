@@ -89,7 +89,7 @@ typedef struct pagani_genz {
  * launch whose stamp differs from the version it was compiled with, so a
  * caller built against other headers fails with PAGANI_EUNSUPPORTED instead
  * of reading a list laid out differently. */
-#define PAGANI_EVAL_ABI_VERSION 3
+#define PAGANI_EVAL_ABI_VERSION 4
 
 /* Uniform data of one evaluation launch: the degree-9 rule (Rule.cuh),
  * the integration volume and the launch shape.  Built by the library; an
@@ -109,8 +109,9 @@ typedef struct pagani_eval_consts {
   int slices;                      /* waves sharing one batch of 64 regions */
   int slice_begin[PAGANI_MAX_SLICES + 1]; /* generic-point slice bounds     */
   double lambda3, lambda4, lambda5; /* generator magnitudes of sets 3, 4, 8 */
-  uint32_t abi_version;             /* PAGANI_EVAL_ABI_VERSION of the library */
-  uint32_t pad;
+  double h0[PAGANI_MAX_DIM]; /* level-form lists: every axis's unit-cube width at level 0 */
+  uint32_t abi_version;      /* PAGANI_EVAL_ABI_VERSION of the library */
+  uint32_t levels;           /* the list's widths are levels (pagani_eval_args) */
 } pagani_eval_consts;
 
 /* Iteration descriptor of the device-resident loop, in device memory: the
@@ -141,7 +142,14 @@ typedef struct pagani_dev_iter {
  * child t / (n/2) of parent t mod (n/2), the left half along that axis
  * for child 0 and the right half for child 1 (Sub_region_splitter.cuh:8-46
  * writes those children out; this library derives them where they are
- * read). */
+ * read).
+ *
+ * Widths are doubles, or levels when consts.levels is set (the library's
+ * own uniform start): H (H2) then points at one uint16_t per axis and row,
+ * the number of times the row's width along that axis was halved from
+ * consts.h0[j], so the width is h0[j] * 2^-level -- the same double that
+ * halving h0[j] that many times gives (a run has at most 700 iterations,
+ * each halving a region along one axis). */
 typedef struct pagani_eval_args {
   size_t n;
   const double* L;
@@ -263,6 +271,18 @@ int pagani_ws_set_device_loop(pagani_ws* ws, int enable);
  * of the same shape (enable = 1, the default); results are identical
  * either way.  No reference counterpart. */
 int pagani_ws_set_graphs(pagani_ws* ws, int enable);
+/* The heuristic classifier's threshold search (heuristic_classifier.cuh:
+ * 362-451) runs on the device by default: the host queues rounds of probe
+ * passes and replays of the reference's loop and reads the outcome once.
+ * enable = 0 runs the same replay on the host, one probe pass per round trip.
+ * Both classify identically. */
+int pagani_ws_set_device_search(pagani_ws* ws, int enable);
+/* The library's uniform start keeps each region's widths as levels (two
+ * bytes per axis: the number of halvings of 1/k, pagani_eval_args) instead of
+ * doubles, so the filter + split and the evaluation move 6 bytes less per
+ * axis and region; results are the same bits.  enable = 0 keeps doubles
+ * (lists given by the caller always keep doubles). */
+int pagani_ws_set_level_form(pagani_ws* ws, int enable);
 /* waves per batch of 64 regions in the evaluation kernel; 0 = automatic */
 int pagani_ws_set_slices(pagani_ws* ws, int slices);
 /* No reference counterpart (the reference fixes n x 64 threads,

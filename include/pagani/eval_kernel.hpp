@@ -666,8 +666,10 @@ struct RegionRow {
   std::size_t row, stride;
   int axis;  // the halved axis, -1 for a flat list
   bool right;
+  const double* h0;  // level form: the widths at level 0 (H holds levels); else NULL
   __device__ __forceinline__ RegionRow(const std::uint8_t* __restrict__ pdim, std::size_t n,
-                                       std::size_t t)
+                                       std::size_t t, const double* level0 = nullptr)
+      : h0(level0)
   {
     if (pdim) {
       const std::size_t m = n / 2;
@@ -686,8 +688,12 @@ struct RegionRow {
                                          const double* __restrict__ H, int j, double& lo,
                                          double& len) const
   {
-    lo = L[static_cast<std::size_t>(j) * stride + row];
-    len = H[static_cast<std::size_t>(j) * stride + row];
+    const std::size_t o = static_cast<std::size_t>(j) * stride + row;
+    lo = L[o];
+    if (h0)
+      len = __builtin_ldexp(h0[j], -static_cast<int>(reinterpret_cast<const std::uint16_t*>(H)[o]));
+    else
+      len = H[o];
     if (j == axis) {
       const double half = len / 2;
       if (right) lo = lo + half;
@@ -703,14 +709,16 @@ template <int D, class F>
 __device__ __forceinline__ void region_frame(F& f, const double* __restrict__ L,
                                              const double* __restrict__ H,
                                              const std::uint8_t* __restrict__ pdim, std::size_t n,
-                                             std::size_t t, const double* lows,
-                                             const double* range, double (&cR)[D],
-                                             double (&hR)[D], double& vol, int& maxdim)
+                                             std::size_t t, const pagani_eval_consts& C,
+                                             double (&cR)[D], double (&hR)[D], double& vol,
+                                             int& maxdim)
 {
+  const double* lows = C.lows;
+  const double* range = C.range;
   double maxr = 0.0;
   vol = 1.0;
   maxdim = 0;
-  const RegionRow rr(pdim, n, t);
+  const RegionRow rr(pdim, n, t, C.levels ? C.h0 : nullptr);
 #pragma unroll
   for (int j = 0; j < D; ++j) {
     double lo, len;
@@ -766,7 +774,7 @@ __device__ __forceinline__ void eval_regions_wave(
   double cR[D], hR[D];
   double vol;
   int maxdim;
-  region_frame<D>(f, L, H, pdim, n, t, lows, range, cR, hR, vol, maxdim);
+  region_frame<D>(f, L, H, pdim, n, t, C, cR, hR, vol, maxdim);
 
   double rs[PAGANI_NRULES] = {0.0, 0.0, 0.0, 0.0, 0.0};
   int bisect = maxdim;
@@ -965,7 +973,7 @@ __global__ void __launch_bounds__(256)
   const int p = static_cast<int>(i % F_);
   double cR[D], hR[D], vol;
   int maxdim;
-  detail::region_frame<D>(f, L, H, P, n, t, C.lows, C.range, cR, hR, vol, maxdim);
+  detail::region_frame<D>(f, L, H, P, n, t, C, cR, hR, vol, maxdim);
   Real x[D];
 #pragma unroll
   for (int j = 0; j < D; ++j)

@@ -186,11 +186,12 @@ def _oracle_classify(d, err, iter_err, iter_fin, tot_fin, hist, epsrel):
     return res, flags
 
 
-def _check_classify(pg, gpu, d, err, iter_fin, tot_fin, hist, epsrel):
+def _check_classify(pg, gpu, d, err, iter_fin, tot_fin, hist, epsrel, device_search=True):
     import torch
     iter_err = err.sum()
     ores, oflags = _oracle_classify(d, err, iter_err, iter_fin, tot_fin, hist, epsrel)
     with pg.Workspace(d, 0) as ws:
+        ws.set_device_search(device_search)
         flags = torch.empty(err.size, dtype=torch.uint8, device=gpu)
         g = ws.classify(dev(err, gpu), err.size, epsrel, hist, iter_err, iter_fin, tot_fin, flags)
         gflags = host(flags)
@@ -204,14 +205,17 @@ def _check_classify(pg, gpu, d, err, iter_fin, tot_fin, hist, epsrel):
     assert np.array_equal(gflags, oflags)
 
 
-def test_classifier_kat(pg, gpu):
-    """test_heuristic_classifier.cpp:32-104 on the device."""
+@pytest.mark.parametrize("device_search", [True, False])
+def test_classifier_kat(pg, gpu, device_search):
+    """test_heuristic_classifier.cpp:32-104 on the device, the threshold
+    search run on the device and on the host."""
     errs = np.array([.075, .99, .079, 101.96, 101.33, 1.93, 101.99])
-    _check_classify(pg, gpu, 3, errs, 0.0, 4.2, [7000.0] * 3, 1e-3)
+    _check_classify(pg, gpu, 3, errs, 0.0, 4.2, [7000.0] * 3, 1e-3, device_search)
 
 
+@pytest.mark.parametrize("device_search", [True, False])
 @pytest.mark.parametrize("seed", range(6))
-def test_classifier_random(pg, gpu, seed):
+def test_classifier_random(pg, gpu, seed, device_search):
     """Log-normal error lists of 10^4..10^6 regions under varied budgets."""
     rng = np.random.default_rng(seed)
     n = int(10 ** rng.uniform(4, 6))
@@ -220,4 +224,25 @@ def test_classifier_random(pg, gpu, seed):
     est = iter_err / float(rng.choice([1e-3, 1e-4, 1e-5]))
     tot_fin = iter_err * rng.uniform(0, 2)
     iter_fin = iter_err * rng.uniform(0, 0.3)
-    _check_classify(pg, gpu, 8, err, iter_fin, tot_fin, [est] * 3, 1e-4)
+    _check_classify(pg, gpu, 8, err, iter_fin, tot_fin, [est] * 3, 1e-4, device_search)
+
+
+def test_classifier_search_device_vs_host_in_the_loop(pg, gpu):
+    """A loop that classifies (the 8D Gaussian of the benchmark at 1e-6 from
+    4^8 regions classifies in about ten iterations): the device search and
+    the host search give the same trajectory and result, bit for bit."""
+    from gpuintegration_amd import workloads as W
+    fam, a, u = W.gaussian_8d(2024)
+    out = []
+    for device_search in (True, False):
+        with pg.Workspace(8, 0) as ws:
+            ws.set_device_search(device_search)
+            ws.set_timing(True)
+            r = ws.integrate_genz(fam, a, u, epsrel=1e-6, epsabs=1e-20, init_k=4)
+            out.append((r, ws.trace(), ws.stats()))
+    (r1, t1, s1), (r2, t2, s2) = out
+    assert s1["classify_passes"] > 0 and s1["classify_passes"] == s2["classify_passes"]
+    assert (r1.estimate, r1.errorest, r1.nregions, r1.iters, r1.status) == \
+        (r2.estimate, r2.errorest, r2.nregions, r2.iters, r2.status)
+    for k in ("n", "m", "estimate", "errorest"):
+        assert np.array_equal(t1[k], t2[k]), k
