@@ -17,6 +17,9 @@ The headline line also carries, each timed in the same run:
   mcubes   the m-Cubes VEGAS integrator on the same integrand
   batch    the 512-integral Genz batch (family and d in {3..8} drawn,
            epsrel log-uniform in [1e-6, 1e-3]) from one call
+  persistent_batch  the batch's persistent kernel (whole integrals, one
+           workgroup each) off and on, on 512 short integrals and on the
+           512-integral batch
   large    8D cos(sum (i+1) x_i) and 8D discontinuous at epsrel 1e-8 with
            the library's own memory model (pools of 2e8-5e8 regions)
   plain_cpu_vs_gpu  5D oscillatory at epsrel 1e-3: the sequential
@@ -364,6 +367,7 @@ def config2(args, pg, W):
     if not args.no_mcubes:
         out["mcubes"] = mcubes_leg(args, pg, W)
     out["batch"] = config4(args, pg, W, leg=True)
+    out["persistent_batch"] = persistent_leg(args, pg, W)
     # the same batch with the fixed launch-shape rule instead of the
     # autotuner's table
     rule = config4(args, pg, W, leg=True, tuned=False)
@@ -466,10 +470,12 @@ def config5(args, pg, W):
             "runs": out}
 
 
-def config4(args, pg, W, leg=False, tuned=True):
+def config4(args, pg, W, leg=False, tuned=True, segments=None):
     """512 independent integrals (family, d in {3..8}, epsrel log-uniform in
     [1e-6, 1e-3]) from one pagani_integrate_batch call.  As a leg of the
-    headline line: one timed call after a 16-item warm-up."""
+    headline line: one timed call after a 16-item warm-up.  segments: the
+    persistent kernel's segment size (pagani_batch_set_segments; None:
+    --segments, 0: off)."""
     import collections
     items = W.batch(args.batch_count, args.seed)
     steps = 1 if leg else args.steps
@@ -477,6 +483,8 @@ def config4(args, pg, W, leg=False, tuned=True):
     b = pg.Batch(0, args.workers)
     b.set_timing(True)
     b.set_tuned_slices(tuned)
+    seg = args.segments if segments is None else segments
+    b.set_segments(seg)
     for _ in range(warmup):
         b.integrate(items[:16])
     _sync()
@@ -508,8 +516,9 @@ def config4(args, pg, W, leg=False, tuned=True):
         "higher_is_better": True, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{args.batch_count} Genz integrals, family ~ U(6), d ~ U{{3..8}}, "
                                "parameters per family, epsrel = 10^(-3-3U), epsabs 1e-20",
-                   "seed": args.seed, "workers": args.workers},
+                   "seed": args.seed, "workers": args.workers, "segment_regions": seg},
         "regions_per_s": st["regions_evaluated"] / dt,
+        "segment_integrals": st["segment_integrals"] // steps,
         "converged": conv, "rel_err_within_epsrel": within,
         "codes_negative": sum(1 for c in codes if c < 0),
         "phase_ms_per_step": {k: st[k] / steps for k in
@@ -517,6 +526,49 @@ def config4(args, pg, W, leg=False, tuned=True):
                                "device_ms", "alloc_ms")},
         "by_family": fam_rows,
     }
+
+
+def persistent_leg(args, pg, W, reps=3):
+    """The batch's persistent kernel (pagani_batch_set_segments) on many
+    short integrals (workloads.small_batch: 512 items, d <= 6, epsrel >=
+    1e-4) and on the 512-integral batch, off and on: median of `reps` timed
+    calls after one untimed call, one worker."""
+    import statistics
+    seg = args.persistent_segments
+
+    def timed(items, regions, n):
+        b = pg.Batch(0, 1)
+        b.set_segments(regions)
+        b.integrate(items)
+        _sync()
+        ts = []
+        for _ in range(n):
+            b.reset_stats()
+            t0 = time.perf_counter()
+            res, codes = b.integrate(items)
+            _sync()
+            ts.append(time.perf_counter() - t0)
+        st = b.stats()
+        b.close()
+        return statistics.median(ts), st, res
+
+    out = {"segment_regions": seg, "workers": 1}
+    for name, items, n in (("small_batch", W.small_batch(512, 7), reps),
+                           ("batch", W.batch(args.batch_count, args.seed), 1)):
+        t_off, _, r_off = timed(items, 0, n)
+        t_on, st, r_on = timed(items, seg, n)
+        same = sum(1 for x, y in zip(r_on, r_off)
+                   if (x.nregions, x.iters, x.status) == (y.nregions, y.iters, y.status))
+        out[name] = {
+            "items": len(items), "timed_calls": n,
+            "integrals_per_s_off": len(items) / t_off, "integrals_per_s_on": len(items) / t_on,
+            "speedup": t_off / t_on,
+            "segment_integrals": st["segment_integrals"], "segment_ms": st["segment_ms"],
+            "same_iterations": same,
+            "max_rel_diff_estimate": max(abs(x.estimate - y.estimate) / max(abs(y.estimate), 1e-300)
+                                         for x, y in zip(r_on, r_off)),
+        }
+    return out
 
 
 def mcubes_leg(args, pg, W, runs=3):
@@ -639,6 +691,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--segments", type=int, default=0,
+                    help="the batch's persistent kernel: regions per segment (0: off)")
+    ap.add_argument("--persistent-segments", type=int, default=65536,
+                    help="segment size of the persistent-kernel leg")
     ap.add_argument("--eval-only", action="store_true",
                     help="time the evaluation kernel alone (for counter runs)")
     ap.add_argument("--plain", action="store_true",

@@ -123,6 +123,8 @@ class Stats(ctypes.Structure):
         ("device_rounds", ctypes.c_uint64),
         ("budget_fixes", ctypes.c_uint64),
         ("graph_launches", ctypes.c_uint64),
+        ("segment_integrals", ctypes.c_uint64),
+        ("segment_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -249,6 +251,7 @@ _SIGS = {
     "pagani_batch_set_mem_model": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_batch_set_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pagani_batch_set_tuned_slices": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pagani_batch_set_segments": (ctypes.c_int, [_vp, ctypes.c_size_t]),
     "pagani_batch_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(Stats)]),
     "pagani_batch_reset_stats": (ctypes.c_int, [_vp]),
     "pagani_integrate_batch": (ctypes.c_int, [_vp, ctypes.POINTER(_BatchItem), _sz,
@@ -697,6 +700,12 @@ class Batch:
 
     def set_tuned_slices(self, on=True):
         self._check(self.lib.pagani_batch_set_tuned_slices(self._b, 1 if on else 0))
+
+    def set_segments(self, regions):
+        """The persistent kernel (pagani_batch_set_segments): FP64 items whose
+        initial split fits `regions` regions run whole, one workgroup each, in
+        one launch per (dimension, family); 0 turns it off."""
+        self._check(self.lib.pagani_batch_set_segments(self._b, int(regions)))
 
     def stats(self):
         s = Stats()

@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
 #include <cstring>
 #include <mutex>
 #include <numeric>
@@ -28,7 +29,11 @@
 #include <vector>
 
 #include "pagani.h"
+#include "segments.hpp"
 #include "ws_internal.hpp"
+
+// concurrent launches of the persistent kernel
+constexpr int kSegStreams = 4;
 
 struct pagani_batch {
   int device = 0;
@@ -42,6 +47,18 @@ struct pagani_batch {
   std::vector<std::array<pagani_ws*, PAGANI_MAX_DIM + 1>> ws;
   std::mutex mu;
   std::string error;
+  // the persistent kernel (pagani_batch_set_segments): regions per segment
+  // (0: off), the pool and the item arrays, kept between calls
+  std::size_t seg_regions = 0;
+  int num_cus = 256;
+  char* seg_pool = nullptr;
+  std::size_t seg_pool_bytes = 0;
+  void* seg_dev = nullptr;  // items, outputs and the counter
+  std::size_t seg_dev_bytes = 0;
+  std::uint64_t seg_integrals = 0;  // finished in segments (statistics)
+  double seg_ms = 0;                // host time of the segment launches
+  hipStream_t seg_stream[kSegStreams] = {};
+  hipEvent_t seg_event[kSegStreams] = {};
 };
 
 namespace {
@@ -92,6 +109,169 @@ int run_item(pagani_batch* b, int w, const pagani_batch_item& it, pagani_result*
   return rc;
 }
 
+// Grow a device buffer (contents not kept).
+int grow(pagani_batch* b, void** p, std::size_t* cap, std::size_t need)
+{
+  if (need <= *cap) return PAGANI_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, need) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error(b, "hipMalloc of the segment pool failed");
+    return PAGANI_ENOMEM;
+  }
+  *cap = need;
+  return PAGANI_OK;
+}
+
+// The items of one persistent-kernel launch: sel[first, first + count).
+struct SegGroup {
+  int d, fam;
+  std::size_t first, count;
+  unsigned grid;
+  std::size_t bytes, pool_off;
+  pagani_eval_consts C;
+};
+
+// Streams and events of the persistent kernel's launches, made on first use.
+int seg_streams(pagani_batch* b)
+{
+  if (b->seg_stream[0]) return PAGANI_OK;
+  for (int i = 0; i < kSegStreams; ++i)
+    if (hipStreamCreateWithFlags(&b->seg_stream[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->seg_event[i], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error(b, "stream creation for the segment kernel failed");
+      return PAGANI_EHIP;
+    }
+  return PAGANI_OK;
+}
+
+// The persistent kernel over the eligible items (FP64 Genz descriptors whose
+// initial split fits a segment); done[i] is set for the items it finished.
+// One launch per (dimension, family), all at once on kSegStreams streams,
+// highest dimension first, so that while one launch waits for its slowest
+// integral the others' workgroups take the freed compute units; within a
+// launch the tightest epsrel goes first.
+int run_segments(pagani_batch* b, const pagani_batch_item* items, std::size_t count,
+                 pagani_result* results, int* codes, std::vector<char>& done)
+{
+  namespace I = pagani::internal;
+  const auto t0 = std::chrono::steady_clock::now();
+  const std::size_t S = b->seg_regions;
+  std::vector<I::SegItem> sel;
+  std::vector<std::size_t> idx;
+  std::vector<SegGroup> groups;
+  std::size_t pool = 0;
+  for (int d = PAGANI_GENZ_MAX_DIM; d >= PAGANI_MIN_DIM; --d)
+    for (int fam = PAGANI_GENZ_OSCILLATORY; fam <= PAGANI_GENZ_DISCONTINUOUS; ++fam) {
+      std::vector<std::size_t> mine;
+      for (std::size_t i = 0; i < count; ++i) {
+        const pagani_batch_item& it = items[i];
+        if (it.f.ndim != d || it.f.family != fam || it.precision != PAGANI_FP64) continue;
+        if (!(it.epsrel >= 0.0) || !(it.epsabs >= 0.0)) continue;
+        const std::size_t k = it.init_k ? it.init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
+        std::size_t n = 1;
+        bool fits = true;
+        for (int j = 0; j < d && fits; ++j) {
+          n *= k;
+          fits = n <= S;
+        }
+        if (fits) mine.push_back(i);
+      }
+      if (mine.empty()) continue;
+      std::stable_sort(mine.begin(), mine.end(), [&](std::size_t x, std::size_t y) {
+        return items[x].epsrel < items[y].epsrel;
+      });
+      // the rule's launch constants at dimension d
+      pagani_ws*& ws = b->ws[0][d];
+      if (!ws) ws = pagani_ws_create(d, b->device);
+      if (!ws) {
+        set_error(b, std::string("workspace creation failed: ") + pagani_create_error());
+        return PAGANI_EHIP;
+      }
+      SegGroup g{};
+      g.d = d;
+      g.fam = fam;
+      g.first = sel.size();
+      g.count = mine.size();
+      if (pagani_ws_eval_consts(ws, nullptr, nullptr, 64, &g.C) != PAGANI_OK) {
+        set_error(b, "launch constants failed");
+        return PAGANI_EHIP;
+      }
+      g.grid = I::segment_grid(d, fam,
+                               static_cast<unsigned>(std::min<std::size_t>(
+                                   mine.size(), 4 * static_cast<std::size_t>(b->num_cus))));
+      g.bytes = I::segment_bytes(d, S);
+      g.pool_off = pool;
+      pool += g.grid * g.bytes;
+      groups.push_back(g);
+      for (std::size_t i : mine) {
+        const pagani_batch_item& it = items[i];
+        I::SegItem s{};
+        s.f = it.f;
+        s.epsrel = it.epsrel;
+        s.epsabs = it.epsabs;
+        s.init_k = it.init_k ? it.init_k : (d < 5 ? 4 : (d <= 10 ? 2 : 1));
+        s.relerr = it.relerr_classification ? 1 : 0;
+        sel.push_back(s);
+        idx.push_back(i);
+      }
+    }
+  if (groups.empty()) return PAGANI_OK;
+  int rc;
+  if ((rc = seg_streams(b))) return rc;
+  if ((rc = grow(b, reinterpret_cast<void**>(&b->seg_pool), &b->seg_pool_bytes, pool))) return rc;
+  // device: the items and the launches' counters (one copy in), then the
+  // outputs (one copy out)
+  const std::size_t ib = sel.size() * sizeof(I::SegItem);
+  const std::size_t cb = (groups.size() * sizeof(std::uint32_t) + 255) / 256 * 256;
+  const std::size_t ob = sel.size() * sizeof(I::SegOut);
+  if ((rc = grow(b, &b->seg_dev, &b->seg_dev_bytes, ib + cb + ob))) return rc;
+  std::vector<char> in(ib + cb, 0);
+  std::memcpy(in.data(), sel.data(), ib);
+  char* dev = static_cast<char*>(b->seg_dev);
+  auto* ditems = reinterpret_cast<I::SegItem*>(dev);
+  auto* dnext = reinterpret_cast<std::uint32_t*>(dev + ib);
+  auto* dout = reinterpret_cast<I::SegOut*>(dev + ib + cb);
+  std::vector<I::SegOut> out(sel.size());
+  hipStream_t* st = b->seg_stream;
+  bool ok = hipMemcpyAsync(dev, in.data(), ib + cb, hipMemcpyHostToDevice, st[0]) == hipSuccess &&
+            hipEventRecord(b->seg_event[0], st[0]) == hipSuccess;
+  for (int k = 1; ok && k < kSegStreams; ++k)
+    ok = hipStreamWaitEvent(st[k], b->seg_event[0], 0) == hipSuccess;
+  for (std::size_t q = 0; ok && q < groups.size(); ++q) {
+    const SegGroup& g = groups[q];
+    ok = I::launch_segments(st[q % kSegStreams], g.d, g.fam, ditems + g.first,
+                            static_cast<std::uint32_t>(g.count), dnext + q,
+                            b->seg_pool + g.pool_off, g.bytes, S, g.grid, g.C, item_total(b),
+                            b->mem_model, dout + g.first) == hipSuccess;
+  }
+  for (int k = 1; ok && k < kSegStreams; ++k)
+    ok = hipEventRecord(b->seg_event[k], st[k]) == hipSuccess &&
+         hipStreamWaitEvent(st[0], b->seg_event[k], 0) == hipSuccess;
+  ok = ok &&
+       hipMemcpyAsync(out.data(), dout, ob, hipMemcpyDeviceToHost, st[0]) == hipSuccess &&
+       hipStreamSynchronize(st[0]) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    for (int k = 0; k < kSegStreams; ++k) (void)hipStreamSynchronize(st[k]);
+    set_error(b, "segment kernel failed");
+    return PAGANI_EHIP;
+  }
+  for (std::size_t q = 0; q < sel.size(); ++q) {
+    if (out[q].outcome != 0) continue;
+    results[idx[q]] = out[q].r;
+    codes[idx[q]] = PAGANI_OK;
+    done[idx[q]] = 1;
+    b->seg_integrals++;
+  }
+  b->seg_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                   .count();
+  return PAGANI_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -129,6 +309,7 @@ pagani_batch* pagani_batch_create(int device, int workers)
   b->device = device;
   b->workers = workers;
   b->hbm = prop.totalGlobalMem;
+  b->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   std::array<pagani_ws*, PAGANI_MAX_DIM + 1> none;
   none.fill(nullptr);
   b->ws.assign(static_cast<std::size_t>(workers), none);
@@ -141,6 +322,12 @@ void pagani_batch_destroy(pagani_batch* b)
   for (auto& row : b->ws)
     for (pagani_ws* w : row)
       if (w) pagani_ws_destroy(w);
+  if (b->seg_pool) (void)hipFree(b->seg_pool);
+  if (b->seg_dev) (void)hipFree(b->seg_dev);
+  for (int k = 0; k < kSegStreams; ++k) {
+    if (b->seg_event[k]) (void)hipEventDestroy(b->seg_event[k]);
+    if (b->seg_stream[k]) (void)hipStreamDestroy(b->seg_stream[k]);
+  }
   delete b;
 }
 
@@ -172,6 +359,17 @@ int pagani_batch_set_timing(pagani_batch* b, int enable)
 {
   if (!b) return PAGANI_EBADARG;
   b->timing = enable != 0;
+  return PAGANI_OK;
+}
+
+int pagani_batch_set_segments(pagani_batch* b, size_t regions)
+{
+  if (!b) return PAGANI_EBADARG;
+  if (regions != 0 && (regions < 256 || regions > (std::size_t(1) << 22))) {
+    b->error = "segment size outside [256, 2^22] regions";
+    return PAGANI_EBADARG;
+  }
+  b->seg_regions = regions;
   return PAGANI_OK;
 }
 
@@ -212,6 +410,9 @@ int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
       out->budget_fixes += s.budget_fixes;
       out->graph_launches += s.graph_launches;
     }
+  out->segment_integrals = b->seg_integrals;
+  out->segment_ms = b->seg_ms;
+  out->integrals += b->seg_integrals;
   return PAGANI_OK;
 }
 
@@ -221,6 +422,8 @@ int pagani_batch_reset_stats(pagani_batch* b)
   for (auto& row : b->ws)
     for (pagani_ws* w : row)
       if (w) pagani_ws_reset_stats(w);
+  b->seg_integrals = 0;
+  b->seg_ms = 0;
   return PAGANI_OK;
 }
 
@@ -234,9 +437,22 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
     return PAGANI_EBADARG;
   }
   b->error.clear();
-  // largest first: higher dimension, then tighter epsrel, then input order
-  std::vector<std::size_t> order(count);
-  std::iota(order.begin(), order.end(), std::size_t(0));
+  std::vector<char> done(count, 0);
+  if (b->seg_regions) {
+    if (hipSetDevice(b->device) != hipSuccess) {
+      (void)hipGetLastError();
+      b->error = "hipSetDevice failed";
+      return PAGANI_EHIP;
+    }
+    const int rc = run_segments(b, items, count, results, codes, done);
+    if (rc) return rc;
+  }
+  // the rest, largest first: higher dimension, then tighter epsrel, then
+  // input order
+  std::vector<std::size_t> order;
+  for (std::size_t i = 0; i < count; ++i)
+    if (!done[i]) order.push_back(i);
+  const std::size_t nrest = order.size();
   std::stable_sort(order.begin(), order.end(), [&](std::size_t x, std::size_t y) {
     if (items[x].f.ndim != items[y].f.ndim) return items[x].f.ndim > items[y].f.ndim;
     return items[x].epsrel < items[y].epsrel;
@@ -246,7 +462,7 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
     if (hipSetDevice(b->device) != hipSuccess) {
       (void)hipGetLastError();
       set_error(b, "hipSetDevice failed in a batch worker");
-      for (std::size_t k; (k = next.fetch_add(1)) < count;) {
+      for (std::size_t k; (k = next.fetch_add(1)) < nrest;) {
         const std::size_t i = order[k];
         std::memset(&results[i], 0, sizeof(results[i]));
         results[i].status = 1;
@@ -260,7 +476,7 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
     // pools never crowd out the running one and the pool grows once per
     // worker instead of once per dimension.
     int last_d = 0;
-    for (std::size_t k; (k = next.fetch_add(1)) < count;) {
+    for (std::size_t k; (k = next.fetch_add(1)) < nrest;) {
       const std::size_t i = order[k];
       const int d = items[i].f.ndim;
       if (d != last_d && d >= PAGANI_MIN_DIM && d <= PAGANI_MAX_DIM) {
@@ -276,8 +492,9 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
       codes[i] = run_item(b, w, items[i], &results[i]);
     }
   };
-  const int nthreads = static_cast<int>(std::min<std::size_t>(b->workers, count));
-  if (nthreads == 1) {
+  const int nthreads = static_cast<int>(std::min<std::size_t>(b->workers, nrest));
+  if (nthreads == 0) {
+  } else if (nthreads == 1) {
     work(0);
   } else {
     std::vector<std::thread> pool;

@@ -5,6 +5,8 @@
 // Gaussian, 2 pi u_0 for the oscillatory phase.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cmath>
 #include <type_traits>
 
@@ -14,8 +16,10 @@
 namespace pagani {
 namespace internal {
 
+// Host and device (the batch's segment kernel builds the functors on the
+// device from the descriptors in its item list).
 template <class Real, int D, class Fn>
-int visit_genz(const pagani_genz* g, Fn&& fn)
+__host__ __device__ int visit_genz(const pagani_genz* g, Fn&& fn)
 {
   switch (g->family) {
     case PAGANI_GENZ_OSCILLATORY: {

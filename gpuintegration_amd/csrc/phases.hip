@@ -9,72 +9,13 @@
 #include <type_traits>
 
 #include "classify_search.hpp"
+#include "device_common.hpp"
 #include "devloop_decide.hpp"
 
 namespace pagani {
 namespace phases {
 
 namespace {
-
-struct DD {
-  double hi, lo;
-};
-
-// Compensated addition (two-sum of the leading parts, then the tails).
-// Non-finite sums fall back to the plain sum so inf/NaN propagate as in an
-// uncompensated reduction.
-__device__ __forceinline__ DD dd_add(DD a, DD b)
-{
-  const double s = a.hi + b.hi;
-  if (!__builtin_isfinite(s)) return DD{s, 0.0};
-  const double bb = s - a.hi;
-  double e = (a.hi - (s - bb)) + (b.hi - bb);
-  e += a.lo + b.lo;
-  const double hi = s + e;
-  return DD{hi, e - (hi - s)};
-}
-
-__device__ __forceinline__ DD dd_add(DD a, double b) { return dd_add(a, DD{b, 0.0}); }
-
-__device__ __forceinline__ DD wave_reduce(DD v)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    DD o;
-    o.hi = __shfl_xor(v.hi, off, 64);
-    o.lo = __shfl_xor(v.lo, off, 64);
-    v = dd_add(v, o);
-  }
-  return v;
-}
-
-// Reduce a DD over a 256-thread block; every thread gets the result.
-__device__ __forceinline__ DD block_reduce(DD v, DD* sh)
-{
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  v = wave_reduce(v);
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  DD r = sh[0];
-#pragma unroll
-  for (int w = 1; w < kThreads / 64; ++w) r = dd_add(r, sh[w]);
-  __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ std::uint32_t block_count(std::uint32_t c, std::uint32_t* sh)
-{
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
-  if (lane == 0) sh[wave] = c;
-  __syncthreads();
-  std::uint32_t r = 0;
-#pragma unroll
-  for (int w = 0; w < kThreads / 64; ++w) r += sh[w];
-  __syncthreads();
-  return r;
-}
 
 // Write the chunk record: four DD sums and a count.
 __device__ __forceinline__ void write_record(DD s0, DD s1, DD s2, DD s3, std::uint32_t c,
@@ -162,17 +103,7 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
         const bool right = t >= h;
         const std::size_t sib = right ? t - h : t + h;
         const std::size_t par = right ? t - h : t;
-        const double es = est[sib];
-        const double rsib = err_in[sib];
-        const double diff = fabs(.25 * ((es + e) - pest[par]));
-        const double E = r + rsib;
-        if (E > 0.0) {
-          const double cc = 1 + 2 * diff / E;
-          r *= cc;
-        }
-        r += diff;
-        const bool pass = relerr && r < fmax(epsrel * fabs(e), 1e-200);
-        a = !pass;
+        refine_region(e, r, est[sib], err_in[sib], pest[par], epsrel, relerr, r, a);
       }
       err_out[t] = r;
       active[t] = a ? 1 : 0;

@@ -206,6 +206,9 @@ typedef struct pagani_stats {
   uint64_t device_rounds;     /* its host round trips */
   uint64_t budget_fixes;      /* iterations that took fix_error_budget_overflow */
   uint64_t graph_launches;    /* device-loop rounds replayed from a HIP graph */
+  uint64_t segment_integrals; /* batch items finished by the persistent kernel
+                                 (included in integrals) */
+  double segment_ms;          /* host time of the persistent kernel's launches */
 } pagani_stats;
 
 /* Result of Heuristic_classifier::classify, heuristic_classifier.cuh:22-58 */
@@ -445,6 +448,18 @@ int pagani_batch_set_mem_model(pagani_batch* b, int model);
 int pagani_batch_set_timing(pagani_batch* b, int enable);
 /* pagani_ws_set_tuned_slices for every workspace of the batch */
 int pagani_batch_set_tuned_slices(pagani_batch* b, int enable);
+/* No reference counterpart (the reference runs every integral through its
+ * host loop).  regions > 0 turns on the persistent kernel: FP64 Genz items
+ * whose initial split has at most `regions` regions run first, whole, in one
+ * launch per (dimension, family) -- one workgroup per integral, each with a
+ * segment of `regions` regions of one pool, no host round trip per
+ * iteration.  An item whose list would outgrow its segment, that needs the
+ * error-budget fix (Workspace.cuh:121-146) or that comes past a tenth of its
+ * memory budget (where the classifier may run) is integrated again by the
+ * regular loop.  The iterations are those of pagani_integrate_genz; the
+ * results agree with it to rounding (the sums are taken in another order),
+ * not to the bit.  0 (the default) = off, else regions in [256, 2^22]. */
+int pagani_batch_set_segments(pagani_batch* b, size_t regions);
 /* statistics summed over all workspaces of the batch */
 int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out);
 int pagani_batch_reset_stats(pagani_batch* b);

@@ -1,0 +1,146 @@
+"""The batch's persistent kernel (pagani_batch_set_segments) on the GPU.
+
+One workgroup runs an item's whole loop in its own segment of one pool.  The
+iterations are the single-integral loop's (Workspace.cuh:362-494 with the
+device-resident loop's decisions), but every region is evaluated with one
+wave per 64-region batch and the iteration's sums are reduced in another
+order, so results agree with the regular path to rounding, not to the bit.
+Checked here against the regular path (the same batch with the kernel off),
+against the oracle's loop (status, region and iteration counts) and against
+the closed forms; items the kernel hands back (list beyond the segment,
+FP32) must come out bit for bit as the regular path gives them.
+"""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEG = 65536
+
+
+def _items(count, seed, max_d=8, min_eps=1e-6):
+    from gpuintegration_amd import workloads as W
+    out = []
+    for fam, a, u, eps in W.batch(count, seed=seed):
+        if len(a) > max_d:
+            a, u = a[:max_d], u[:max_d]
+        out.append((fam, a, u, max(eps, min_eps)))
+    return out
+
+
+def _hbm():
+    import torch
+    return torch.cuda.get_device_properties(0).total_memory
+
+
+def _run(pg, items, seg, workers=1, mem=None, **kw):
+    with pg.Batch(0, workers) as b:
+        if mem:
+            b.set_device_mem(mem)
+        if seg:
+            b.set_segments(seg)
+        res, codes = b.integrate(items, **kw)
+        st = b.stats()
+    assert all(c >= 0 for c in codes), codes
+    return res, st
+
+
+def _key(r):
+    return (r.estimate, r.errorest, r.nregions, r.iters, r.status)
+
+
+def test_segments_agree_with_the_regular_path(pg, gpu):
+    items = _items(160, seed=31, max_d=7, min_eps=1e-5)
+    seg, st = _run(pg, items, SEG)
+    reg, _ = _run(pg, items, 0)
+    assert st["segment_integrals"] >= len(items) // 2, st["segment_integrals"]
+    assert st["integrals"] == len(items) and st["segment_ms"] > 0
+    same_path = 0
+    for i, (s, r) in enumerate(zip(seg, reg)):
+        assert s.status == r.status, i
+        # within the combined error estimates always
+        assert abs(s.estimate - r.estimate) <= s.errorest + r.errorest, i
+        if (s.nregions, s.iters) == (r.nregions, r.iters):
+            same_path += 1
+            # the same iterations: the totals differ by the rounding of the
+            # per-region arithmetic and the order of the sums.  A region's
+            # error is a difference of rule sums of the estimate's size, so
+            # its rounding is measured against the estimate (measured:
+            # 9e-18 on an estimate of 0.58, 8e-9 of the error itself)
+            assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
+            assert abs(s.errorest - r.errorest) <= \
+                1e-12 * abs(r.estimate) + 1e-9 * r.errorest + 1e-300, i
+    # a different path needs a decision that rounding tips over
+    assert same_path >= len(items) - 2, same_path
+
+
+def test_segments_are_deterministic(pg, gpu):
+    items = _items(96, seed=17, max_d=6, min_eps=1e-4)
+    a, sa = _run(pg, items, SEG)
+    # the other order; two workers (the regular path's items then run on two
+    # streams), with the first run's memory-model total
+    b, sb = _run(pg, items[::-1], SEG, workers=2, mem=_hbm())
+    assert sa["segment_integrals"] == sb["segment_integrals"] > 0
+    assert [_key(x) for x in a] == [_key(x) for x in b[::-1]]
+
+
+def test_items_beyond_the_segment_run_on_the_regular_path(pg, gpu):
+    # 256 regions: the default split of d >= 5 (2^d) fits, most lists
+    # outgrow it within a few iterations and go back to the regular loop
+    items = _items(48, seed=5, max_d=7, min_eps=1e-4)
+    seg, st = _run(pg, items, 256)
+    reg, _ = _run(pg, items, 0)
+    assert 0 < st["segment_integrals"] < len(items)
+    # the kernel's own items agree to rounding; every item it gave back is
+    # the regular path's, bit for bit
+    same = 0
+    for i, (s, r) in enumerate(zip(seg, reg)):
+        assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
+        assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
+        same += _key(s) == _key(r)
+    assert same >= len(items) - st["segment_integrals"]
+
+
+def test_fp32_items_skip_the_kernel(pg, gpu):
+    items = _items(12, seed=3, max_d=5, min_eps=1e-3)
+    seg, st = _run(pg, items, SEG, precision=pg.FP32)
+    reg, _ = _run(pg, items, 0, precision=pg.FP32)
+    assert st["segment_integrals"] == 0
+    assert [_key(x) for x in seg] == [_key(x) for x in reg]
+
+
+def test_segments_match_the_oracle_loop(pg, gpu):
+    """C0 and discontinuous items (no closed-form check needed) at small d:
+    the oracle's status, region count and iteration count; estimates within
+    the combined error estimates."""
+    import oracle_api as O
+    hbm = _hbm()
+    items = [it for it in _items(200, seed=23, max_d=4, min_eps=1e-5)
+             if it[0] in ("c0", "discontinuous")][:24]
+    seg, st = _run(pg, items, SEG)
+    # measured: 21 of these 24 stay within 65536 regions and the budget
+    assert st["segment_integrals"] >= len(items) * 3 // 4, st["segment_integrals"]
+    for i, ((fam, a, u, eps), r) in enumerate(zip(items, seg)):
+        o = O.integrate(len(a), fam, O.params(a, u), eps, 1e-20, total_mem=hbm)
+        assert (o.status, o.nregions, o.iters) == (r.status, r.nregions, r.iters), i
+        assert abs(r.estimate - o.estimate) <= r.errorest + o.errorest, i
+
+
+def test_segments_are_accurate(pg, gpu):
+    from gpuintegration_amd import workloads as W
+    items = [it for it in _items(120, seed=41, max_d=6, min_eps=1e-5)
+             if it[0] in ("oscillatory", "product_peak", "corner_peak", "gaussian")]
+    seg, st = _run(pg, items, SEG)
+    assert st["segment_integrals"] > 0
+    for (fam, a, u, eps), r in zip(items, seg):
+        if r.status == 0:
+            exact = W.exact(fam, a, u)
+            assert abs(r.estimate - exact) <= eps * abs(exact), (fam, len(a), eps)
+
+
+def test_segment_size_bounds(pg, gpu):
+    with pg.Batch(0, 1) as b:
+        for bad in (1, 255, (1 << 22) + 1):
+            with pytest.raises(pg.PaganiError):
+                b.set_segments(bad)
+        b.set_segments(0)
+        b.set_segments(1 << 22)
