@@ -76,6 +76,14 @@ def cpu_model():
     return "unknown"
 
 
+_T0 = time.perf_counter()
+
+
+def _note(msg):
+    """A progress line on stderr (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg} ({time.perf_counter() - _T0:.1f} s)", file=sys.stderr, flush=True)
+
+
 def _sync():
     try:
         import torch
@@ -341,8 +349,10 @@ def config2(args, pg, W):
             "traffic_vs_algorithmic": fc["split_hbm_bytes_per_region"] / (5 * d + 17.5),
         },
     }
+    _note("headline done")
     if not args.no_generic:
         out["generic"] = generic_leg(args, W, counters)
+        _note("generic leg done")
     if not args.no_mixed:
         # mixed precision beside FP64 at one stated tolerance both can certify
         eps_m = args.mixed_epsrel
@@ -362,12 +372,16 @@ def config2(args, pg, W):
                          abs_err_vs_analytic=abs(r64.estimate - exact), errorest=r64.errorest,
                          nregions=r64.nregions),
         }
+        _note("mixed leg done")
     if args.quick:
         return out
     if not args.no_mcubes:
         out["mcubes"] = mcubes_leg(args, pg, W)
+        _note("mcubes leg done")
     out["batch"] = config4(args, pg, W, leg=True)
+    _note("batch leg done")
     out["persistent_batch"] = persistent_leg(args, pg, W)
+    _note("persistent batch leg done")
     # the same batch with the fixed launch-shape rule instead of the
     # autotuner's table
     rule = config4(args, pg, W, leg=True, tuned=False)
@@ -375,10 +389,14 @@ def config2(args, pg, W):
                            "batch_integrals_per_s_rule": rule["value"],
                            "batch_eval_ms_tuned": out["batch"]["phase_ms_per_step"]["eval_ms"],
                            "batch_eval_ms_rule": rule["phase_ms_per_step"]["eval_ms"]}
+    _note("launch-shape leg done")
     out["large"] = config5(args, pg, W)
+    _note("large leg done")
     out["plain_cpu_vs_gpu"] = config1(args, pg, W)
+    _note("plain CPU/GPU leg done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(fam, a, u, args.cpu_epsrel)
+        _note("CPU baseline done")
     return out
 
 
