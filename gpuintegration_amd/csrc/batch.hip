@@ -49,13 +49,15 @@ struct pagani_batch {
   std::string error;
   // the persistent kernel (pagani_batch_set_segments): regions per segment
   // (0: off), the pool and the item arrays, kept between calls
-  std::size_t seg_regions = 0;
+  std::size_t seg_size = 0;
   int num_cus = 256;
   char* seg_pool = nullptr;
   std::size_t seg_pool_bytes = 0;
   void* seg_dev = nullptr;  // items, outputs and the counter
   std::size_t seg_dev_bytes = 0;
   std::uint64_t seg_integrals = 0;  // finished in segments (statistics)
+  std::uint64_t seg_evaluated = 0;  // regions the kernel evaluated (statistics)
+  std::uint64_t seg_iterations = 0;
   double seg_ms = 0;                // host time of the segment launches
   hipStream_t seg_stream[kSegStreams] = {};
   hipEvent_t seg_event[kSegStreams] = {};
@@ -109,6 +111,89 @@ int run_item(pagani_batch* b, int w, const pagani_batch_item& it, pagani_result*
   return rc;
 }
 
+// An item the persistent kernel handed over (its state before iteration
+// o.it, the list in the segment at `base`) continues on worker w: the list
+// and the parents' estimates are copied into the workspace's pool, the
+// loop state is held as a suspended run's, and the run resumes.
+int run_item_resume(pagani_batch* b, int w, const pagani_batch_item& it,
+                    const pagani::internal::SegOut& o, char* base, pagani_result* out)
+{
+  namespace I = pagani::internal;
+  std::memset(out, 0, sizeof(*out));
+  out->status = 1;
+  out->lastPhase = -1;
+  const int d = it.f.ndim;
+  pagani_ws*& ws = b->ws[w][d];
+  if (!ws) {
+    ws = pagani_ws_create(d, b->device);
+    if (!ws) {
+      set_error(b, std::string("workspace creation failed: ") + pagani_create_error());
+      return PAGANI_EHIP;
+    }
+  }
+  pagani_ws_set_device_mem(ws, item_total(b));
+  pagani_ws_set_mem_model(ws, b->mem_model);
+  pagani_ws_set_timing(ws, b->timing ? 1 : 0);
+  pagani_ws_set_tuned_slices(ws, b->tuned ? 1 : 0);
+  pagani_ws_set_graphs(ws, b->workers == 1 ? 1 : 0);
+  const std::size_t n = o.n, np = o.has_parents ? n / 2 : 0;
+  const I::SegView v = I::segment_view(base, d, b->seg_size);
+  ws->susp.held = false;
+  ws->live_n = 0;
+  ws->live_evaluated = false;
+  ws->snap_n = 0;
+  ws->trace.clear();
+  ws->cur = 0;
+  int rc;
+  if ((rc = I::ensure(ws, ws->L[0], d * n, "region bounds")) ||
+      (rc = I::ensure(ws, ws->H[0], d * n, "region bounds")) ||
+      (np && (rc = I::ensure(ws, ws->pest, np, "parent estimates")))) {
+    if (rc != PAGANI_ENOMEM) set_error(b, pagani_ws_last_error(ws));
+    return rc;
+  }
+  if (hipMemcpyAsync(ws->L[0].p, v.L[o.cur], d * n * sizeof(double), hipMemcpyDeviceToDevice,
+                     ws->stream) != hipSuccess ||
+      hipMemcpyAsync(ws->H[0].p, v.H[o.cur], d * n * sizeof(double), hipMemcpyDeviceToDevice,
+                     ws->stream) != hipSuccess ||
+      (np && hipMemcpyAsync(ws->pest.p, v.pest, np * sizeof(double), hipMemcpyDeviceToDevice,
+                            ws->stream) != hipSuccess)) {
+    (void)hipGetLastError();
+    set_error(b, "copy of a segment's list failed");
+    return PAGANI_EHIP;
+  }
+  I::LoopState st;
+  st.held = true;
+  st.it = o.it;
+  st.n = n;
+  st.has_parents = o.has_parents;
+  st.split = 0;
+  st.lv = pagani::phases::Levels{};
+  st.user_regions = 0;
+  st.relerr = it.relerr_classification ? 1 : 0;
+  st.epsrel = it.epsrel;
+  st.epsabs = it.epsabs;
+  for (int j = 0; j < d; ++j) {
+    st.lows[j] = 0.0;
+    st.highs[j] = 1.0;
+  }
+  st.cum = o.r;
+  st.cum.status = 1;
+  st.cum.lastPhase = -1;
+  for (int k = 0; k < 3; ++k) st.hist[k] = o.hist[k];
+  st.iters_collected = o.iters_collected;
+  st.mem_model = b->mem_model;
+  st.mem_total = item_total(b);
+  st.genz = it.f;
+  st.precision = it.precision;
+  ws->susp = st;
+  ws->live_n = n;
+  ws->split = false;
+  ws->lv = st.lv;
+  rc = pagani_integrate_genz_resume(ws, &it.f, it.precision, out);
+  if (rc < 0 && rc != PAGANI_ENOMEM) set_error(b, pagani_ws_last_error(ws));
+  return rc;
+}
+
 // Grow a device buffer (contents not kept).
 int grow(pagani_batch* b, void** p, std::size_t* cap, std::size_t need)
 {
@@ -155,11 +240,12 @@ int seg_streams(pagani_batch* b)
 // integral the others' workgroups take the freed compute units; within a
 // launch the tightest epsrel goes first.
 int run_segments(pagani_batch* b, const pagani_batch_item* items, std::size_t count,
-                 pagani_result* results, int* codes, std::vector<char>& done)
+                 pagani_result* results, int* codes, std::vector<char>& done,
+                 std::vector<pagani::internal::SegOut>& handoff, std::vector<char*>& hbase)
 {
   namespace I = pagani::internal;
   const auto t0 = std::chrono::steady_clock::now();
-  const std::size_t S = b->seg_regions;
+  const std::size_t S = b->seg_size;
   std::vector<I::SegItem> sel;
   std::vector<std::size_t> idx;
   std::vector<SegGroup> groups;
@@ -260,9 +346,21 @@ int run_segments(pagani_batch* b, const pagani_batch_item* items, std::size_t co
     set_error(b, "segment kernel failed");
     return PAGANI_EHIP;
   }
+  std::size_t gq = 0;
   for (std::size_t q = 0; q < sel.size(); ++q) {
-    if (out[q].outcome != 0) continue;
-    results[idx[q]] = out[q].r;
+    while (q >= groups[gq].first + groups[gq].count) ++gq;
+    const I::SegOut& o = out[q];
+    b->seg_evaluated += o.evaluated;
+    b->seg_iterations += o.iterations;
+    if (o.outcome != 0) {
+      if (o.n > 0) {
+        const SegGroup& g = groups[gq];
+        handoff[idx[q]] = o;
+        hbase[idx[q]] = b->seg_pool + g.pool_off + static_cast<std::size_t>(o.block) * g.bytes;
+      }
+      continue;
+    }
+    results[idx[q]] = o.r;
     codes[idx[q]] = PAGANI_OK;
     done[idx[q]] = 1;
     b->seg_integrals++;
@@ -369,7 +467,7 @@ int pagani_batch_set_segments(pagani_batch* b, size_t regions)
     b->error = "segment size outside [256, 2^22] regions";
     return PAGANI_EBADARG;
   }
-  b->seg_regions = regions;
+  b->seg_size = regions;
   return PAGANI_OK;
 }
 
@@ -413,6 +511,8 @@ int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out)
   out->segment_integrals = b->seg_integrals;
   out->segment_ms = b->seg_ms;
   out->integrals += b->seg_integrals;
+  out->regions_evaluated += b->seg_evaluated;
+  out->iterations += b->seg_iterations;
   return PAGANI_OK;
 }
 
@@ -423,6 +523,8 @@ int pagani_batch_reset_stats(pagani_batch* b)
     for (pagani_ws* w : row)
       if (w) pagani_ws_reset_stats(w);
   b->seg_integrals = 0;
+  b->seg_evaluated = 0;
+  b->seg_iterations = 0;
   b->seg_ms = 0;
   return PAGANI_OK;
 }
@@ -438,13 +540,16 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
   }
   b->error.clear();
   std::vector<char> done(count, 0);
-  if (b->seg_regions) {
+  // items the persistent kernel handed over (hbase[i] != NULL)
+  std::vector<pagani::internal::SegOut> handoff(b->seg_size ? count : 0);
+  std::vector<char*> hbase(b->seg_size ? count : 0, nullptr);
+  if (b->seg_size) {
     if (hipSetDevice(b->device) != hipSuccess) {
       (void)hipGetLastError();
       b->error = "hipSetDevice failed";
       return PAGANI_EHIP;
     }
-    const int rc = run_segments(b, items, count, results, codes, done);
+    const int rc = run_segments(b, items, count, results, codes, done, handoff, hbase);
     if (rc) return rc;
   }
   // the rest, largest first: higher dimension, then tighter epsrel, then
@@ -489,7 +594,9 @@ int pagani_integrate_batch(pagani_batch* b, const pagani_batch_item* items, size
         }
       }
       last_d = d;
-      codes[i] = run_item(b, w, items[i], &results[i]);
+      codes[i] = !hbase.empty() && hbase[i]
+                     ? run_item_resume(b, w, items[i], handoff[i], hbase[i], &results[i])
+                     : run_item(b, w, items[i], &results[i]);
     }
   };
   const int nthreads = static_cast<int>(std::min<std::size_t>(b->workers, nrest));

@@ -61,7 +61,7 @@ struct SegPtrs {
   std::uint8_t *dim, *active;
 };
 
-__device__ SegPtrs seg_ptrs(char* base, int d, std::size_t S)
+__host__ __device__ SegPtrs seg_ptrs(char* base, int d, std::size_t S)
 {
   auto al = [](std::size_t b) { return (b + 255) / 256 * 256; };
   SegPtrs p;
@@ -88,6 +88,9 @@ struct SegLoop {
   std::uint64_t n, m;
   int status;   // the result's status
   int stop;     // 0 go on, 1 finished here, 2 give the item to the regular loop
+  double hist[3];  // Heuristic_classifier::store_estimate
+  std::uint64_t iters_collected;
+  std::uint64_t evaluated, iterations;
 };
 
 template <int D, class F>
@@ -121,11 +124,16 @@ __device__ void run_item(const F& f0, const SegItem& it, const SegPtrs& p, std::
     s.iters = 0;
     s.status = 1;
     s.stop = 0;
+    s.hist[0] = s.hist[1] = s.hist[2] = 0.0;
+    s.iters_collected = 0;
+    s.evaluated = 0;
+    s.iterations = 0;
   }
   int cur = 0;
   bool parents = false;
+  int iter = 0;
   __syncthreads();
-  for (int iter = 0; iter < 700; ++iter) {
+  for (; iter < 700; ++iter) {
     // evaluation: 64-region batches, one lane per region
     const std::size_t nb = (n + 63) / 64;
     for (std::size_t b = wave; b < nb; b += kSegWaves) {
@@ -179,14 +187,23 @@ __device__ void run_item(const F& f0, const SegItem& it, const SegPtrs& p, std::
         s.status = 0;
         s.nregions += n;
         s.stop = 1;
+        s.evaluated += n;
+        s.iterations++;
       } else {
         // compute_finished_estimates, PaganiUtils.cuh:368-385
         const double fin_est = iter_est - act_est;
         const double fin_err = iter_err - act_err;
         const bool overflow = s.cum_err + fin_err > fabs(s.cum_est + iter_est) * it.epsrel;
         if (overflow || mem_ratio(n, D, mem_total, mem_model) > .1 || 2 * m > S) {
+          // the regular loop redoes this iteration from the state before it
           s.stop = 2;
         } else {
+          s.evaluated += n;
+          s.iterations++;
+          s.hist[0] = s.hist[1];
+          s.hist[1] = s.hist[2];
+          s.hist[2] = s.cum_est + iter_est;
+          s.iters_collected++;
           s.cum_est += fin_est;
           s.cum_err += fin_err;
           s.nregions += n - m;
@@ -264,6 +281,17 @@ __device__ void run_item(const F& f0, const SegItem& it, const SegPtrs& p, std::
     o.r.nregions = s.nregions;
     o.r.iters = s.iters;
     o.outcome = s.stop == 2 ? 1 : 0;
+    o.evaluated = s.evaluated;
+    o.iterations = s.iterations;
+    if (s.stop == 2) {
+      o.it = static_cast<std::uint64_t>(iter);
+      o.n = n;
+      o.has_parents = parents ? 1 : 0;
+      o.cur = cur;
+      o.block = blockIdx.x;
+      for (int k = 0; k < 3; ++k) o.hist[k] = s.hist[k];
+      o.iters_collected = s.iters_collected;
+    }
     *out = o;
   }
   __syncthreads();
@@ -367,6 +395,18 @@ unsigned resident(unsigned grid)
 }  // namespace
 
 std::size_t segment_bytes(int d, std::size_t regions) { return seg_bytes(d, regions); }
+
+SegView segment_view(char* base, int d, std::size_t regions)
+{
+  const SegPtrs p = seg_ptrs(base, d, regions);
+  SegView v;
+  for (int k = 0; k < 2; ++k) {
+    v.L[k] = p.L[k];
+    v.H[k] = p.H[k];
+  }
+  v.pest = p.pest;
+  return v;
+}
 
 hipError_t launch_segments(hipStream_t s, int d, int family, const SegItem* items,
                            std::uint32_t count, std::uint32_t* next, char* pool,

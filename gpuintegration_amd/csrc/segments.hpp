@@ -21,13 +21,32 @@ struct SegItem {
   int pad;
 };
 
-// Its result: outcome 0 when the kernel finished it, 1 when the regular
-// loop must integrate it (r is then meaningless).
+// Its result: outcome 0 when the kernel finished it (r), 1 when the
+// regular loop takes it over.  Handed over with n > 0, the loop state
+// before iteration `it`: r holds the running totals, the list of n regions
+// (parents' estimates when has_parents) is in buffer pair `cur` of the
+// segment of workgroup `block`, and hist / iters_collected are the
+// classifier's stored estimates.  n == 0: the regular loop starts afresh.
 struct SegOut {
   pagani_result r;
   int outcome;
-  int pad;
+  int has_parents;
+  int cur;
+  unsigned block;
+  std::uint64_t it, n;
+  std::uint64_t evaluated;  // regions evaluated in the kernel's finished iterations
+  std::uint64_t iterations;
+  double hist[3];
+  std::uint64_t iters_collected;
 };
+
+// The parts of one segment (device pointers), for the host's hand-over.
+struct SegView {
+  double* L[2];
+  double* H[2];
+  double* pest;
+};
+SegView segment_view(char* base, int d, std::size_t regions);
 
 // Bytes of one segment of `regions` regions at dimension d.
 std::size_t segment_bytes(int d, std::size_t regions);

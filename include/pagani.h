@@ -207,7 +207,9 @@ typedef struct pagani_stats {
   uint64_t budget_fixes;      /* iterations that took fix_error_budget_overflow */
   uint64_t graph_launches;    /* device-loop rounds replayed from a HIP graph */
   uint64_t segment_integrals; /* batch items finished by the persistent kernel
-                                 (included in integrals) */
+                                 (included in integrals; its regions and
+                                 iterations are in regions_evaluated and
+                                 iterations) */
   double segment_ms;          /* host time of the persistent kernel's launches */
 } pagani_stats;
 
@@ -455,10 +457,12 @@ int pagani_batch_set_tuned_slices(pagani_batch* b, int enable);
  * segment of `regions` regions of one pool, no host round trip per
  * iteration.  An item whose list would outgrow its segment, that needs the
  * error-budget fix (Workspace.cuh:121-146) or that comes past a tenth of its
- * memory budget (where the classifier may run) is integrated again by the
- * regular loop.  The iterations are those of pagani_integrate_genz; the
- * results agree with it to rounding (the sums are taken in another order),
- * not to the bit.  0 (the default) = off, else regions in [256, 2^22]. */
+ * memory budget (where the classifier may run) is handed over: its list and
+ * loop state move to a worker's workspace and the regular loop continues
+ * from that iteration.  The iterations are those of pagani_integrate_genz;
+ * the results agree with it to rounding (the sums are taken in another
+ * order), not to the bit.  0 (the default) = off, else regions in
+ * [256, 2^22]. */
 int pagani_batch_set_segments(pagani_batch* b, size_t regions);
 /* statistics summed over all workspaces of the batch */
 int pagani_batch_get_stats(const pagani_batch* b, pagani_stats* out);

@@ -7,8 +7,10 @@ wave per 64-region batch and the iteration's sums are reduced in another
 order, so results agree with the regular path to rounding, not to the bit.
 Checked here against the regular path (the same batch with the kernel off),
 against the oracle's loop (status, region and iteration counts) and against
-the closed forms; items the kernel hands back (list beyond the segment,
-FP32) must come out bit for bit as the regular path gives them.
+the closed forms.  An item whose list outgrows its segment is handed over
+with its loop state and continues on the regular loop from the iteration
+it stopped at; items the kernel does not take (FP32) come out bit for bit
+as the regular path gives them.
 """
 import pytest
 
@@ -83,21 +85,24 @@ def test_segments_are_deterministic(pg, gpu):
     assert [_key(x) for x in a] == [_key(x) for x in b[::-1]]
 
 
-def test_items_beyond_the_segment_run_on_the_regular_path(pg, gpu):
+def test_items_beyond_the_segment_continue_on_the_regular_path(pg, gpu):
     # 256 regions: the default split of d >= 5 (2^d) fits, most lists
-    # outgrow it within a few iterations and go back to the regular loop
+    # outgrow it within a few iterations and are handed over
     items = _items(48, seed=5, max_d=7, min_eps=1e-4)
     seg, st = _run(pg, items, 256)
-    reg, _ = _run(pg, items, 0)
+    reg, st0 = _run(pg, items, 0)
     assert 0 < st["segment_integrals"] < len(items)
-    # the kernel's own items agree to rounding; every item it gave back is
-    # the regular path's, bit for bit
-    same = 0
+    # handed over, an item continues from its state: the same iterations as
+    # the regular path, totals to rounding
     for i, (s, r) in enumerate(zip(seg, reg)):
         assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
         assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
-        same += _key(s) == _key(r)
-    assert same >= len(items) - st["segment_integrals"]
+        assert abs(s.errorest - r.errorest) <= \
+            1e-12 * abs(r.estimate) + 1e-9 * r.errorest + 1e-300, i
+    # no iteration is done twice: the kernel's finished iterations and the
+    # regular loop's after the hand-over evaluate what the regular path does
+    assert st["regions_evaluated"] == st0["regions_evaluated"]
+    assert st["iterations"] == st0["iterations"]
 
 
 def test_fp32_items_skip_the_kernel(pg, gpu):
