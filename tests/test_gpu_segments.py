@@ -149,3 +149,26 @@ def test_segment_size_bounds(pg, gpu):
                 b.set_segments(bad)
         b.set_segments(0)
         b.set_segments(1 << 22)
+
+
+@pytest.mark.parametrize("relerr,init_k", [(False, 0), (True, 3)])
+def test_segments_up_to_ten_dimensions_and_loop_options(pg, gpu, relerr, init_k):
+    """d = 2..10 (every dimension the Genz descriptors cover), the relerr
+    classification off, and an explicit initial split: the same iterations
+    as the regular path, totals to rounding."""
+    from gpuintegration_amd import workloads as W
+    items = []
+    for i, (fam, a, u, eps) in enumerate(W.batch(60, seed=77)):
+        d = 2 + i % 9
+        a = (list(a) * 4)[:d]
+        u = (list(u) * 4)[:d]
+        items.append((fam, a, u, max(eps, 1e-3 if d >= 9 else 1e-4)))
+    if init_k == 3:  # 3^d regions must fit the segment
+        items = [it for it in items if 3 ** len(it[1]) <= SEG]
+    seg, st = _run(pg, items, SEG, relerr=relerr, init_k=init_k)
+    reg, st0 = _run(pg, items, 0, relerr=relerr, init_k=init_k)
+    assert st["segment_integrals"] > 0
+    assert st["regions_evaluated"] == st0["regions_evaluated"]
+    for i, (s, r) in enumerate(zip(seg, reg)):
+        assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
+        assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
