@@ -759,17 +759,18 @@ __device__ __forceinline__ double point_generator(const pagani_eval_consts& C, i
 }
 
 // One wave's share of the evaluation of up to 64 regions, one lane per
-// region t of an n-region SoA list (split form when pdim is not NULL).
-// `wave` indexes the `slices` waves that split the batch's points (0 when
-// slices == 1); with more than one, every wave of the workgroup passes one
-// barrier.  `lows`, `range`, `jac` describe the integration box.
+// region t of an n-region SoA list (split form when pdim is not NULL), with
+// the results left in registers: avg, er and the split axis.  `wave`
+// indexes the `slices` waves that split the batch's points (0 when slices
+// == 1); with more than one, every wave of the workgroup passes one barrier,
+// and only wave 0 returns true (holding the results).  `lows`, `range`,
+// `jac` describe the integration box.
 template <int D, class F, class Real>
-__device__ __forceinline__ void eval_regions_wave(
+__device__ __forceinline__ bool eval_region_values(
     F& f, const double* __restrict__ L, const double* __restrict__ H,
-    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t, bool live,
+    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t,
     const double* lows, const double* range, double jac, const pagani_eval_consts& C, int slices,
-    int wave, int lane, double* part, double* __restrict__ est, double* __restrict__ err,
-    std::uint8_t* __restrict__ dim)
+    int wave, int lane, double* part, double& avg_out, double& er_out, int& axis_out)
 {
   double cR[D], hR[D];
   double vol;
@@ -862,7 +863,7 @@ __device__ __forceinline__ void eval_regions_wave(
             rs[r] += part[((w - 1) * PAGANI_NRULES + r) * 64 + lane];
         }
       }
-      if (wave > 0) return;
+      if (wave > 0) return false;
     }
   }
 
@@ -877,15 +878,32 @@ __device__ __forceinline__ void eval_regions_wave(
     e2 = __builtin_fmax(e2, __builtin_fabs(s[3] + C.scale[k][2] * s[2]) * C.norm[k][2]);
     e3 = __builtin_fmax(e3, __builtin_fabs(s[4] + C.scale[k][3] * s[3]) * C.norm[k][3]);
   }
-  const double avg = vol * s[0];
-  const double er =
-      vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3)
-                 ? e1
-                 : 5.0 * __builtin_fmax(__builtin_fmax(e1, e2), e3));
+  avg_out = vol * s[0];
+  er_out = vol * ((5.0 * e1 <= e2 && 5.0 * e2 <= e3)
+                      ? e1
+                      : 5.0 * __builtin_fmax(__builtin_fmax(e1, e2), e3));
+  axis_out = bisect;
+  return true;
+}
+
+// eval_region_values, stored: region t's estimate, error and split axis.
+template <int D, class F, class Real>
+__device__ __forceinline__ void eval_regions_wave(
+    F& f, const double* __restrict__ L, const double* __restrict__ H,
+    const std::uint8_t* __restrict__ pdim, std::size_t n, std::size_t t, bool live,
+    const double* lows, const double* range, double jac, const pagani_eval_consts& C, int slices,
+    int wave, int lane, double* part, double* __restrict__ est, double* __restrict__ err,
+    std::uint8_t* __restrict__ dim)
+{
+  double avg, er;
+  int axis;
+  if (!eval_region_values<D, F, Real>(f, L, H, pdim, n, t, lows, range, jac, C, slices, wave,
+                                      lane, part, avg, er, axis))
+    return;
   if (live) {
     est[t] = avg;
     err[t] = er;
-    dim[t] = static_cast<std::uint8_t>(bisect);
+    dim[t] = static_cast<std::uint8_t>(axis);
   }
 }
 
