@@ -306,9 +306,28 @@ int run_segments(pagani_batch* b, const pagani_batch_item* items, std::size_t co
       }
     }
   if (groups.empty()) return PAGANI_OK;
+  // the pool takes at most a quarter of the device: fewer workgroups per
+  // launch when the segments would need more (they then take more items
+  // each), and none at all -- every item on the regular loop -- when even
+  // that cannot be allocated
+  const std::size_t cap = b->hbm / 4;
+  if (pool > cap) {
+    const std::size_t share = cap / groups.size();
+    pool = 0;
+    for (SegGroup& g : groups) {
+      const std::size_t fit = share / g.bytes;
+      g.grid = static_cast<unsigned>(std::max<std::size_t>(1, std::min<std::size_t>(g.grid, fit)));
+      g.pool_off = pool;
+      pool += g.grid * g.bytes;
+    }
+  }
   int rc;
   if ((rc = seg_streams(b))) return rc;
-  if ((rc = grow(b, reinterpret_cast<void**>(&b->seg_pool), &b->seg_pool_bytes, pool))) return rc;
+  if (grow(b, reinterpret_cast<void**>(&b->seg_pool), &b->seg_pool_bytes, pool) != PAGANI_OK) {
+    std::lock_guard<std::mutex> g(b->mu);
+    b->error.clear();
+    return PAGANI_OK;
+  }
   // device: the items and the launches' counters (one copy in), then the
   // outputs (one copy out)
   const std::size_t ib = sel.size() * sizeof(I::SegItem);

@@ -172,3 +172,20 @@ def test_segments_up_to_ten_dimensions_and_loop_options(pg, gpu, relerr, init_k)
     for i, (s, r) in enumerate(zip(seg, reg)):
         assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
         assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
+
+
+def test_segments_larger_than_the_pool_cap(pg, gpu):
+    """Segments of 2^22 10D regions (1.4 GB each) for 64 integrals: 90 GB
+    of segments against a pool capped at a quarter of the device, so the
+    launch runs fewer workgroups that take more items each; results as with
+    the kernel off."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    items = [("gaussian", list(rng.uniform(3, 4, 10)), list(rng.uniform(0.3, 0.7, 10)), 1e-3)
+             for _ in range(64)]
+    seg, st = _run(pg, items, 1 << 22)
+    reg, st0 = _run(pg, items, 0)
+    assert st["segment_integrals"] >= len(items) // 2
+    for i, (s, r) in enumerate(zip(seg, reg)):
+        assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
+        assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
