@@ -14,9 +14,12 @@ The headline line also carries, each timed in the same run:
            integrand written as a plain functor (only operator(), the
            reference's formula): the kernel's point-by-point path
   mixed    FP32 integrand + FP64 sums beside FP64 at epsrel 1e-5
+  mixed_6d the 6D corner-peak and product-peak sweep (epsrel 1e-3..1e-7),
+           FP32 integrand beside FP64 (also --config 3)
   mcubes   the m-Cubes VEGAS integrator on the same integrand
   batch    the 512-integral Genz batch (family and d in {3..8} drawn,
-           epsrel log-uniform in [1e-6, 1e-3]) from one call
+           epsrel log-uniform in [1e-6, 1e-3]) from one call, the short
+           integrals in the persistent kernel (--segments regions each)
   persistent_batch  the batch's persistent kernel (whole integrals, one
            workgroup each) off and on, on 512 short integrals and on the
            512-integral batch
@@ -375,6 +378,8 @@ def config2(args, pg, W):
         _note("mixed leg done")
     if args.quick:
         return out
+    out["mixed_6d"] = config3(args, pg, W)
+    _note("6D mixed-precision sweep done")
     if not args.no_mcubes:
         out["mcubes"] = mcubes_leg(args, pg, W)
         _note("mcubes leg done")
@@ -709,7 +714,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3, 4, 5])
     ap.add_argument("--batch-count", type=int, default=512)
-    ap.add_argument("--segments", type=int, default=0,
+    ap.add_argument("--segments", type=int, default=65536,
                     help="the batch's persistent kernel: regions per segment (0: off)")
     ap.add_argument("--persistent-segments", type=int, default=65536,
                     help="segment size of the persistent-kernel leg")
