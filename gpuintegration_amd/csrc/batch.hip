@@ -14,6 +14,11 @@
 // but the item and that total: not on the worker, the order or the overlap.
 // The default total is the device's HBM divided by the number of workers,
 // so that the pools the memory model admits fit on the device together.
+//
+// With pagani_batch_set_segments, the short FP64 Genz items first run whole
+// in the persistent kernel (segments.hip); the ones it hands over continue
+// on the workers from the state it reached.  Their results then agree with
+// the single-integral loop's to rounding rather than to the bit.
 
 #include <hip/hip_runtime.h>
 

@@ -4,19 +4,21 @@
 //
 // A batch of many small integrals spends its time on launches and host
 // round trips, not on arithmetic: each iteration of each integral is a few
-// launches over a few thousand regions.  Here one launch per dimension runs
-// a fixed grid of workgroups that take items from a shared counter; a
-// workgroup runs an item's whole loop (Workspace.cuh:362-494) in its
-// segment -- uniform split, evaluation (its four waves take 64-region
-// batches, one lane per region, the kernel of include/pagani/eval_kernel.hpp),
-// RefineError with the relerr flags, the iteration's sums (compensated, in a
-// fixed order), the loop's decisions, filter + split -- with no host
-// involvement, and writes the result.  An item whose list would outgrow the
-// segment, or that reaches a branch the host takes (the error-budget fix,
-// Workspace.cuh:121-146, or a memory model past a tenth of its budget, where
-// the heuristic classifier may run), stops and is integrated again by the
-// regular loop.  The decisions are those of the device-resident loop
-// (devloop_decide.hpp) for the iterations it keeps.
+// launches over a few thousand regions.  Here one launch per (dimension,
+// Genz family) runs a fixed grid of workgroups that take items from a
+// shared counter; a workgroup runs an item's whole loop (Workspace.cuh:
+// 362-494) in its segment -- uniform split, evaluation (its four waves take
+// 64-region batches, one lane per region, the kernel of
+// include/pagani/eval_kernel.hpp), RefineError with the relerr flags, the
+// iteration's sums (compensated, in a fixed order), the loop's decisions,
+// filter + split -- with no host involvement, and writes the result.  An
+// item whose list would outgrow the segment, or that reaches a branch the
+// host takes (the error-budget fix, Workspace.cuh:121-146, or a memory
+// model past a tenth of its budget, where the heuristic classifier may
+// run), stops before that iteration changes its totals and writes its loop
+// state (SegOut); the batch hands the list and the state to the regular
+// loop, which continues from there.  The decisions are those of the
+// device-resident loop (devloop_decide.hpp) for the iterations it keeps.
 //
 // Results agree with a lone run of the item to rounding: the per-region
 // arithmetic is the same, but the evaluation always takes one wave per
