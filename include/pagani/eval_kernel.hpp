@@ -144,6 +144,26 @@ template <class F>
 struct has_centre_value<F, std::void_t<decltype(std::declval<const F&>().centre_value(0.0))>>
     : std::true_type {};
 
+// Exponential integrands may give the factors of the two points c -+ delta
+// (delta = g h >= 0) of one axis together: `void factor_pair(int j, double
+// c, double delta, double aux, double& f_minus, double& f_plus)`, with an
+// optional per-axis `double axis_aux(int j, double c)` computed once for
+// the five pairs of the axis; where the two factors are an exponential and
+// its reciprocal, the pair costs one exponential instead of two.
+template <class F, class = void>
+struct has_factor_pair : std::false_type {};
+template <class F>
+struct has_factor_pair<
+    F, std::void_t<decltype(std::declval<const F&>().factor_pair(
+           0, 0.0, 0.0, 0.0, std::declval<double&>(), std::declval<double&>()))>>
+    : std::true_type {};
+
+template <class F, class = void>
+struct has_axis_aux : std::false_type {};
+template <class F>
+struct has_axis_aux<F, std::void_t<decltype(std::declval<const F&>().axis_aux(0, 0.0))>>
+    : std::true_type {};
+
 template <class F, class = void>
 struct is_exponential : std::false_type {};
 template <class F>
@@ -233,6 +253,7 @@ __device__ __forceinline__ bool factored_sums(F& f, const double (&cR)[D], const
   constexpr bool kMul = is_multiplicative<F>::value || kExp;
   using Op = SepOp<kMul>;
   constexpr bool kMasked = is_masked<F>::value;
+  constexpr bool kPair = kExp && has_factor_pair<F>::value;
   const int bisect0 = bisect;
   // the centre's joined terms (exponential: the centre's value) and the
   // axes invalid at the centre (a point is zero unless it changes them all)
@@ -283,10 +304,26 @@ __device__ __forceinline__ bool factored_sums(F& f, const double (&cR)[D], const
       return Op::change(f.term(j, v), t0);
     };
     const double uj = kMasked ? (((vmask >> j) & 1u) ? 0.0 : 1.0) : 1.0;
-    const double d1p = change(__builtin_fma(-C.lambda1, hR[j], cR[j]));
-    const double d1n = change(__builtin_fma(C.lambda1, hR[j], cR[j]));
-    const double d2p = change(__builtin_fma(-C.lambda2, hR[j], cR[j]));
-    const double d2n = change(__builtin_fma(C.lambda2, hR[j], cR[j]));
+    double aux = 0.0;
+    if constexpr (kPair && has_axis_aux<F>::value) aux = f.axis_aux(j, cR[j]);
+    // the factors (term changes) of the points c - g h and c + g h
+    auto pair = [&](double g, double& fm, double& fp) {
+      const double vm = __builtin_fma(-g, hR[j], cR[j]);
+      const double vp = __builtin_fma(g, hR[j], cR[j]);
+      if constexpr (kPair) {
+        f.factor_pair(j, cR[j], g * hR[j], aux, fm, fp);
+        if constexpr (kMasked) {
+          if (!f.valid(j, vm)) fm = 0.0;
+          if (!f.valid(j, vp)) fp = 0.0;
+        }
+      } else {
+        fm = change(vm);
+        fp = change(vp);
+      }
+    };
+    double d1p, d1n, d2p, d2n;
+    pair(C.lambda1, d1p, d1n);
+    pair(C.lambda2, d2p, d2n);
     // sets 0-2 and the fourth difference, Sample.cuh:176-218
     {
       const double fa = gated(1u << j, Op::join(S0, d1p));
@@ -304,12 +341,13 @@ __device__ __forceinline__ bool factored_sums(F& f, const double (&cR)[D], const
     }
     // sets 3-4: +-lambda3, +-lambda4 on one axis
     {
-      const double s3 = change(__builtin_fma(-C.lambda3, hR[j], cR[j])) +
-                        change(__builtin_fma(C.lambda3, hR[j], cR[j]));
+      double m3, p3, m4, p4;
+      pair(C.lambda3, m3, p3);
+      const double s3 = m3 + p3;
       acc3 = __builtin_fma(E3, s3, acc3 * uj);
       E3 *= uj;
-      const double s4 = change(__builtin_fma(-C.lambda4, hR[j], cR[j])) +
-                        change(__builtin_fma(C.lambda4, hR[j], cR[j]));
+      pair(C.lambda4, m4, p4);
+      const double s4 = m4 + p4;
       acc4 = __builtin_fma(E4, s4, acc4 * uj);
       E4 *= uj;
     }
@@ -326,8 +364,11 @@ __device__ __forceinline__ bool factored_sums(F& f, const double (&cR)[D], const
       e0 *= uj;
     }
     // set 8: the 2^D corners, g_j = +-lambda5 on every axis
-    prod *= change(__builtin_fma(-C.lambda5, hR[j], cR[j])) +
-            change(__builtin_fma(C.lambda5, hR[j], cR[j]));
+    {
+      double m5, p5;
+      pair(C.lambda5, m5, p5);
+      prod *= m5 + p5;
+    }
   }
 #pragma unroll
   for (int r = 0; r < PAGANI_NRULES; ++r)

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 #include "pagani.h"
 
@@ -197,6 +198,17 @@ namespace math {
 // else in FP64, so that an FP32 integrand's centre values and factors stay
 // normal over the FP64 range of the products.
 __device__ __forceinline__ double exp_factor(double x, double) { return exp_bounded(x); }
+
+// 1 / x for finite, normal x: the hardware reciprocal refined by two
+// Newton steps (each doubles the correct bits: r' = r + r (1 - x r)).
+__device__ __forceinline__ double recip(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
 __device__ __forceinline__ double exp_factor(double x, float)
 {
   return __builtin_fabs(x) <= 80.0 ? static_cast<double>(::expf(static_cast<float>(x)))
@@ -286,6 +298,35 @@ struct C0 {
   {
     return static_cast<double>(math::exp_neg(static_cast<Real>(-s)));
   }
+  // The points v -+ delta of one axis change |v| by -+delta when they stay
+  // on v's side of the kink: factors e^delta (toward 0) and e^-delta (away),
+  // one exponential and its reciprocal.  A point across the kink (delta >
+  // |v|) has e^(2|v| - delta) = (e^-delta e^|v|) e^|v|, with aux = e^|v|
+  // (|v| <= 650 where the factored form is taken, so aux stays finite and
+  // the first product is at most 1).
+  // (FP64 integrands only: for an FP32 one the extra registers cost more
+  // than the exponentials they save)
+  template <class R = Real, std::enable_if_t<std::is_same_v<R, double>, int> = 0>
+  __device__ double axis_aux(int, double v) const
+  {
+    return math::exp_factor(__builtin_fabs(v), Real());
+  }
+  template <class R = Real, std::enable_if_t<std::is_same_v<R, double>, int> = 0>
+  __device__ void factor_pair(int, double v, double delta, double aux, double& fm,
+                              double& fp) const
+  {
+    const double e = math::exp_factor(-delta, Real());
+    const double r = math::recip(e);
+    const double across = (e * aux) * aux;
+    const bool cross = delta > __builtin_fabs(v);
+    if (v >= 0.0) {
+      fm = cross ? across : r;
+      fp = e;
+    } else {
+      fm = e;
+      fp = cross ? across : r;
+    }
+  }
   // outer(s + d) = outer(s) e^-d, non-negative terms (see Gaussian)
   static constexpr bool exponential = true;
   __device__ double factor(double d) const { return math::exp_factor(-d, Real()); }
@@ -348,6 +389,13 @@ struct Discontinuous {
   __device__ double factor(double d) const
   {
     return static_cast<double>(math::exp_bounded(static_cast<Real>(d)));
+  }
+  // the points t -+ delta change a_j t by -+a_j delta: factors e^(-a_j
+  // delta) and its reciprocal
+  __device__ void factor_pair(int j, double, double delta, double, double& fm, double& fp) const
+  {
+    fm = factor(-static_cast<double>(a[j]) * delta);
+    fp = math::recip(fm);
   }
   __device__ bool factor_safe(double s0) const
   {
