@@ -96,8 +96,9 @@ class EvalConsts(ctypes.Structure):
         ("lambda3", ctypes.c_double),
         ("lambda4", ctypes.c_double),
         ("lambda5", ctypes.c_double),
+        ("h0", ctypes.c_double * MAX_DIM),
         ("abi_version", ctypes.c_uint32),
-        ("pad", ctypes.c_uint32),
+        ("levels", ctypes.c_uint32),
     ]
 
 

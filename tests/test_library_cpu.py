@@ -138,3 +138,37 @@ def test_workload_closed_forms_match_oracle():
         want = O.exact(fam, len(a), O.params(a, u))
         got = W.exact(fam, a, u)
         assert got == pytest.approx(want, rel=1e-12, abs=1e-300), (fam, len(a))
+
+
+def test_binding_structs_match_the_header(pg, tmp_path):
+    """Every struct the Python binding passes across the C ABI has the
+    header's size and field offsets (a C program built with gcc from
+    include/pagani.h prints them)."""
+    import subprocess
+    structs = {
+        "pagani_result": pg._Result, "pagani_genz": pg._Genz,
+        "pagani_batch_item": pg._BatchItem, "pagani_eval_consts": pg.EvalConsts,
+        "pagani_stats": pg.Stats, "pagani_class_res": pg.ClassRes,
+        "pagani_mc_params": pg.McParams, "pagani_mc_result": pg.McResult,
+        "pagani_mc_stats": pg.McStats,
+    }
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "pagani.h"',
+             "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append('  printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for fname, _ in py._fields_:
+            lines.append('  printf("%s.%s %%zu\\n", offsetof(%s, %s));'
+                         % (cname, fname, cname, fname))
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(line.split() for line in
+               subprocess.run([str(exe)], capture_output=True, text=True,
+                              check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)
