@@ -189,3 +189,45 @@ def test_segments_larger_than_the_pool_cap(pg, gpu):
     for i, (s, r) in enumerate(zip(seg, reg)):
         assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
         assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
+
+
+def test_bad_and_mixed_items_with_segments(pg, gpu):
+    """With the kernel on, an item of an unknown family still fails alone
+    (EBADARG, status 1), FP32 items run on the regular loop beside FP64
+    items in the kernel, and every good item's result is the one the batch
+    gives with the kernel off, to rounding."""
+    import ctypes
+    items = _items(12, seed=9, max_d=6, min_eps=1e-3)
+    lib = pg.load()
+
+    def run(seg):
+        with pg.Batch(0, 1) as b:
+            if seg:
+                b.set_segments(seg)
+            arr = (pg._BatchItem * len(items))()
+            for i, (fam, a, u, eps) in enumerate(items):
+                arr[i].f = pg.genz(fam, a, u)
+                arr[i].epsrel = eps
+                arr[i].epsabs = 1e-20
+                arr[i].precision = pg.FP32 if i % 3 == 1 else pg.FP64
+                arr[i].relerr_classification = 1
+            arr[4].f.family = 99
+            res = (pg._Result * len(items))()
+            codes = (ctypes.c_int * len(items))()
+            rc = lib.pagani_integrate_batch(b._b, arr, len(items), res, codes)
+            st = b.stats()
+        return rc, list(codes), [res[i] for i in range(len(items))], st
+
+    rc, codes, res, st = run(SEG)
+    rc0, codes0, res0, _ = run(0)
+    assert rc == rc0 == pg.EBADARG
+    assert codes == codes0 and codes[4] == pg.EBADARG and res[4].status == 1
+    assert st["segment_integrals"] > 0
+    for i, (s, r) in enumerate(zip(res, res0)):
+        if i == 4:
+            continue
+        assert codes[i] >= 0
+        assert s.status == r.status and (s.nregions, s.iters) == (r.nregions, r.iters), i
+        assert abs(s.estimate - r.estimate) <= 1e-11 * abs(r.estimate) + 1e-300, i
+        if i % 3 == 1:  # FP32: never in the kernel, bit for bit
+            assert (s.estimate, s.errorest) == (r.estimate, r.errorest), i
