@@ -175,9 +175,11 @@ int host_probe(pagani_ws* ws, const double* err, std::size_t n, double t, double
 }
 
 // classify, :362-451.  The flags of the accepted threshold (*last_thr) are
-// written by the caller.
+// written by the caller.  have_minmax: d_scal already holds err's minimum
+// and maximum (the loop's refinement computed them).
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
-             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr)
+             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr,
+             bool have_minmax)
 {
   std::memset(&res, 0, sizeof(res));
   int rc = ensure_chunks(ws, n);
@@ -190,9 +192,12 @@ int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, dou
   in.target_error = std::fabs(c.hist[2]) * c.epsrel;
   in.max_budget = c.max_budget;
   in.max_active = c.max_active;
-  // min and max of the errors
-  HIP_TRY(ws, ph::minmax(ws->stream, n, err, ws->part.p));
-  HIP_TRY(ws, launch_finalize(ws, n, false, true));
+  // min and max of the errors: from the refinement's records when nothing
+  // has finalized since (an iteration that took the error-budget fix has)
+  if (!have_minmax) {
+    HIP_TRY(ws, ph::minmax(ws->stream, n, err, ws->part.p));
+    HIP_TRY(ws, launch_finalize(ws, n, false, true));
+  }
   SearchStep step{};
   HostCache cache;
   if (ws->device_search) {
@@ -649,6 +654,7 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
 
   std::uint64_t it = st.it;
   for (; it < 700 && n > 0; it++) {
+    ws->scal_minmax = false;  // d_scal's min / max belong to an earlier list
     if (ws->iter_limit && iters_this_call == ws->iter_limit) {
       st.it = it;
       st.n = n;
@@ -744,7 +750,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
       HIP_TRY(ws, ph::refine(ws->stream, n, ws->est.p, ws->err0.p, errs,
                              has_parents ? ws->pest.p : nullptr, epsrel, relerr_it, ws->active.p,
                              ws->part.p, ws->cnt.p));
-      HIP_TRY(ws, launch_finalize(ws, n, true, false, has_parents));
+      HIP_TRY(ws, launch_finalize(ws, n, true, true, has_parents));
+      ws->scal_minmax = true;
       record(ws, EV_REF1);
     }
     errs = has_parents ? ws->err1.p : ws->err0.p;
@@ -825,7 +832,8 @@ int integrate_impl(pagani_ws* ws, pagani_eval_fn eval, void* ctx, double epsrel,
         pagani_class_res res;
         double thr = 0.0;
         record(ws, EV_CLS0);
-        rc = classify(ws, clf, errs, n, iter_err, fin_err, cum.errorest, res, &thr);
+        rc = classify(ws, clf, errs, n, iter_err, fin_err, cum.errorest, res, &thr,
+                      ws->scal_minmax);
         if (rc) return rc;
         const bool ok = res.pass_mem && res.pass_errorest_budget;
         if (ok) {

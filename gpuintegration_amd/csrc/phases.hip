@@ -92,6 +92,7 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
   const std::size_t h = n / 2;
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
+  double mn = HUGE_VAL, mx = -HUGE_VAL;  // of the refined errors
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
@@ -109,6 +110,8 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
       active[t] = a ? 1 : 0;
       se = dd_add(se, e);
       sr = dd_add(sr, r);
+      mn = fmin(mn, r);
+      mx = fmax(mx, r);
       if (a) {
         sae = dd_add(sae, e);
         sar = dd_add(sar, r);
@@ -117,6 +120,30 @@ __device__ __forceinline__ void refine_chunk(std::size_t n, const double* __rest
     }
   }
   write_record(se, sr, sae, sar, c, part, cnt, chunk);
+  // the errors' minimum and maximum in the record's last two slots, as
+  // k_minmax writes them: the classifier's search range without a pass of
+  // its own when it runs on these errors
+  __shared__ double smn[kThreads / 64], smx[kThreads / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, off, 64));
+    mx = fmax(mx, __shfl_xor(mx, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    smn[wave] = mn;
+    smx[wave] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      mn = fmin(mn, smn[w]);
+      mx = fmax(mx, smx[w]);
+    }
+    double* p = part + chunk * kPartDoubles;
+    p[8] = mn;
+    p[9] = mx;
+  }
 }
 
 // A region's sibling lies n/2 away, in chunk c + L of a paired list's

@@ -154,6 +154,9 @@ struct pagani_ws {
   pagani::internal::SearchState* d_search = nullptr;
   pagani::internal::SearchState* h_search = nullptr;
   bool device_search = true;
+  // d_scal holds the min and max of the iteration's refined errors (set
+  // after the refinement's finalize, cleared by every other finalize)
+  bool scal_minmax = false;
   hipEvent_t ev[10] = {};
   // device-resident loop, and its full-length rounds as HIP graphs
   bool device_loop = true;
@@ -340,6 +343,7 @@ inline int ensure_chunks(pagani_ws* ws, std::size_t n)
 inline hipError_t launch_finalize(pagani_ws* ws, std::size_t n, bool counts, bool minmax,
                                   bool paired = false)
 {
+  ws->scal_minmax = false;
   return ph::finalize(ws->stream, ph::num_chunks(n, paired), ws->part.p,
                       counts ? ws->cnt.p : nullptr,
                       counts ? ws->off.p : nullptr, ws->fpart.p, ws->fcnt.p, ws->foff.p,
@@ -491,7 +495,8 @@ struct Classifier {
 };
 
 int classify(pagani_ws* ws, Classifier& c, const double* err, std::size_t n, double iter_err,
-             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr);
+             double iter_fin_err, double tot_fin_err, pagani_class_res& res, double* last_thr,
+             bool have_minmax = false);
 
 }  // namespace internal
 }  // namespace pagani
