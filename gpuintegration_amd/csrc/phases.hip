@@ -214,11 +214,19 @@ __global__ void __launch_bounds__(kThreads)
   chunk_range(blockIdx.x, n, paired != 0, base, end);
   DD se{0, 0}, sr{0, 0}, sae{0, 0}, sar{0, 0};
   std::uint32_t c = 0;
-#pragma unroll 4
+  // every load of the chunk first, so they are all in flight together
+  double ev[kPerThread], rv[kPerThread];
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
+    ev[k] = t < end ? est[t] : 0.0;
+    rv[k] = t < end ? err[t] : 0.0;
+  }
+#pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const std::size_t t = base + static_cast<std::size_t>(k) * kThreads + threadIdx.x;
     if (t < end) {
-      const double e = est[t], r = err[t];
+      const double e = ev[k], r = rv[k];
       const bool a = r > thr;
       flags[t] = a ? 1 : 0;
       se = dd_add(se, e);
