@@ -784,6 +784,19 @@ __device__ __forceinline__ void region_frame(F& f, const double* __restrict__ L,
   }
 }
 
+// The tile of workgroup b of a grid of g: workgroups are dealt to the eight
+// XCDs round robin (b to XCD b mod 8), so each XCD takes a run of
+// consecutive tiles, one after another, and the cache lines that
+// neighbouring tiles share (the level, axis and split-axis arrays hold less
+// than a line per tile) are fetched into one XCD's L2 once.  A bijection of
+// [0, g).
+__device__ __forceinline__ std::size_t xcd_tile(unsigned b, unsigned g)
+{
+  const unsigned x = b & 7u, q = g >> 3, r = g & 7u;
+  const unsigned first = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return static_cast<std::size_t>(first) + (b >> 3);
+}
+
 // Generator coordinate j of rule point p (the order of pagani_rule_point):
 // the centre, +-lambda1 and +-lambda2 per axis, then the table.
 __device__ __forceinline__ double point_generator(const pagani_eval_consts& C, int p, int j)
@@ -987,7 +1000,10 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     if (!it->split) P = nullptr;
     slices = static_cast<int>(it->slices);
   }
-  if (static_cast<std::size_t>(blockIdx.x) * 64 >= n) return;
+  // the list's tiles of 64 regions; workgroups past them return at once
+  const std::size_t tiles = (n + 63) / 64;
+  if (blockIdx.x >= tiles) return;
+  const std::size_t tile = detail::xcd_tile(blockIdx.x, static_cast<unsigned>(tiles));
   const int lane = threadIdx.x & 63;
   // wave index is uniform across the wave: keep it (and everything derived
   // from it, the slice bounds and the point loop) in scalar registers
@@ -996,20 +1012,20 @@ __global__ void __launch_bounds__(64 * PAGANI_MAX_SLICES)
     if (slices > 1) __syncthreads();
     return;
   }
-  // A flat list gives workgroup b the regions 64 b + lane.  A split list
-  // (n = 2m children, child t of parent t mod m) gives it both children of
-  // the parents 32 b .. 32 b + 31: lanes l and l + 32 read the same parent
-  // row in one load instruction, so each row leaves HBM once, and the writes
-  // are two runs of 32 consecutive regions.
+  // A flat list gives tile b the regions 64 b + lane.  A split list (n = 2m
+  // children, child t of parent t mod m) gives it both children of the
+  // parents 32 b .. 32 b + 31: lanes l and l + 32 read the same parent row
+  // in one load instruction, so each row leaves HBM once, and the writes are
+  // two runs of 32 consecutive regions.
   std::size_t t;
   bool live;
   if (P) {
     const std::size_t m = n / 2;
-    const std::size_t p0 = static_cast<std::size_t>(blockIdx.x) * 32 + (lane & 31);
+    const std::size_t p0 = tile * 32 + (lane & 31);
     live = p0 < m;
     t = (live ? p0 : m - 1) + (lane >= 32 ? m : 0);
   } else {
-    const std::size_t t0 = static_cast<std::size_t>(blockIdx.x) * 64 + lane;
+    const std::size_t t0 = tile * 64 + lane;
     live = t0 < n;
     t = live ? t0 : n - 1;
   }
