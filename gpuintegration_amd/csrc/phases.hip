@@ -932,13 +932,17 @@ __device__ __forceinline__ void compact_parents_chunk(
     double* __restrict__ pest, double* __restrict__ perr, std::size_t chunk, bool paired,
     const Levels& lv)
 {
-  // Every step's positions first (ballots per step and wave, one barrier),
-  // then the moves, with no barrier between the steps, so the row reads of
-  // all steps can be in flight together.  Positions follow the steps in
-  // order, so the list is the one a step-by-step scan gives.
+  // The positions first (ballots per step and wave, one barrier): each kept
+  // region's offset in the chunk, in step order, so the list is the one a
+  // step-by-step scan gives.  Each kept region leaves its index in LDS at its
+  // position, and after a second barrier every thread moves the regions at
+  // positions tid, tid + 1024, ...: all lanes busy on kept regions and the
+  // writes dense, whatever share of the chunk is kept.
   constexpr int kSteps = static_cast<int>(kChunk / kCompactThreads);
   constexpr int kWaves = kCompactThreads / 64;
   __shared__ std::uint32_t wsum[kSteps][kWaves];
+  static_assert(kChunk <= 65536, "chunk offsets are 16-bit");
+  __shared__ std::uint16_t src[kChunk];
   std::size_t base, end;
   chunk_range(chunk, n, paired, base, end);
   const std::size_t h = n / 2;
@@ -955,28 +959,29 @@ __device__ __forceinline__ void compact_parents_chunk(
     if (lane == 0) wsum[k][wave] = static_cast<std::uint32_t>(__popcll(ballot));
   }
   __syncthreads();
-  std::uint64_t idx[kSteps];
-  {
-    std::uint64_t run = off[chunk] + foff[chunk / kFold];
+  std::uint32_t kept = 0;  // the chunk's kept regions
 #pragma unroll
-    for (int k = 0; k < kSteps; ++k) {
-      std::uint32_t woff = 0, tot = 0;
-#pragma unroll
-      for (int w = 0; w < kWaves; ++w) {
-        const std::uint32_t c = wsum[k][w];
-        if (w < wave) woff += c;
-        tot += c;
-      }
-      idx[k] = run + woff + pre[k];
-      run += tot;
-    }
-  }
-  // one step's moves at a time: 46 VGPRs, two workgroups per CU (unrolled,
-  // 101 VGPRs and one workgroup per CU measured 3 % slower)
-#pragma unroll 1
   for (int k = 0; k < kSteps; ++k) {
-    if (!a[k] || idx[k] >= m) continue;
-    const std::size_t t = base + static_cast<std::size_t>(k) * kCompactThreads + threadIdx.x;
+    std::uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      const std::uint32_t c = wsum[k][w];
+      if (w < wave) woff += c;
+      tot += c;
+    }
+    if (a[k])
+      src[kept + woff + pre[k]] =
+          static_cast<std::uint16_t>(k * kCompactThreads + static_cast<int>(threadIdx.x));
+    kept += tot;
+  }
+  __syncthreads();
+  const std::uint64_t first = off[chunk] + foff[chunk / kFold];
+  // one region's move at a time: few VGPRs, two workgroups per CU
+#pragma unroll 1
+  for (std::uint32_t q = threadIdx.x; q < kept; q += kCompactThreads) {
+    const std::size_t i = first + q;
+    if (i >= m) break;
+    const std::size_t t = base + src[q];
     const double ev = est[t];
     const double rv = perr ? err[t] : 0.0;
     const int b = dim[t];
@@ -991,7 +996,6 @@ __device__ __forceinline__ void compact_parents_chunk(
       stride = h;
       axis = pdim[row];
     }
-    const std::size_t i = idx[k];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       double l, w;
